@@ -1,0 +1,115 @@
+/*
+ * pcx_gpu.h -- C ABI of the MI355X (gfx950) FFT and overlap-save cross-correlation path.
+ *
+ * This is the device boundary of the packed library's FFT units.  Each entry point replaces the
+ * body of one reference member function; the host-side C++ classes in
+ * packed_amd/include/pcx_gpu/fft.hpp keep the reference's template signatures and call these.
+ * Reference citations are relative to the packed source tree.
+ *
+ * Data layout: the packed split-complex layout of vector.hpp.  With pack size P (a power of two),
+ * the real part of complex i lives at float offset i + (i / P) * P and its imaginary part P floats
+ * later (element_access.hpp:8-11).  P = 1 is std::complex<float> interleaving, P = 8 the
+ * pcxo::vector<float> default, P >= N fully split re/im arrays.
+ * All data pointers are device (or host-pinned) memory; `stream` is a hipStream_t (NULL = the
+ * legacy default stream).  Calls are asynchronous on `stream`.
+ *
+ * Errors: every call returns a pcx_status; pcx_last_error() returns the message of the last
+ * failure on the calling thread, worded like the reference's std::invalid_argument messages.
+ */
+#ifndef PCX_GPU_H
+#define PCX_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    PCX_SUCCESS              = 0,
+    PCX_ERR_INVALID_SIZE     = 1, /* fft.hpp:1806-1812 check_size */
+    PCX_ERR_INVALID_SUB_SIZE = 2, /* fft.hpp:1814-1826 check_sub_size */
+    PCX_ERR_SIZE_MISMATCH    = 3, /* fft.hpp:732-738, 764-777, 832-837, 868-881 */
+    PCX_ERR_INPLACE_REPACK   = 4, /* fft.hpp:785-791, 889-897 */
+    PCX_ERR_INVALID_PACK     = 5, /* pack sizes are powers of two (types.hpp pack_size concept) */
+    PCX_ERR_INVALID_ARGUMENT = 6, /* null handle / pointer, bad enum, negative count */
+    PCX_ERR_UNSUPPORTED      = 7, /* a size or mode this build does not implement */
+    PCX_ERR_HIP              = 8, /* a HIP runtime call failed */
+    PCX_ERR_OUT_OF_MEMORY    = 9
+} pcx_status;
+
+/* fft.hpp:540-544 */
+typedef enum { PCX_ORDER_NORMAL = 0, PCX_ORDER_BIT_REVERSED = 1, PCX_ORDER_UNORDERED = 2 } pcx_order;
+
+/* Arithmetic of the transform: plain FP32, or split-precision MFMA stages. */
+typedef enum { PCX_PREC_FP32 = 0, PCX_PREC_BF16X3 = 1, PCX_PREC_FP16X2 = 2 } pcx_precision;
+
+typedef struct pcx_plan_s*   pcx_plan_t;
+typedef struct pcx_fxcorr_s* pcx_fxcorr_t;
+
+int         pcx_version(void);
+const char* pcx_last_error(void);
+
+/* ------------------------------------------------------------------ FFT unit
+ * Replaces fft_unit<float, Order, Allocator>::fft_unit(fft_size, sub_size, allocator)
+ * (fft.hpp:600-605): validates fft_size (power of two > 1) and sub_size (power of two >= 8) with
+ * the reference's messages and builds the device twiddle tables.  sub_size is validated for API
+ * compatibility only; the GPU schedule is chosen by the plan. */
+int     pcx_plan_create(int64_t fft_size, int64_t sub_size, int order, pcx_plan_t* plan);
+int     pcx_plan_destroy(pcx_plan_t plan);
+/* fft.hpp:620-622 size() */
+int64_t pcx_plan_size(pcx_plan_t plan);
+int     pcx_plan_order(pcx_plan_t plan);
+
+/* Forward FFT.  Replaces fft_raw<PD,PS>(dst, src[, src_size]) and operator()(dst, src)
+ * (fft.hpp:624-687, 728-820).  In place when dst == src.  src_size <= fft_size values are read
+ * and zero-extended.  The spectrum is stored in the plan's order; an unordered spectrum uses pack
+ * max(dst_pack, 8) (avx2_fft.hpp:209-213). */
+int pcx_fft(pcx_plan_t plan, float* dst, int64_t dst_pack, const float* src, int64_t src_pack,
+            int64_t src_size, void* stream);
+
+/* Inverse FFT.  Replaces ifft_raw<Normalized,PD,PS>(dst, src) and ifft<Normalized>(dst, src)
+ * (fft.hpp:689-721, 828-915).  src holds a spectrum in the plan's order (unordered: pack
+ * max(src_pack, 8), avx2_fft.hpp:227); output is natural order, scaled by 1/N if normalized. */
+int pcx_ifft(pcx_plan_t plan, int normalized, float* dst, int64_t dst_pack, const float* src,
+             int64_t src_pack, void* stream);
+
+/* Batched FFT over `batch` vectors: vector b at src + b*src_dist / dst + b*dst_dist floats.
+ * Batched form of the two calls above (the reference runs one fft_unit call per vector; its
+ * fft_unit_par, fft.hpp:2168-2572, is not followed: its output order is wrong). */
+int pcx_fft_batch(pcx_plan_t plan, int inverse, int normalized, float* dst, int64_t dst_pack,
+                  int64_t dst_dist, const float* src, int64_t src_pack, int64_t src_dist,
+                  int64_t src_size, int64_t batch, void* stream);
+
+/* ------------------------------------------------------------------ cross-correlation
+ * Replaces fxcorr_unit<float>(g, fft_size) (fxcorr.hpp:62-69): spectrum of g (device, pack
+ * g_pack, g_size taps) zero-extended to fft_size, scaled by 1/fft_size. */
+int pcx_fxcorr_create(const float* g, int64_t g_pack, int64_t g_size, int64_t fft_size, void* stream,
+                      pcx_fxcorr_t* xc);
+int pcx_fxcorr_destroy(pcx_fxcorr_t xc);
+/* Replaces fxcorr_unit::operator()(vector&) (fxcorr.hpp:102-136) when dst == src, and the
+ * out-of-place overload (fxcorr.hpp:138-140, an empty stub in the reference) otherwise:
+ * out[i] = sum_{m < g_size} f[i+m] * conj(g[m]), i < len, f zero beyond len, by overlap-save with
+ * block step floor((fft_size - g_size + 1) / src_pack) * src_pack (at least 1). */
+int pcx_fxcorr_exec(pcx_fxcorr_t xc, float* dst, int64_t dst_pack, const float* src, int64_t src_pack,
+                    int64_t len, void* stream);
+int64_t pcx_fxcorr_step(pcx_fxcorr_t xc, int64_t src_pack);
+
+/* ------------------------------------------------------------------ memory and utilities
+ * The reference's Allocator template parameter (allocators.hpp:11-48) decides where buffers
+ * live; these back the C++ shim's device allocator. */
+int pcx_malloc(void** ptr, size_t bytes);
+int pcx_free(void* ptr);
+int pcx_memcpy_htod(void* dst, const void* src, size_t bytes);
+int pcx_memcpy_dtoh(void* dst, const void* src, size_t bytes);
+int pcx_memset(void* dst, int value, size_t bytes);
+int pcx_synchronize(void* stream);
+/* Seeded synthetic input written straight into HBM: splitmix64 outputs 2i and 2i+1 give
+ * uniform [-1,1) re/im (dist 0) or Box-Muller normal(0, sigma^2) re/im (dist 1). */
+int pcx_generate(float* dst, int64_t pack, int64_t n, uint64_t seed, int dist, double sigma, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,109 @@
+// fxcorr_kernels.hpp -- fused overlap-save cross-correlation for gfx950.
+//
+// One workgroup row per block of the signal (fxcorr.hpp:102-136 runs these blocks one after
+// another; here they are independent rows of one launch).  Per block:
+//   load N samples (zero beyond the signal) -> forward Stockham FFT -> in registers,
+//   Y = conj(X) * K with K = FFT(g)/N -> forward Stockham FFT of Y with the radix order
+//   reversed -> conj -> store the first `step` outputs.
+// conj(FFT(conj(X * conj(K)))) is the unnormalised inverse of X * conj(K), so the pointwise
+// conj-multiply is fused between the last forward stage and the first inverse stage: the
+// forward FFT ends with the small radix and the reversed plan starts with it, so both use the
+// same register-to-element mapping and no LDS round trip sits between them.
+#pragma once
+#include "fft_kernels.hpp"
+
+namespace pcxk {
+
+struct XcArgs {
+    const float* src;   // signal, pack src_lp
+    float*       dst;   // output, pack dst_lp (may equal src: then halo holds the overlap)
+    const float* hre;   // optional saved overlap: block b's samples [step, N) at b*hlen
+    const float* him;
+    const float* kre;   // kernel spectrum FFT(g)/N, natural order, split
+    const float* kim;
+    const float2* twA;  // W_N
+    int64_t      len;   // signal length L
+    int64_t      step;  // outputs kept per block
+    int64_t      nblocks;
+    int64_t      hlen;  // N - step
+    int          src_lp, dst_lp;
+};
+
+template<int LA, int G>
+__global__ __launch_bounds__(G * Plan<LA>::TA) void k_fxcorr(XcArgs p) {
+    using PF           = Plan<LA, false>;
+    using PI           = Plan<LA, true>;
+    constexpr int TA   = PF::TA;
+    constexpr int PPT  = PF::PPT;
+    constexpr int R0   = PF::first_radix;
+    constexpr int RL   = PF::last_radix;
+    constexpr int RLI  = PI::last_radix;
+    constexpr int STR  = RowShape<LA, G>::STRIDE;
+    static_assert(PI::first_radix == RL, "fused multiply needs matching register maps");
+    __shared__ float lre[G * STR];
+    __shared__ float lim[G * STR];
+
+    const int     t    = threadIdx.x;
+    const int     g    = t / TA;
+    const int     jt   = t % TA;
+    const int64_t b    = (int64_t)blockIdx.x * G + g;
+    const bool    ok   = b < p.nblocks;
+    const int64_t off  = b * p.step;
+    auto          addr = [&](int a) { return g * STR + a + (a >> 5); };
+
+    cf v[PPT];
+#pragma unroll
+    for (int m = 0; m < PPT / R0; ++m)
+#pragma unroll
+        for (int r = 0; r < R0; ++r) {
+            int     a = in_elem<LA>(jt, m, r);
+            int64_t i = off + a;
+            cf      x = {0.f, 0.f};
+            if (ok && i < p.len) {
+                if (p.hre && a >= p.step) {
+                    int64_t h = b * p.hlen + (a - p.step);
+                    x         = {p.hre[h], p.him[h]};
+                } else {
+                    x = ld_cx(p.src, i, p.src_lp);
+                }
+            }
+            v[m * R0 + r] = x;
+        }
+    stockham<LA, false>(v, lre, lim, jt, addr, p.twA);
+#pragma unroll
+    for (int m = 0; m < PPT / RL; ++m)
+#pragma unroll
+        for (int r = 0; r < RL; ++r) {
+            int e         = out_elem<LA, false>(jt, m, r);
+            cf  k         = {p.kre[e], p.kim[e]};
+            v[m * RL + r] = cmul(cconj(v[m * RL + r]), k);
+        }
+    stockham<LA, true>(v, lre, lim, jt, addr, p.twA);
+    if (ok) {
+        int64_t keep = p.len - off < p.step ? p.len - off : p.step;
+#pragma unroll
+        for (int m = 0; m < PPT / RLI; ++m)
+#pragma unroll
+            for (int r = 0; r < RLI; ++r) {
+                int a = out_elem<LA, true>(jt, m, r);
+                if (a < keep)
+                    st_cx(p.dst, off + a, p.dst_lp, cconj(v[m * RLI + r]));
+            }
+    }
+}
+
+// Saves, for every block b, the samples [b*step + step, b*step + N) that later blocks overwrite
+// when the correlation runs in place.
+__global__ void k_fxcorr_halo(const float* src, int src_lp, int64_t len, int64_t step, int64_t hlen,
+                              int64_t nblocks, float* hre, float* him) {
+    int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nblocks * hlen)
+        return;
+    int64_t b = idx / hlen, h = idx % hlen;
+    int64_t i = b * step + step + h;
+    cf      x = i < len ? ld_cx(src, i, src_lp) : cf{0.f, 0.f};
+    hre[idx]  = x.x;
+    him[idx]  = x.y;
+}
+
+} // namespace pcxk

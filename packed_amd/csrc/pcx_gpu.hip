@@ -1,0 +1,567 @@
+// pcx_gpu.hip -- C ABI of the gfx950 FFT / overlap-save cross-correlation path (include/pcx_gpu.h).
+//
+// Plans own device twiddle tables and, for N > 16384, a scratch buffer for the four-step
+// intermediate.  Everything is launched asynchronously on the caller's stream.
+#include "pcx_gpu.h"
+
+#include "fft_kernels.hpp"
+#include "fxcorr_kernels.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <array>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <utility>
+
+using namespace pcxk;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, std::string msg) {
+    g_err = std::move(msg);
+    return code;
+}
+
+#define HIPCHK(x)                                                                                  \
+    do {                                                                                           \
+        hipError_t e_ = (x);                                                                       \
+        if (e_ != hipSuccess)                                                                      \
+            return fail(PCX_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_));              \
+    } while (0)
+
+constexpr int kMaxSinglePass = 14; // N <= 16384 runs in one LDS-resident pass
+constexpr int kMaxLog2       = 24; // four-step limit: N2 <= 1024 columns, N1 <= 16384
+
+bool is_pow2(int64_t v) { return v > 0 && (v & (v - 1)) == 0; }
+int  ilog2(int64_t v) {
+    int l = 0;
+    while (((int64_t)1 << l) < v)
+        ++l;
+    return l;
+}
+
+std::string s(int64_t v) { return std::to_string(v); }
+
+// fft.hpp:1806-1812
+int check_size(int64_t n) {
+    if (n > 1 && is_pow2(n))
+        return PCX_SUCCESS;
+    return fail(PCX_ERR_INVALID_SIZE, "fft_size (which is  " + s(n) + ") is not an integer power of two");
+}
+// fft.hpp:1814-1826
+int check_sub_size(int64_t sub) {
+    if (sub >= 8 && is_pow2(sub))
+        return PCX_SUCCESS;
+    if (sub < 8)
+        return fail(PCX_ERR_INVALID_SUB_SIZE,
+                    "fft_sub_size (which is  " + s(sub) + ") is smaller than minimum (which is 8)");
+    return fail(PCX_ERR_INVALID_SUB_SIZE, "fft_sub_size (which is  " + s(sub) + ") is not an integer power of two");
+}
+int check_pack(int64_t p, const char* what) {
+    if (is_pow2(p) && p <= ((int64_t)1 << 40))
+        return PCX_SUCCESS;
+    return fail(PCX_ERR_INVALID_PACK, std::string(what) + " pack size (which is " + s(p) + ") is not a power of two");
+}
+
+hipError_t gen_twiddles(float2* t, int64_t M, int64_t stride, int64_t count) {
+    int64_t nb = (count + 255) / 256;
+    hipLaunchKernelGGL(k_twiddles, dim3((unsigned)nb), dim3(256), 0, 0, t, M, stride, count);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- kernel dispatch tables
+using RowLaunch = void (*)(const RowArgs&, int64_t nrows, hipStream_t);
+using ColLaunch = void (*)(const ColArgs&, hipStream_t);
+using XcLaunch  = void (*)(const XcArgs&, hipStream_t);
+
+template<int LA, int G>
+void launch_row(const RowArgs& a, int64_t nrows, hipStream_t st) {
+    int64_t nwg = (nrows + G - 1) / G;
+    hipLaunchKernelGGL((k_row<LA, G>), dim3((unsigned)nwg), dim3(G * Plan<LA>::TA), 0, st, a);
+}
+template<int LA, int C>
+void launch_col(const ColArgs& a, hipStream_t st) {
+    hipLaunchKernelGGL((k_col<LA, C>), dim3((unsigned)a.ntiles), dim3(C * Plan<LA>::TA), 0, st, a);
+}
+template<int LA>
+void launch_xc(const XcArgs& a, hipStream_t st) {
+    constexpr int G   = default_rows<LA>();
+    int64_t       nwg = (a.nblocks + G - 1) / G;
+    hipLaunchKernelGGL((k_fxcorr<LA, G>), dim3((unsigned)nwg), dim3(G * Plan<LA>::TA), 0, st, a);
+}
+
+template<int... L>
+constexpr auto row_table(std::integer_sequence<int, L...>) {
+    return std::array<RowLaunch, sizeof...(L)>{&launch_row<L + 1, default_rows<L + 1>()>...};
+}
+template<int... L>
+constexpr auto xc_table(std::integer_sequence<int, L...>) {
+    return std::array<XcLaunch, sizeof...(L)>{&launch_xc<L + 1>...};
+}
+// index LA-1 for LA in [1, 14]
+const auto kRow = row_table(std::make_integer_sequence<int, kMaxSinglePass>{});
+const auto kXc  = xc_table(std::make_integer_sequence<int, kMaxSinglePass>{});
+
+// Pass-B / pass-1 rows of the four-step (N1 = 2^LA): 32 rows per workgroup where the workgroup
+// stays within 1024 threads so strided stores write 128-byte runs.
+RowLaunch row_pass_launch(int la) {
+    switch (la) {
+    case 7: return &launch_row<7, 32>;
+    case 8: return &launch_row<8, 32>;
+    case 9: return &launch_row<9, 32>;
+    case 10: return &launch_row<10, 16>;
+    case 11: return &launch_row<11, 8>;
+    case 12: return &launch_row<12, 4>;
+    case 13: return &launch_row<13, 2>;
+    case 14: return &launch_row<14, 1>;
+    default: return nullptr;
+    }
+}
+// Column passes (N2 = 2^LA rows per column, C columns per tile, C * N2 <= 16384 for LDS).
+ColLaunch col_pass_launch(int la, int* C) {
+    switch (la) {
+    case 7: *C = 32; return &launch_col<7, 32>;
+    case 8: *C = 32; return &launch_col<8, 32>;
+    case 9: *C = 32; return &launch_col<9, 32>;
+    case 10: *C = 16; return &launch_col<10, 16>;
+    default: return nullptr;
+    }
+}
+
+} // namespace
+
+// ---------------------------------------------------------------- plan
+struct pcx_plan_s {
+    int64_t n     = 0;
+    int     lg    = 0;
+    int     order = 0;
+    bool    two_pass = false;
+    int     n1_lg = 0, n2_lg = 0; // N = N1 * N2, N1 contiguous (row) length, N2 column length
+    float2* tw_row = nullptr;     // W_N (single pass) or W_N1
+    float2* tw_col = nullptr;     // W_N2
+    float2* twl    = nullptr;     // W_N^e, e < 2^tw_ls
+    float2* twh    = nullptr;     // W_N^(e << tw_ls)
+    int     tw_ls  = 0;
+    float*  scratch      = nullptr;
+    int64_t scratch_cap  = 0; // transforms the scratch holds
+    int64_t chunk        = 0; // transforms per four-step chunk
+    std::mutex mu;
+};
+
+extern "C" int pcx_version(void) { return 1; }
+extern "C" const char* pcx_last_error(void) { return g_err.c_str(); }
+
+extern "C" int pcx_plan_create(int64_t fft_size, int64_t sub_size, int order, pcx_plan_t* out) {
+    if (!out)
+        return fail(PCX_ERR_INVALID_ARGUMENT, "plan output pointer is null");
+    *out = nullptr;
+    if (int e = check_size(fft_size))
+        return e;
+    if (int e = check_sub_size(sub_size))
+        return e;
+    if (order < 0 || order > 2)
+        return fail(PCX_ERR_INVALID_ARGUMENT, "order (which is " + s(order) + ") is not a fft_order");
+    int lg = ilog2(fft_size);
+    if (lg > kMaxLog2)
+        return fail(PCX_ERR_UNSUPPORTED, "fft_size (which is " + s(fft_size) + ") exceeds the supported maximum 2^24");
+    if (order == PCX_ORDER_UNORDERED && lg < 5)
+        return fail(PCX_ERR_UNSUPPORTED,
+                    "unordered fft_size (which is " + s(fft_size) + ") is below the 32-point finisher size");
+    auto* p  = new pcx_plan_s;
+    p->n     = fft_size;
+    p->lg    = lg;
+    p->order = order;
+    hipError_t he = hipSuccess;
+    if (lg <= kMaxSinglePass) {
+        he = hipMalloc(&p->tw_row, sizeof(float2) * fft_size);
+        if (he == hipSuccess)
+            he = gen_twiddles(p->tw_row, fft_size, 1, fft_size);
+    } else {
+        p->two_pass = true;
+        p->n2_lg    = lg / 2 > 10 ? 10 : lg / 2;
+        p->n1_lg    = lg - p->n2_lg;
+        p->tw_ls    = (lg + 1) / 2;
+        int64_t n1 = (int64_t)1 << p->n1_lg, n2 = (int64_t)1 << p->n2_lg;
+        int64_t nl = (int64_t)1 << p->tw_ls, nh = fft_size >> p->tw_ls;
+        he = hipMalloc(&p->tw_row, sizeof(float2) * n1);
+        if (he == hipSuccess)
+            he = hipMalloc(&p->tw_col, sizeof(float2) * n2);
+        if (he == hipSuccess)
+            he = hipMalloc(&p->twl, sizeof(float2) * nl);
+        if (he == hipSuccess)
+            he = hipMalloc(&p->twh, sizeof(float2) * nh);
+        if (he == hipSuccess)
+            he = gen_twiddles(p->tw_row, n1, 1, n1);
+        if (he == hipSuccess)
+            he = gen_twiddles(p->tw_col, n2, 1, n2);
+        if (he == hipSuccess)
+            he = gen_twiddles(p->twl, fft_size, 1, nl);
+        if (he == hipSuccess)
+            he = gen_twiddles(p->twh, fft_size, nl, nh);
+        // chunk of transforms whose intermediate stays cache-resident between the two passes
+        int64_t chunk_bytes = (int64_t)64 << 20;
+        if (const char* env = std::getenv("PCX_CHUNK_MB"))
+            chunk_bytes = (int64_t)std::atoll(env) << 20;
+        p->chunk = chunk_bytes / (fft_size * 8);
+        if (p->chunk < 1)
+            p->chunk = 1;
+    }
+    if (he == hipSuccess)
+        he = hipDeviceSynchronize();
+    if (he != hipSuccess) {
+        pcx_plan_destroy(p);
+        return fail(PCX_ERR_HIP, std::string("plan setup: ") + hipGetErrorString(he));
+    }
+    *out = p;
+    return PCX_SUCCESS;
+}
+
+extern "C" int pcx_plan_destroy(pcx_plan_t p) {
+    if (!p)
+        return PCX_SUCCESS;
+    (void)hipFree(p->tw_row);
+    (void)hipFree(p->tw_col);
+    (void)hipFree(p->twl);
+    (void)hipFree(p->twh);
+    (void)hipFree(p->scratch);
+    delete p;
+    return PCX_SUCCESS;
+}
+
+extern "C" int64_t pcx_plan_size(pcx_plan_t p) { return p ? p->n : 0; }
+extern "C" int     pcx_plan_order(pcx_plan_t p) { return p ? p->order : -1; }
+
+namespace {
+
+int run_batch(pcx_plan_t p, int inverse, int normalized, float* dst, int64_t dst_pack, int64_t dst_dist,
+              const float* src, int64_t src_pack, int64_t src_dist, int64_t src_size, int64_t batch,
+              hipStream_t st) {
+    if (!p)
+        return fail(PCX_ERR_INVALID_ARGUMENT, "plan is null");
+    if (!dst || !src)
+        return fail(PCX_ERR_INVALID_ARGUMENT, "data pointer is null");
+    if (batch < 0)
+        return fail(PCX_ERR_INVALID_ARGUMENT, "batch (which is " + s(batch) + ") is negative");
+    if (int e = check_pack(dst_pack, "destination"))
+        return e;
+    if (int e = check_pack(src_pack, "source"))
+        return e;
+    const int64_t n = p->n;
+    if (src_size > n)
+        return fail(PCX_ERR_SIZE_MISMATCH,
+                    "source size (which is " + s(src_size) + ") is bigger than fft size (which is " + s(n) + ")");
+    if (src_size < 0)
+        return fail(PCX_ERR_SIZE_MISMATCH, "source size (which is " + s(src_size) + ") is negative");
+    // spectra in unordered order keep pack max(P, 8) (avx2_fft.hpp:209-213, 227)
+    if (p->order == PCX_ORDER_UNORDERED) {
+        if (inverse && src_pack < 8)
+            src_pack = 8;
+        if (!inverse && dst_pack < 8)
+            dst_pack = 8;
+    }
+    if ((const float*)dst == src && dst_pack != src_pack && (dst_pack > 8 || src_pack > 8))
+        return fail(PCX_ERR_INPLACE_REPACK, "cannot perform in-place repack from source pack size (which is " +
+                                                s(src_pack) + ") to destination pack size (which is " + s(dst_pack) + ")");
+    if (batch == 0)
+        return PCX_SUCCESS;
+    const int   src_lp = ilog2(src_pack), dst_lp = ilog2(dst_pack);
+    const float scale  = (inverse && normalized) ? (float)(1.0 / (double)n) : 1.0f;
+    const bool  permuted = p->order != PCX_ORDER_NORMAL;
+    if (inverse)
+        src_size = n;
+
+    if (!p->two_pass) {
+        RowArgs a{};
+        a.src = src, a.dst = dst, a.src_dist = src_dist, a.dst_dist = dst_dist;
+        a.src_size = src_size, a.nrows = batch, a.src_lp = src_lp, a.dst_lp = dst_lp, a.rows_lg = 0;
+        a.load_kind  = (inverse && permuted) ? LD_PERM : LD_CONTIG;
+        a.store_kind = (!inverse && permuted) ? ST_PERM : ST_CONTIG;
+        a.order = p->order, a.conj_in = inverse, a.conj_out = inverse, a.scale = scale;
+        a.twA = p->tw_row, a.twl = nullptr, a.twh = nullptr, a.tw_ls = 0;
+        kRow[p->lg - 1](a, batch, st);
+        HIPCHK(hipGetLastError());
+        return PCX_SUCCESS;
+    }
+
+    // ---- four-step: N = N1 * N2 through a plan-owned scratch, `chunk` transforms at a time
+    std::lock_guard<std::mutex> lock(p->mu);
+    int64_t chunk = batch < p->chunk ? batch : p->chunk;
+    if (p->scratch_cap < chunk) {
+        if (p->scratch) {
+            HIPCHK(hipStreamSynchronize(st));
+            HIPCHK(hipFree(p->scratch));
+            p->scratch = nullptr;
+        }
+        hipError_t he = hipMalloc(&p->scratch, sizeof(float) * 2 * n * chunk);
+        if (he != hipSuccess)
+            return fail(PCX_ERR_OUT_OF_MEMORY, "scratch allocation of " + s(8 * n * chunk) + " bytes failed");
+        p->scratch_cap = chunk;
+    }
+    int       C    = 0;
+    ColLaunch colf = col_pass_launch(p->n2_lg, &C);
+    RowLaunch rowf = row_pass_launch(p->n1_lg);
+    if (!colf || !rowf)
+        return fail(PCX_ERR_UNSUPPORTED, "no four-step kernels for fft_size " + s(n));
+    const int64_t n1 = (int64_t)1 << p->n1_lg, n2 = (int64_t)1 << p->n2_lg;
+    for (int64_t b0 = 0; b0 < batch; b0 += chunk) {
+        int64_t      cb = batch - b0 < chunk ? batch - b0 : chunk;
+        const float* cs = src + b0 * src_dist;
+        float*       cd = dst + b0 * dst_dist;
+        ColArgs      ca{};
+        RowArgs      ra{};
+        ca.twA = p->tw_col, ca.n1_lg = p->n1_lg, ca.ntiles = cb * (n1 / C);
+        ra.twA = p->tw_row, ra.rows_lg = p->n2_lg, ra.nrows = cb * n2, ra.order = p->order;
+        if (!inverse) {
+            // pass A: columns of src -> scratch, times W_N^(n1*k2)
+            ca.src = cs, ca.src_dist = src_dist, ca.src_lp = src_lp, ca.src_size = src_size;
+            ca.dst = p->scratch, ca.dst_dist = 2 * n, ca.dst_lp = p->lg;
+            ca.conj_in = 0, ca.conj_out = 0, ca.scale = 1.0f;
+            ca.twl = p->twl, ca.twh = p->twh, ca.tw_ls = p->tw_ls;
+            colf(ca, st);
+            // pass B: rows of scratch -> dst in the plan's order
+            ra.src = p->scratch, ra.src_dist = 2 * n, ra.src_lp = p->lg, ra.src_size = n;
+            ra.dst = cd, ra.dst_dist = dst_dist, ra.dst_lp = dst_lp;
+            ra.load_kind = LD_CONTIG, ra.store_kind = permuted ? ST_PERM : ST_STRIDED;
+            ra.conj_in = 0, ra.conj_out = 0, ra.scale = 1.0f;
+            ra.twl = nullptr, ra.twh = nullptr;
+            rowf(ra, ra.nrows, st);
+        } else {
+            // pass 1: rows (over k1) of the spectrum -> scratch, times W_N^(n1*k2)
+            ra.src = cs, ra.src_dist = src_dist, ra.src_lp = src_lp, ra.src_size = n;
+            ra.dst = p->scratch, ra.dst_dist = 2 * n, ra.dst_lp = p->lg;
+            ra.load_kind = permuted ? LD_PERM : LD_STRIDED, ra.store_kind = ST_CONTIG;
+            ra.conj_in = 1, ra.conj_out = 0, ra.scale = 1.0f;
+            ra.twl = p->twl, ra.twh = p->twh, ra.tw_ls = p->tw_ls;
+            rowf(ra, ra.nrows, st);
+            // pass 2: columns (over k2) of scratch -> dst, natural order
+            ca.src = p->scratch, ca.src_dist = 2 * n, ca.src_lp = p->lg, ca.src_size = n;
+            ca.dst = cd, ca.dst_dist = dst_dist, ca.dst_lp = dst_lp;
+            ca.conj_in = 0, ca.conj_out = 1, ca.scale = scale;
+            ca.twl = nullptr, ca.twh = nullptr;
+            colf(ca, st);
+        }
+        HIPCHK(hipGetLastError());
+    }
+    return PCX_SUCCESS;
+}
+
+int64_t footprint(int64_t n, int64_t pack) { return ((n + pack - 1) / pack) * pack * 2; }
+
+} // namespace
+
+extern "C" int pcx_fft(pcx_plan_t p, float* dst, int64_t dst_pack, const float* src, int64_t src_pack,
+                       int64_t src_size, void* stream) {
+    if (!p)
+        return fail(PCX_ERR_INVALID_ARGUMENT, "plan is null");
+    return run_batch(p, 0, 0, dst, dst_pack, 0, src, src_pack, 0, src_size, 1, (hipStream_t)stream);
+}
+
+extern "C" int pcx_ifft(pcx_plan_t p, int normalized, float* dst, int64_t dst_pack, const float* src,
+                        int64_t src_pack, void* stream) {
+    if (!p)
+        return fail(PCX_ERR_INVALID_ARGUMENT, "plan is null");
+    return run_batch(p, 1, normalized, dst, dst_pack, 0, src, src_pack, 0, p->n, 1, (hipStream_t)stream);
+}
+
+extern "C" int pcx_fft_batch(pcx_plan_t p, int inverse, int normalized, float* dst, int64_t dst_pack,
+                             int64_t dst_dist, const float* src, int64_t src_pack, int64_t src_dist,
+                             int64_t src_size, int64_t batch, void* stream) {
+    if (!p)
+        return fail(PCX_ERR_INVALID_ARGUMENT, "plan is null");
+    if (batch > 1) {
+        int64_t dp = dst_pack, sp = src_pack;
+        if (p->order == PCX_ORDER_UNORDERED) {
+            if (!inverse && dp < 8)
+                dp = 8;
+            if (inverse && sp < 8)
+                sp = 8;
+        }
+        if (is_pow2(dp) && is_pow2(sp) &&
+            (dst_dist < footprint(p->n, dp) || src_dist < footprint(p->n, sp)))
+            return fail(PCX_ERR_INVALID_ARGUMENT, "batch distance is smaller than one vector");
+        if ((const float*)dst == src && dst_dist != src_dist)
+            return fail(PCX_ERR_INVALID_ARGUMENT, "in-place batch needs equal source and destination distances");
+    }
+    return run_batch(p, inverse, normalized, dst, dst_pack, dst_dist, src, src_pack, src_dist,
+                     inverse ? p->n : src_size, batch, (hipStream_t)stream);
+}
+
+// ---------------------------------------------------------------- cross-correlation
+struct pcx_fxcorr_s {
+    int64_t n = 0, m = 0;
+    int     lg = 0;
+    float2* tw = nullptr;
+    float*  kspec = nullptr; // FFT(g)/N, split re[N] im[N]
+    float*  halo  = nullptr;
+    int64_t halo_cap = 0;
+    std::mutex mu;
+};
+
+extern "C" int pcx_fxcorr_create(const float* g, int64_t g_pack, int64_t g_size, int64_t fft_size, void* stream,
+                                 pcx_fxcorr_t* out) {
+    if (!out)
+        return fail(PCX_ERR_INVALID_ARGUMENT, "output pointer is null");
+    *out = nullptr;
+    if (!g)
+        return fail(PCX_ERR_INVALID_ARGUMENT, "kernel pointer is null");
+    if (int e = check_size(fft_size))
+        return e;
+    if (int e = check_pack(g_pack, "kernel"))
+        return e;
+    if (g_size < 1 || g_size > fft_size)
+        return fail(PCX_ERR_SIZE_MISMATCH,
+                    "source size (which is " + s(g_size) + ") is bigger than fft size (which is " + s(fft_size) + ")");
+    int lg = ilog2(fft_size);
+    if (lg > kMaxSinglePass)
+        return fail(PCX_ERR_UNSUPPORTED, "fxcorr fft_size (which is " + s(fft_size) + ") exceeds 16384");
+    hipStream_t st = (hipStream_t)stream;
+    auto*       x  = new pcx_fxcorr_s;
+    x->n = fft_size, x->m = g_size, x->lg = lg;
+    hipError_t he = hipMalloc(&x->tw, sizeof(float2) * fft_size);
+    if (he == hipSuccess)
+        he = hipMalloc(&x->kspec, sizeof(float) * 2 * fft_size);
+    if (he == hipSuccess)
+        he = gen_twiddles(x->tw, fft_size, 1, fft_size);
+    if (he == hipSuccess)
+        he = hipDeviceSynchronize();
+    if (he != hipSuccess) {
+        pcx_fxcorr_destroy(x);
+        return fail(PCX_ERR_HIP, std::string("fxcorr setup: ") + hipGetErrorString(he));
+    }
+    // kernel spectrum: FFT of g zero-extended to N, scaled by 1/N (fxcorr.hpp:67-68)
+    RowArgs a{};
+    a.src = g, a.dst = x->kspec, a.src_dist = 0, a.dst_dist = 0, a.src_size = g_size, a.nrows = 1;
+    a.src_lp = ilog2(g_pack), a.dst_lp = lg, a.rows_lg = 0, a.load_kind = LD_CONTIG, a.store_kind = ST_CONTIG;
+    a.order = 0, a.conj_in = 0, a.conj_out = 0, a.scale = (float)(1.0 / (double)fft_size);
+    a.twA = x->tw;
+    kRow[lg - 1](a, 1, st);
+    he = hipGetLastError();
+    if (he == hipSuccess)
+        he = hipStreamSynchronize(st);
+    if (he != hipSuccess) {
+        pcx_fxcorr_destroy(x);
+        return fail(PCX_ERR_HIP, std::string("fxcorr kernel spectrum: ") + hipGetErrorString(he));
+    }
+    *out = x;
+    return PCX_SUCCESS;
+}
+
+extern "C" int pcx_fxcorr_destroy(pcx_fxcorr_t x) {
+    if (!x)
+        return PCX_SUCCESS;
+    (void)hipFree(x->tw);
+    (void)hipFree(x->kspec);
+    (void)hipFree(x->halo);
+    delete x;
+    return PCX_SUCCESS;
+}
+
+extern "C" int64_t pcx_fxcorr_step(pcx_fxcorr_t x, int64_t src_pack) {
+    if (!x || !is_pow2(src_pack))
+        return -1;
+    int64_t overlap = x->m - 1;
+    if (x->n <= overlap)
+        return -1;
+    // fxcorr.hpp:110: step = ((N - overlap) / P) * P
+    return (x->n - overlap) / src_pack * src_pack;
+}
+
+extern "C" int pcx_fxcorr_exec(pcx_fxcorr_t x, float* dst, int64_t dst_pack, const float* src, int64_t src_pack,
+                               int64_t len, void* stream) {
+    if (!x)
+        return fail(PCX_ERR_INVALID_ARGUMENT, "fxcorr handle is null");
+    if (!dst || !src)
+        return fail(PCX_ERR_INVALID_ARGUMENT, "data pointer is null");
+    if (int e = check_pack(dst_pack, "destination"))
+        return e;
+    if (int e = check_pack(src_pack, "source"))
+        return e;
+    if (len < 0)
+        return fail(PCX_ERR_SIZE_MISMATCH, "signal length (which is " + s(len) + ") is negative");
+    int64_t step = pcx_fxcorr_step(x, src_pack);
+    if (step <= 0)
+        return fail(PCX_ERR_SIZE_MISMATCH, "fft size (which is " + s(x->n) + ") leaves no output per block for a kernel of " +
+                                               s(x->m) + " taps at pack " + s(src_pack));
+    if ((const float*)dst == src && dst_pack != src_pack)
+        return fail(PCX_ERR_INPLACE_REPACK, "cannot perform in-place repack from source pack size (which is " +
+                                                s(src_pack) + ") to destination pack size (which is " + s(dst_pack) + ")");
+    if (len == 0)
+        return PCX_SUCCESS;
+    hipStream_t st = (hipStream_t)stream;
+    int64_t     nblocks = (len + step - 1) / step;
+    XcArgs      a{};
+    a.src = src, a.dst = dst, a.kre = x->kspec, a.kim = x->kspec + x->n, a.twA = x->tw;
+    a.len = len, a.step = step, a.nblocks = nblocks, a.hlen = x->n - step;
+    a.src_lp = ilog2(src_pack), a.dst_lp = ilog2(dst_pack);
+    std::lock_guard<std::mutex> lock(x->mu);
+    if ((const float*)dst == src && nblocks > 1 && a.hlen > 0) {
+        int64_t need = nblocks * a.hlen;
+        if (x->halo_cap < need) {
+            if (x->halo) {
+                HIPCHK(hipStreamSynchronize(st));
+                HIPCHK(hipFree(x->halo));
+                x->halo = nullptr;
+            }
+            hipError_t he = hipMalloc(&x->halo, sizeof(float) * 2 * need);
+            if (he != hipSuccess)
+                return fail(PCX_ERR_OUT_OF_MEMORY, "halo allocation failed");
+            x->halo_cap = need;
+        }
+        a.hre = x->halo, a.him = x->halo + need;
+        int64_t nb = (need + 255) / 256;
+        hipLaunchKernelGGL(k_fxcorr_halo, dim3((unsigned)nb), dim3(256), 0, st, src, a.src_lp, len, step, a.hlen,
+                           nblocks, x->halo, x->halo + need);
+        HIPCHK(hipGetLastError());
+    }
+    kXc[x->lg - 1](a, st);
+    HIPCHK(hipGetLastError());
+    return PCX_SUCCESS;
+}
+
+// ---------------------------------------------------------------- memory and utilities
+extern "C" int pcx_malloc(void** ptr, size_t bytes) {
+    if (!ptr)
+        return fail(PCX_ERR_INVALID_ARGUMENT, "pointer is null");
+    hipError_t e = hipMalloc(ptr, bytes);
+    if (e != hipSuccess)
+        return fail(PCX_ERR_OUT_OF_MEMORY, std::string("hipMalloc: ") + hipGetErrorString(e));
+    return PCX_SUCCESS;
+}
+extern "C" int pcx_free(void* ptr) {
+    HIPCHK(hipFree(ptr));
+    return PCX_SUCCESS;
+}
+extern "C" int pcx_memcpy_htod(void* dst, const void* src, size_t bytes) {
+    HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return PCX_SUCCESS;
+}
+extern "C" int pcx_memcpy_dtoh(void* dst, const void* src, size_t bytes) {
+    HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return PCX_SUCCESS;
+}
+extern "C" int pcx_memset(void* dst, int value, size_t bytes) {
+    HIPCHK(hipMemset(dst, value, bytes));
+    return PCX_SUCCESS;
+}
+extern "C" int pcx_synchronize(void* stream) {
+    HIPCHK(hipStreamSynchronize((hipStream_t)stream));
+    return PCX_SUCCESS;
+}
+extern "C" int pcx_generate(float* dst, int64_t pack, int64_t n, uint64_t seed, int dist, double sigma, void* stream) {
+    if (!dst)
+        return fail(PCX_ERR_INVALID_ARGUMENT, "data pointer is null");
+    if (int e = check_pack(pack, "destination"))
+        return e;
+    if (n <= 0)
+        return PCX_SUCCESS;
+    int64_t nb = (n + 255) / 256;
+    hipLaunchKernelGGL(k_generate, dim3((unsigned)nb), dim3(256), 0, (hipStream_t)stream, dst, ilog2(pack), n, seed,
+                       dist, sigma);
+    HIPCHK(hipGetLastError());
+    return PCX_SUCCESS;
+}

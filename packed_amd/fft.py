@@ -1,0 +1,209 @@
+"""Python mirror of the reference's FFT-unit and cross-correlation interfaces over the C ABI.
+
+``vector``      pcxo::vector<float, P> (vector.hpp:26-382): n complex values in packs of P reals
+                then P imaginaries, ceil(n/P)*2P floats, zero-filled -- here resident in device
+                memory (a torch float32 tensor on the GPU; torch is only the allocator).
+``fft_unit``    pcxo::fft_unit<float, Order> (fft.hpp:565-915): size(), operator() (forward, in or
+                out of place, zero-extended source), ifft<Normalized>(), fft_raw / ifft_raw on
+                raw device pointers.
+``fxcorr_unit`` pcxo::fxcorr_unit<float> (fxcorr.hpp:44-147): construct from the kernel g and an
+                FFT size, call on a signal (in place), or out of place.
+
+Errors raise ``PcxError`` (a ValueError, the counterpart of std::invalid_argument) carrying the
+reference's messages.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import _lib
+from ._lib import ORDER, PcxError, check, lib
+
+__all__ = ["vector", "fft_unit", "fxcorr_unit", "PcxError", "real_size", "pidx"]
+
+
+def _torch():
+    import torch
+
+    if not torch.cuda.is_available():
+        raise RuntimeError("packed_amd needs a ROCm GPU: torch.cuda.is_available() is False")
+    return torch
+
+
+def real_size(n: int, pack: int) -> int:
+    """vector.hpp:376-378: floats occupied by n complex values in pack P."""
+    return ((n + pack - 1) // pack) * pack * 2
+
+
+def pidx(i, pack: int):
+    """element_access.hpp:8-11: float offset of the real part of complex i."""
+    i = np.asarray(i, dtype=np.int64)
+    return i + (i // pack) * pack
+
+
+def _stream_ptr(stream) -> int | None:
+    if stream is None:
+        torch = _torch()
+        return torch.cuda.current_stream().cuda_stream
+    return getattr(stream, "cuda_stream", stream)
+
+
+class vector:
+    """Device-resident packed complex vector (pcxo::vector<float, P> layout)."""
+
+    def __init__(self, n: int, pack: int = 8, data=None):
+        if pack < 1 or pack & (pack - 1):
+            raise PcxError(_lib.ERR_INVALID_PACK, f"pack size (which is {pack}) is not a power of two")
+        torch = _torch()
+        self.n = int(n)
+        self.pack = int(pack)
+        if data is None:
+            data = torch.zeros(real_size(self.n, self.pack), dtype=torch.float32, device="cuda")
+        self.data = data
+
+    def size(self) -> int:
+        return self.n
+
+    def __len__(self) -> int:
+        return self.n
+
+    def data_ptr(self) -> int:
+        return self.data.data_ptr()
+
+    @classmethod
+    def from_complex(cls, z, pack: int = 8) -> "vector":
+        torch = _torch()
+        z = np.asarray(z)
+        buf = np.zeros(real_size(len(z), pack), np.float32)
+        idx = pidx(np.arange(len(z)), pack)
+        buf[idx] = np.real(z)
+        buf[idx + pack] = np.imag(z)
+        return cls(len(z), pack, torch.from_numpy(buf).cuda())
+
+    def to_complex(self, n: int | None = None, pack: int | None = None) -> np.ndarray:
+        """Complex values read in `pack` layout (default: this vector's pack)."""
+        n = self.n if n is None else n
+        pack = self.pack if pack is None else pack
+        buf = self.data.cpu().numpy()
+        idx = pidx(np.arange(n), pack)
+        return buf[idx].astype(np.float64) + 1j * buf[idx + pack].astype(np.float64)
+
+
+class fft_unit:
+    """fft.hpp:565-915 fft_unit<float, Order> on the GPU."""
+
+    def __init__(self, fft_size: int, sub_size: int = 2048, order: str = "normal"):
+        if order not in ORDER:
+            raise PcxError(_lib.ERR_INVALID_ARGUMENT, f"unknown fft_order {order!r}")
+        import ctypes as C
+
+        h = C.c_void_p()
+        check(lib().pcx_plan_create(int(fft_size), int(sub_size), ORDER[order], C.byref(h)))
+        self._h = h
+        self.order = order
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            lib().pcx_plan_destroy(h)
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    def size(self) -> int:
+        return int(lib().pcx_plan_size(self._h))
+
+    # ---------------------------------------------------------------- raw pointer forms
+    def fft_raw(self, dst: int, src: int | None = None, dst_pack: int = 1, src_pack: int | None = None,
+                src_size: int | None = None, stream=None) -> None:
+        """fft_raw<PD,PS>(dst[, src[, src_size]]) (fft.hpp:624-687) on device pointers."""
+        src = dst if src is None else src
+        src_pack = dst_pack if src_pack is None else src_pack
+        src_size = self.size() if src_size is None else src_size
+        check(lib().pcx_fft(self._h, dst, dst_pack, src, src_pack, src_size, _stream_ptr(stream)))
+
+    def ifft_raw(self, dst: int, src: int | None = None, normalized: bool = True, dst_pack: int = 1,
+                 src_pack: int | None = None, stream=None) -> None:
+        """ifft_raw<Normalized,PD,PS>(dst[, src]) (fft.hpp:689-721) on device pointers."""
+        src = dst if src is None else src
+        src_pack = dst_pack if src_pack is None else src_pack
+        check(lib().pcx_ifft(self._h, int(normalized), dst, dst_pack, src, src_pack, _stream_ptr(stream)))
+
+    def batch(self, dst: int, src: int, batch: int, dst_pack: int, dst_dist: int, src_pack: int,
+              src_dist: int, inverse: bool = False, normalized: bool = True, src_size: int | None = None,
+              stream=None) -> None:
+        src_size = self.size() if src_size is None else src_size
+        check(lib().pcx_fft_batch(self._h, int(inverse), int(normalized), dst, dst_pack, dst_dist, src, src_pack,
+                                  src_dist, src_size, batch, _stream_ptr(stream)))
+
+    # ---------------------------------------------------------------- vector forms
+    def _check_dst(self, v: vector, what: str) -> None:
+        if v.size() != self.size():
+            raise PcxError(_lib.ERR_SIZE_MISMATCH, f"{what} size (which is {v.size()}) is not equal to fft size "
+                                                   f"(which is {self.size()})")
+
+    def __call__(self, dst: vector, src: vector | None = None, stream=None) -> None:
+        """operator()(v) in place, or operator()(dst, src) with src zero-extended (fft.hpp:728-820)."""
+        if src is None:
+            if dst.size() != self.size():
+                raise PcxError(_lib.ERR_SIZE_MISMATCH, f"input size (which is {dst.size()}) is not equal to fft "
+                                                       f"size (which is {self.size()})")
+            self.fft_raw(dst.data_ptr(), dst.data_ptr(), dst.pack, dst.pack, self.size(), stream)
+            return
+        self._check_dst(dst, "destination")
+        if src.size() > self.size():
+            raise PcxError(_lib.ERR_SIZE_MISMATCH, f"source size (which is {src.size()}) is bigger than fft size "
+                                                   f"(which is {self.size()})")
+        self.fft_raw(dst.data_ptr(), src.data_ptr(), dst.pack, src.pack, src.size(), stream)
+
+    def ifft(self, dst: vector, src: vector | None = None, normalized: bool = True, stream=None) -> None:
+        """ifft<Normalized>(v) / ifft<Normalized>(dst, src) (fft.hpp:828-915)."""
+        if src is None:
+            if dst.size() != self.size():
+                raise PcxError(_lib.ERR_SIZE_MISMATCH, f"input size (which is {dst.size()} is not equal to fft "
+                                                       f"size (which is {self.size()})")
+            src = dst
+        else:
+            self._check_dst(dst, "destination")
+            if src.size() != self.size():
+                raise PcxError(_lib.ERR_SIZE_MISMATCH, f"source size (which is {src.size()}) is not equal to fft "
+                                                       f"size (which is {self.size()})")
+        self.ifft_raw(dst.data_ptr(), src.data_ptr(), normalized, dst.pack, src.pack, stream)
+
+
+class fxcorr_unit:
+    """fxcorr.hpp:44-147 fxcorr_unit<float> on the GPU: out[i] = sum_m f[i+m] * conj(g[m])."""
+
+    def __init__(self, g: vector, fft_size: int, stream=None):
+        import ctypes as C
+
+        h = C.c_void_p()
+        check(lib().pcx_fxcorr_create(g.data_ptr(), g.pack, g.size(), int(fft_size), _stream_ptr(stream),
+                                      C.byref(h)))
+        self._h = h
+        self.fft_size = int(fft_size)
+        self.g_size = g.size()
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            lib().pcx_fxcorr_destroy(h)
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    def step(self, pack: int = 8) -> int:
+        return int(lib().pcx_fxcorr_step(self._h, pack))
+
+    def __call__(self, f: vector, out: vector | None = None, stream=None) -> None:
+        """operator()(vector&) in place (fxcorr.hpp:102-136), or into `out` (the out-of-place overload)."""
+        dst = f if out is None else out
+        if dst.size() < f.size():
+            raise PcxError(_lib.ERR_SIZE_MISMATCH, f"destination size (which is {dst.size()}) is smaller than "
+                                                   f"the signal (which is {f.size()})")
+        check(lib().pcx_fxcorr_exec(self._h, dst.data_ptr(), dst.pack, f.data_ptr(), f.pack, f.size(),
+                                    _stream_ptr(stream)))

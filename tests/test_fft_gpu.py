@@ -1,0 +1,167 @@
+"""GPU parity of the FFT unit (fft.hpp:565-915) against the FP64 oracle.
+
+Every case: seeded inputs from the oracle's generator, the HIP path through the C ABI, and the
+oracle's restatement of the same call (orc_fft_unit).  Output order is checked index-exactly
+(a misplaced slot gives a relative L2 error of order 1/sqrt(N), far above the bound) and values
+within 2*log2(N)*FLT_EPSILON relative L2 of FP64.
+"""
+import numpy as np
+import pytest
+
+from helpers import batch_buffer, fft_tol, to_device, unpack_batch
+
+pytestmark = pytest.mark.gpu
+
+ORDERS = ["normal", "bit_reversed", "unordered"]
+
+
+def _spec_pack(order, pack):
+    return max(pack, 8) if order == "unordered" else pack
+
+
+def _fwd_case(O, pa, n, order, src_pack, dst_pack, batch=1, src_size=None, seed=0x5EED0001, inplace=False):
+    import torch
+
+    src_size = n if src_size is None else src_size
+    dpk = _spec_pack(order, dst_pack)
+    sdist = O.real_size(n, src_pack)
+    ddist = O.real_size(n, dpk)
+    xs = [O.gen_uniform(seed + b, n) for b in range(batch)]
+    xs_cut = [x[:src_size] for x in xs]
+    sbuf = batch_buffer(xs_cut, src_pack, sdist)
+    unit = pa.fft_unit(n, order=order)
+    s = to_device(sbuf)
+    if inplace:
+        assert sdist == ddist and src_pack == dpk
+        d = s
+    else:
+        d = torch.zeros(ddist * batch, dtype=torch.float32, device="cuda")
+    if batch == 1:
+        unit.fft_raw(d.data_ptr(), s.data_ptr(), dst_pack, src_pack, src_size)
+    else:
+        unit.batch(d.data_ptr(), s.data_ptr(), batch, dst_pack, ddist, src_pack, sdist, src_size=src_size)
+    torch.cuda.synchronize()
+    got = unpack_batch(d.cpu().numpy(), n, dpk, ddist, batch)
+    errs = []
+    for b in range(batch):
+        ref_src = O.to_packed(xs_cut[b], src_pack, n_alloc=n)
+        _, want = O.fft_unit(order, False, True, n, ref_src, src_pack, dst_pack, src_size)
+        errs.append(O.rel_l2(got[b], want))
+    return max(errs)
+
+
+def _inv_case(O, pa, n, order, src_pack, dst_pack, normalized=True, batch=1, seed=0x5EED0002):
+    import torch
+
+    spk = _spec_pack(order, src_pack)
+    sdist = O.real_size(n, spk)
+    ddist = O.real_size(n, dst_pack)
+    zs = [O.gen_normal(seed + b, n) for b in range(batch)]
+    sbuf = batch_buffer(zs, spk, sdist)
+    unit = pa.fft_unit(n, order=order)
+    s = to_device(sbuf)
+    d = torch.zeros(ddist * batch, dtype=torch.float32, device="cuda")
+    if batch == 1:
+        unit.ifft_raw(d.data_ptr(), s.data_ptr(), normalized, dst_pack, src_pack)
+    else:
+        unit.batch(d.data_ptr(), s.data_ptr(), batch, dst_pack, ddist, src_pack, sdist, inverse=True,
+                   normalized=normalized)
+    torch.cuda.synchronize()
+    got = unpack_batch(d.cpu().numpy(), n, dst_pack, ddist, batch)
+    errs = []
+    for b in range(batch):
+        _, want = O.fft_unit(order, True, normalized, n, sbuf[b * sdist:(b + 1) * sdist], spk, dst_pack)
+        errs.append(O.rel_l2(got[b], want))
+    return max(errs)
+
+
+@pytest.fixture(scope="module")
+def pa():
+    import packed_amd
+
+    return packed_amd
+
+
+@pytest.mark.parametrize("lg", list(range(1, 21)))
+@pytest.mark.parametrize("order", ORDERS)
+def test_forward_all_sizes(orc, pa, lg, order):
+    n = 1 << lg
+    if order == "unordered" and lg < 5:
+        pytest.skip("unordered needs the 32-point finisher (N >= 32)")
+    err = _fwd_case(orc, pa, n, order, 8, 8, batch=3 if lg <= 16 else 1)
+    assert err < fft_tol(n), (n, order, err)
+
+
+@pytest.mark.parametrize("lg", list(range(1, 21)))
+@pytest.mark.parametrize("order", ORDERS)
+def test_inverse_all_sizes(orc, pa, lg, order):
+    n = 1 << lg
+    if order == "unordered" and lg < 5:
+        pytest.skip("unordered needs the 32-point finisher (N >= 32)")
+    err = _inv_case(orc, pa, n, order, 8, 8, batch=2 if lg <= 16 else 1)
+    assert err < fft_tol(n), (n, order, err)
+
+
+@pytest.mark.parametrize("n", [64, 1024, 4096, 1 << 15, 1 << 16])
+@pytest.mark.parametrize("packs", [(1, 1), (8, 1), (1, 8), (4, 2), ("N", "N"), ("N", 8)])
+@pytest.mark.parametrize("order", ORDERS)
+def test_pack_layouts(orc, pa, n, packs, order):
+    sp, dp = (n if p == "N" else p for p in packs)
+    assert _fwd_case(orc, pa, n, order, sp, dp) < fft_tol(n)
+    assert _inv_case(orc, pa, n, order, dp, sp) < fft_tol(n)
+
+
+@pytest.mark.parametrize("n,src_size", [(64, 1), (64, 37), (1024, 1000), (4096, 100), (1 << 16, 12345),
+                                        (1 << 18, 1 << 17)])
+@pytest.mark.parametrize("order", ORDERS)
+def test_zero_extension(orc, pa, n, src_size, order):
+    assert _fwd_case(orc, pa, n, order, 8, 8, src_size=src_size) < fft_tol(n)
+
+
+@pytest.mark.parametrize("n", [256, 8192, 1 << 16])
+@pytest.mark.parametrize("order", ORDERS)
+def test_in_place(orc, pa, n, order):
+    assert _fwd_case(orc, pa, n, order, 8, 8, batch=2, inplace=True) < fft_tol(n)
+
+
+@pytest.mark.parametrize("n", [128, 1 << 16])
+@pytest.mark.parametrize("order", ORDERS)
+def test_unnormalized_inverse(orc, pa, n, order):
+    assert _inv_case(orc, pa, n, order, 8, 8, normalized=False) < fft_tol(n)
+
+
+@pytest.mark.parametrize("n,batch", [(64, 1000), (1024, 257), (1 << 16, 5), (1 << 20, 2)])
+def test_batched(orc, pa, n, batch):
+    assert _fwd_case(orc, pa, n, "normal", n, n, batch=batch) < fft_tol(n)
+    assert _inv_case(orc, pa, n, "bit_reversed", n, n, batch=batch) < fft_tol(n)
+
+
+@pytest.mark.parametrize("n", [1024, 1 << 16])
+@pytest.mark.parametrize("order", ORDERS)
+def test_round_trip(orc, pa, n, order):
+    import torch
+
+    x = orc.gen_uniform(0x5EED0001, n)
+    v = pa.vector.from_complex(x, 8)
+    unit = pa.fft_unit(n, order=order)
+    unit(v)
+    unit.ifft(v)
+    torch.cuda.synchronize()
+    assert orc.rel_l2(v.to_complex(), x) < 2 * fft_tol(n)
+
+
+def test_reference_leaky_tone(orc, pa):
+    """tests/test_fft.cpp:299-302 input exp(i*2*pi*i/N*13.37) against its closed-form DFT."""
+    import torch
+
+    for n in [64, 1024, 16384]:
+        i = np.arange(n)
+        f = 13.37
+        x = np.exp(1j * 2 * np.pi * i / n * f).astype(np.complex64)
+        k = np.arange(n)
+        d = 2j * np.pi * (f - k)
+        want = (1 - np.exp(d)) / (1 - np.exp(d / n))
+        v = pa.vector.from_complex(x, 8)
+        pa.fft_unit(n)(v)
+        torch.cuda.synchronize()
+        assert orc.rel_l2(v.to_complex(), want) < fft_tol(n) + 1e-6

@@ -1,0 +1,117 @@
+"""GPU parity of the overlap-save cross-correlation (fxcorr.hpp:44-147).
+
+The definition is the one the reference test encodes (tests/test_fxcorr.cpp:14-32):
+out[i] = sum_{m<M} f[i+m] * conj(g[m]), f zero beyond L.  The reference itself is only correct
+for a single block (its block offset is applied to a float pointer, fxcorr.hpp:112-129), so
+multi-block results are checked against the FP64 definition, not against the reference.
+"""
+import numpy as np
+import pytest
+
+from helpers import EPS32
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def pa():
+    import packed_amd
+
+    return packed_amd
+
+
+BARKER13 = np.array([1, 1, 1, 1, 1, -1, -1, 1, 1, -1, 1, -1, 1], dtype=np.float64)
+
+
+def _xc_tol(f, g):
+    # FP32 overlap-save: error relative to |f|*|g| energy per output, generous multiple of eps*log2(N)
+    return 64 * EPS32
+
+
+def _run(pa, f, g, fft_size, pack=8, inplace=True):
+    import torch
+
+    gv = pa.vector.from_complex(g, pack)
+    xc = pa.fxcorr_unit(gv, fft_size)
+    fv = pa.vector.from_complex(f, pack)
+    if inplace:
+        xc(fv)
+        out = fv
+    else:
+        out = pa.vector(len(f), pack)
+        xc(fv, out)
+    torch.cuda.synchronize()
+    return out.to_complex(), xc
+
+
+def test_reference_barker_case(orc, pa):
+    """tests/test_fxcorr.cpp:100-139: g = Barker-13, f = i*(Barker at 0 and 109) + (rand()%7 - 3),
+    N = 256.  Every value is an integer combination, so the expected output is exact."""
+    size = 128
+    g = np.zeros(size)
+    g[:13] = BARKER13
+    sf = np.zeros(size, np.complex128)
+    sf[:13] += BARKER13
+    sf[109:122] += BARKER13
+    rnd = orc.GlibcRand(1)
+    for i in range(size):
+        sf[i] = sf[i] * 1j + (rnd() % 7 - 3)
+    want = orc.xcorr_naive(sf, g)
+    assert np.array_equal(np.round(want), want)  # exact integers
+    assert want[0].imag == 13 and want[109].imag == 13  # Barker peaks
+    got, _ = _run(pa, sf, g, size * 2)
+    assert np.max(np.abs(got - want)) < 1e-4 * np.max(np.abs(want))
+
+
+def test_reference_random_phase_case(orc, pa):
+    """tests/test_fxcorr.cpp:61-98 test_xcorr<float>(512): M = 512 random phases exp(i*(rand()%6-3)),
+    N = 1024, L = 512, with rand() continuing after the Barker case's 128 draws."""
+    rnd = orc.GlibcRand(1)
+    for _ in range(128):
+        rnd()
+    g = np.array([np.exp(1j * (rnd() % 6 - 3)) for _ in range(512)])
+    f = np.array([np.exp(1j * (rnd() % 6 - 3)) for _ in range(512)])
+    want = orc.xcorr_naive(f, g)
+    got, _ = _run(pa, f.astype(np.complex64), g.astype(np.complex64), 1024)
+    # reference tolerance is k = 1e7 FLT_EPSILON per element (tests/test_fxcorr.cpp:91); ours is tighter
+    assert orc.rel_l2(got, want) < 2e-6
+
+
+@pytest.mark.parametrize("L,M,N", [(5000, 100, 256), (4096, 64, 512), (70000, 4096, 16384), (12345, 1, 64),
+                                   (3000, 57, 128), (1 << 16, 1024, 4096)])
+@pytest.mark.parametrize("inplace", [True, False])
+def test_multi_block(orc, pa, L, M, N, inplace):
+    f = orc.gen_normal(0x5EED0003, L, np.sqrt(0.5)).astype(np.complex128)
+    ph = orc.gen_uniform(0x5EED0013, M)
+    g = np.exp(1j * np.pi * np.real(ph))
+    want = orc.xcorr_naive(f, g)
+    got, xc = _run(pa, f.astype(np.complex64), g.astype(np.complex64), N, inplace=inplace)
+    assert xc.step(8) == (N - (M - 1)) // 8 * 8
+    assert orc.rel_l2(got, want) < 4e-6, orc.rel_l2(got, want)
+
+
+def test_overlap_save_oracle_agrees(orc):
+    """The FP64 overlap-save restatement equals the direct definition (pins the block logic)."""
+    f = orc.gen_normal(7, 9000).astype(np.complex128)
+    g = orc.gen_normal(8, 300).astype(np.complex128)
+    a = orc.xcorr_naive(f, g)
+    b, nb = orc.fxcorr_overlap_save(f, g, 1024, 8)
+    assert nb == -(-9000 // ((1024 - 299) // 8 * 8))
+    assert orc.rel_l2(b, a) < 1e-13
+
+
+def test_peaks_index_exact(orc, pa):
+    """Scaled-down form of the 2^24-sample workload: 16 copies of a 4096-tap unit-modulus kernel
+    buried at SNR 0 dB; the 16 largest outputs sit exactly at the insertion offsets."""
+    L, M, N = 1 << 20, 4096, 16384
+    rng_ph = orc.gen_uniform(0x5EED0004, M)
+    g = np.exp(1j * np.pi * np.real(rng_ph))
+    f = orc.gen_normal(0x5EED0003, L, np.sqrt(0.5)).astype(np.complex128)
+    offs = np.sort(np.random.default_rng(0x5EED0003).choice(np.arange(0, L - M, M), 16, replace=False))
+    for o in offs:
+        f[o:o + M] += g
+    got, _ = _run(pa, f.astype(np.complex64), g.astype(np.complex64), N)
+    top = np.sort(np.argsort(np.abs(got))[-16:])
+    assert np.array_equal(top, offs)
+    want = orc.xcorr_naive(f[: 1 << 16], g)[: (1 << 16) - M]
+    assert orc.rel_l2(got[: (1 << 16) - M], want) < 4e-6
