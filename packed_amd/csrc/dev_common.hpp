@@ -51,19 +51,16 @@ constexpr double kCos16[9] = {1.0,
                               0.38268343236508977173,
                               0.19509032201612826785,
                               0.0};
-// cos(2*pi*k/32) for k in [0, 32)
+// cos(2*pi*k/32) = cos(pi*k/16) for k in [0, 32)
 constexpr double cos32(int k) {
     k &= 31;
-    int m = 2 * k; // angle = pi*m/16, m in [0, 64)
-    if (m <= 8)
-        return kCos16[m];
-    if (m <= 16)
-        return -kCos16[16 - m];
-    if (m <= 24)
-        return -kCos16[m - 16];
-    if (m <= 32)
-        return kCos16[32 - m];
-    return cos32(32 - k);
+    if (k <= 8)
+        return kCos16[k];
+    if (k <= 16)
+        return -kCos16[16 - k];
+    if (k <= 24)
+        return -kCos16[k - 16];
+    return kCos16[32 - k];
 }
 constexpr double sin32(int k) { return cos32(k - 8); } // sin(a) = cos(a - pi/2)
 
