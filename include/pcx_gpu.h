@@ -103,6 +103,7 @@ int pcx_malloc(void** ptr, size_t bytes);
 int pcx_free(void* ptr);
 int pcx_memcpy_htod(void* dst, const void* src, size_t bytes);
 int pcx_memcpy_dtoh(void* dst, const void* src, size_t bytes);
+int pcx_memcpy_dtod(void* dst, const void* src, size_t bytes);
 int pcx_memset(void* dst, int value, size_t bytes);
 int pcx_synchronize(void* stream);
 /* Seeded synthetic input written straight into HBM: splitmix64 outputs 2i and 2i+1 give

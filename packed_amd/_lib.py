@@ -45,6 +45,7 @@ SIGNATURES = {
     "pcx_free": (_int, [_vp]),
     "pcx_memcpy_htod": (_int, [_vp, _vp, _sz]),
     "pcx_memcpy_dtoh": (_int, [_vp, _vp, _sz]),
+    "pcx_memcpy_dtod": (_int, [_vp, _vp, _sz]),
     "pcx_memset": (_int, [_vp, _int, _sz]),
     "pcx_synchronize": (_int, [_vp]),
     "pcx_generate": (_int, [_vp, _i64, _i64, _u64, _int, _dbl, _vp]),

@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <array>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
@@ -107,31 +108,47 @@ constexpr auto xc_table(std::integer_sequence<int, L...>) {
 const auto kRow = row_table(std::make_integer_sequence<int, kMaxSinglePass>{});
 const auto kXc  = xc_table(std::make_integer_sequence<int, kMaxSinglePass>{});
 
-// Pass-B / pass-1 rows of the four-step (N1 = 2^LA): 32 rows per workgroup where the workgroup
-// stays within 1024 threads so strided stores write 128-byte runs.
-RowLaunch row_pass_launch(int la) {
+// Pass-B / pass-1 rows of the four-step (N1 = 2^LA, G rows per workgroup, G * N1 / 16 <= 1024
+// threads).  Strided stores write G consecutive floats per run.
+template<int LA, int... Gs>
+RowLaunch pick_row(int g) {
+    RowLaunch f = nullptr;
+    ((g == Gs ? (f = &launch_row<LA, Gs>, 0) : 0), ...);
+    return f;
+}
+RowLaunch row_pass_launch(int la, int g) {
     switch (la) {
-    case 7: return &launch_row<7, 32>;
-    case 8: return &launch_row<8, 32>;
-    case 9: return &launch_row<9, 32>;
-    case 10: return &launch_row<10, 16>;
-    case 11: return &launch_row<11, 8>;
-    case 12: return &launch_row<12, 4>;
-    case 13: return &launch_row<13, 2>;
-    case 14: return &launch_row<14, 1>;
+    case 6: return pick_row<6, 16, 32, 64, 128>(g);
+    case 7: return pick_row<7, 16, 32, 64, 128>(g);
+    case 8: return pick_row<8, 8, 16, 32, 64>(g);
+    case 9: return pick_row<9, 8, 16, 32>(g);
+    case 10: return pick_row<10, 4, 8, 16>(g);
+    case 11: return pick_row<11, 2, 4, 8>(g);
+    case 12: return pick_row<12, 1, 2, 4>(g);
+    case 13: return pick_row<13, 1, 2>(g);
+    case 14: return pick_row<14, 1>(g);
     default: return nullptr;
     }
 }
+int default_row_g(int la) { return la <= 7 ? 64 : la == 8 ? 32 : la == 9 ? 32 : la == 10 ? 16 : la == 11 ? 8 : la == 12 ? 4 : la == 13 ? 2 : 1; }
 // Column passes (N2 = 2^LA rows per column, C columns per tile, C * N2 <= 16384 for LDS).
-ColLaunch col_pass_launch(int la, int* C) {
+template<int LA, int... Cs>
+ColLaunch pick_col(int c) {
+    ColLaunch f = nullptr;
+    ((c == Cs ? (f = &launch_col<LA, Cs>, 0) : 0), ...);
+    return f;
+}
+ColLaunch col_pass_launch(int la, int c) {
     switch (la) {
-    case 7: *C = 32; return &launch_col<7, 32>;
-    case 8: *C = 32; return &launch_col<8, 32>;
-    case 9: *C = 32; return &launch_col<9, 32>;
-    case 10: *C = 16; return &launch_col<10, 16>;
+    case 6: return pick_col<6, 16, 32, 64>(c);
+    case 7: return pick_col<7, 16, 32, 64>(c);
+    case 8: return pick_col<8, 16, 32, 64>(c);
+    case 9: return pick_col<9, 16, 32>(c);
+    case 10: return pick_col<10, 16>(c);
     default: return nullptr;
     }
 }
+int default_col_c(int la) { return la <= 8 ? 64 : la == 9 ? 32 : 16; }
 
 } // namespace
 
@@ -150,6 +167,8 @@ struct pcx_plan_s {
     float*  scratch      = nullptr;
     int64_t scratch_cap  = 0; // transforms the scratch holds
     int64_t chunk        = 0; // transforms per four-step chunk
+    int     col_c        = 0; // columns per column-pass tile
+    int     row_g        = 0; // rows per row-pass workgroup
     std::mutex mu;
 };
 
@@ -184,6 +203,11 @@ extern "C" int pcx_plan_create(int64_t fft_size, int64_t sub_size, int order, pc
     } else {
         p->two_pass = true;
         p->n2_lg    = lg / 2 > 10 ? 10 : lg / 2;
+        if (const char* env = std::getenv("PCX_N2_LOG2")) { // column-length override for tuning sweeps
+            int v = std::atoi(env);
+            if (v >= 6 && v <= 10 && lg - v >= 6 && lg - v <= 14)
+                p->n2_lg = v;
+        }
         p->n1_lg    = lg - p->n2_lg;
         p->tw_ls    = (lg + 1) / 2;
         int64_t n1 = (int64_t)1 << p->n1_lg, n2 = (int64_t)1 << p->n2_lg;
@@ -203,13 +227,22 @@ extern "C" int pcx_plan_create(int64_t fft_size, int64_t sub_size, int order, pc
             he = gen_twiddles(p->twl, fft_size, 1, nl);
         if (he == hipSuccess)
             he = gen_twiddles(p->twh, fft_size, nl, nh);
-        // chunk of transforms whose intermediate stays cache-resident between the two passes
-        int64_t chunk_bytes = (int64_t)64 << 20;
+        // transforms per four-step chunk: the scratch holds one chunk's intermediate
+        int64_t chunk_bytes = (int64_t)2048 << 20;
         if (const char* env = std::getenv("PCX_CHUNK_MB"))
             chunk_bytes = (int64_t)std::atoll(env) << 20;
         p->chunk = chunk_bytes / (fft_size * 8);
         if (p->chunk < 1)
             p->chunk = 1;
+        p->col_c = default_col_c(p->n2_lg);
+        p->row_g = default_row_g(p->n1_lg);
+        if (const char* env = std::getenv("PCX_TILE")) { // "C,G" override for tuning sweeps
+            int c = 0, g = 0;
+            if (std::sscanf(env, "%d,%d", &c, &g) == 2 && col_pass_launch(p->n2_lg, c) && row_pass_launch(p->n1_lg, g)) {
+                p->col_c = c;
+                p->row_g = g;
+            }
+        }
     }
     if (he == hipSuccess)
         he = hipDeviceSynchronize();
@@ -302,9 +335,9 @@ int run_batch(pcx_plan_t p, int inverse, int normalized, float* dst, int64_t dst
             return fail(PCX_ERR_OUT_OF_MEMORY, "scratch allocation of " + s(8 * n * chunk) + " bytes failed");
         p->scratch_cap = chunk;
     }
-    int       C    = 0;
-    ColLaunch colf = col_pass_launch(p->n2_lg, &C);
-    RowLaunch rowf = row_pass_launch(p->n1_lg);
+    const int C    = p->col_c;
+    ColLaunch colf = col_pass_launch(p->n2_lg, C);
+    RowLaunch rowf = row_pass_launch(p->n1_lg, p->row_g);
     if (!colf || !rowf)
         return fail(PCX_ERR_UNSUPPORTED, "no four-step kernels for fft_size " + s(n));
     const int64_t n1 = (int64_t)1 << p->n1_lg, n2 = (int64_t)1 << p->n2_lg;
@@ -542,6 +575,10 @@ extern "C" int pcx_memcpy_htod(void* dst, const void* src, size_t bytes) {
 }
 extern "C" int pcx_memcpy_dtoh(void* dst, const void* src, size_t bytes) {
     HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return PCX_SUCCESS;
+}
+extern "C" int pcx_memcpy_dtod(void* dst, const void* src, size_t bytes) {
+    HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
     return PCX_SUCCESS;
 }
 extern "C" int pcx_memset(void* dst, int value, size_t bytes) {

@@ -104,8 +104,8 @@ class fft_unit:
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h is not None and h.value:
-            lib().pcx_plan_destroy(h)
+        if h is not None and h.value and _lib is not None and _lib._lib is not None:
+            _lib._lib.pcx_plan_destroy(h)
             self._h = None
 
     @property
@@ -188,8 +188,8 @@ class fxcorr_unit:
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h is not None and h.value:
-            lib().pcx_fxcorr_destroy(h)
+        if h is not None and h.value and _lib is not None and _lib._lib is not None:
+            _lib._lib.pcx_fxcorr_destroy(h)
             self._h = None
 
     @property

@@ -1,0 +1,257 @@
+// C++ parity test of the drop-in classes in packed_amd/include/pcx_gpu/fft.hpp, written after the
+// reference's tests/test_fft.cpp (test_fft_float 288-435, test_fftu_float 437-639) and
+// tests/test_fxcorr.cpp (61-139), with the FP64 oracle (oracle/pcx_oracle.h) as the checker.
+// Exit code = number of failed checks.
+#include "pcx_gpu/fft.hpp"
+#include "pcx_oracle.h"
+
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <limits>
+#include <string>
+#include <vector>
+
+using cxf = std::complex<float>;
+using cxd = std::complex<double>;
+
+static int g_fail = 0;
+#define CHECK(cond, ...)                                                                           \
+    do {                                                                                           \
+        if (!(cond)) {                                                                             \
+            ++g_fail;                                                                              \
+            std::printf("FAIL %s:%d: ", __FILE__, __LINE__);                                       \
+            std::printf(__VA_ARGS__);                                                              \
+            std::printf("\n");                                                                     \
+        }                                                                                          \
+    } while (0)
+
+static double rel_l2(const std::vector<cxf>& a, const std::vector<cxd>& b) {
+    double num = 0, den = 0;
+    for (size_t i = 0; i < a.size(); ++i) {
+        num += std::norm(cxd(a[i]) - b[i]);
+        den += std::norm(b[i]);
+    }
+    return std::sqrt(num / (den > 0 ? den : 1));
+}
+
+// tests/test_pcx.hpp:5-27 equal_eps with k epsilons
+static bool equal_eps(cxd a, cxd b, double k) {
+    if (a == b)
+        return true;
+    double largest = std::max(std::abs(a), std::abs(b));
+    return std::abs(a - b) < largest * k * std::numeric_limits<float>::epsilon();
+}
+
+static std::vector<cxd> oracle_fft(int order, bool inverse, bool normalized, const std::vector<cxf>& in) {
+    int64_t            n = (int64_t)in.size();
+    std::vector<float> src(2 * n), dst(2 * n);
+    for (int64_t i = 0; i < n; ++i) {
+        src[orc_pidx(i, n)]     = in[i].real();
+        src[orc_pidx(i, n) + n] = in[i].imag();
+    }
+    std::vector<double> re(n), im(n);
+    orc_fft_unit(order, inverse, normalized, n, dst.data(), n, src.data(), n, n, re.data(), im.data());
+    std::vector<cxd> out(n);
+    for (int64_t i = 0; i < n; ++i)
+        out[i] = {re[i], im[i]};
+    return out;
+}
+
+static std::vector<cxf> leaky_tone(size_t n) { // test_fft.cpp:299-302
+    const float      pi = 3.14159265358979323846f;
+    std::vector<cxf> v(n);
+    for (size_t i = 0; i < n; ++i)
+        v[i] = std::exp(cxf(0.F, 2 * pi * i / n * 13.37F));
+    return v;
+}
+
+template<size_t PackSize>
+static void test_fft_float(size_t size) {
+    using namespace pcx_gpu;
+    double depth = std::log2((double)size);
+    double tol   = 2 * depth * std::numeric_limits<float>::epsilon();
+    auto   x     = leaky_tone(size);
+    auto   want  = oracle_fft(0, false, true, x);
+
+    vector<float, PackSize> vec(x);
+    vector<float, PackSize> vec_out(size);
+    fft_unit<float, fft_order::normal> unit(size, 2048);
+    unit(vec_out, vec); // out of place
+    auto got = vec_out.to_host();
+    CHECK(rel_l2(got, want) < tol, "normal oop N=%zu P=%zu err=%g", size, PackSize, rel_l2(got, want));
+    int bad = 0;
+    for (size_t i = 0; i < size; ++i)
+        bad += !equal_eps(cxd(got[i]), want[i], (double)size); // reference tolerance k = N (test_fft.cpp:363)
+    CHECK(bad == 0, "normal N=%zu P=%zu: %d elements outside N*eps", size, PackSize, bad);
+
+    unit(vec); // in place
+    got = vec.to_host();
+    CHECK(rel_l2(got, want) < tol, "normal in-place N=%zu P=%zu", size, PackSize);
+    unit.ifft(vec); // round trip, normalised
+    got = vec.to_host();
+    std::vector<cxd> xd(x.begin(), x.end());
+    CHECK(rel_l2(got, xd) < 2 * tol, "ifft round trip N=%zu P=%zu err=%g", size, PackSize, rel_l2(got, xd));
+
+    // pack-1 (std::complex interleaved) destination from the pack-P source
+    vector<float, 1> svec(size);
+    unit(svec, vector<float, PackSize>(x));
+    got = svec.to_host();
+    CHECK(rel_l2(got, want) < tol, "pack-1 destination N=%zu", size);
+}
+
+template<size_t PackSize>
+static void test_fftu_float(size_t size) {
+    using namespace pcx_gpu;
+    double tol = 2 * std::log2((double)size) * std::numeric_limits<float>::epsilon();
+    auto   x   = leaky_tone(size);
+    {
+        auto                                     want = oracle_fft(1, false, true, x);
+        vector<float, PackSize>                  vec(x), out(size);
+        fft_unit<float, fft_order::bit_reversed> unit(size);
+        unit(out, vec);
+        auto got = out.to_host();
+        CHECK(rel_l2(got, want) < tol, "bit_reversed N=%zu err=%g", size, rel_l2(got, want));
+        unit.fft_raw<PackSize>(vec.data()); // fft_raw vs operator(): identical
+        auto raw = vec.to_host();
+        CHECK(raw == got, "bit_reversed fft_raw != operator() N=%zu", size);
+        unit.template ifft<true>(vec);
+        std::vector<cxd> xd(x.begin(), x.end());
+        CHECK(rel_l2(vec.to_host(), xd) < 2 * tol, "bit_reversed round trip N=%zu", size);
+    }
+    {
+        auto                                  want = oracle_fft(2, false, true, x);
+        vector<float, PackSize>               vec(x), out(size);
+        fft_unit<float, fft_order::unordered> unit(size);
+        unit(out, vec);
+        auto got = out.to_host();
+        CHECK(rel_l2(got, want) < tol, "unordered N=%zu err=%g", size, rel_l2(got, want));
+        // unnormalised inverse: N * x
+        vector<float, PackSize> back(size);
+        unit.template ifft<false>(back, out);
+        auto             r = back.to_host();
+        std::vector<cxd> nx(size);
+        for (size_t i = 0; i < size; ++i)
+            nx[i] = cxd(x[i]) * (double)size;
+        CHECK(rel_l2(r, nx) < 2 * tol, "unordered unnormalised inverse N=%zu", size);
+    }
+}
+
+static void test_zero_extension() {
+    using namespace pcx_gpu;
+    size_t                                   n = 1024, m = 700;
+    auto                                     x = leaky_tone(m);
+    vector<float, 8>                         src(x), dst(n);
+    fft_unit<float, fft_order::bit_reversed> unit(n);
+    unit(dst, src); // source shorter than the transform: zero-extended
+    std::vector<cxf> xz(n, cxf(0, 0));
+    for (size_t i = 0; i < m; ++i)
+        xz[i] = x[i];
+    auto want = oracle_fft(1, false, true, xz);
+    CHECK(rel_l2(dst.to_host(), want) < 2 * 10 * std::numeric_limits<float>::epsilon(), "zero extension");
+}
+
+static void test_errors() {
+    using namespace pcx_gpu;
+    try {
+        fft_unit<float> u(1000);
+        CHECK(false, "size 1000 accepted");
+    } catch (const std::invalid_argument& e) {
+        CHECK(std::string(e.what()) == "fft_size (which is  1000) is not an integer power of two", "msg: %s", e.what());
+    }
+    try {
+        fft_unit<float> u(1024, 4);
+        CHECK(false, "sub_size 4 accepted");
+    } catch (const std::invalid_argument& e) {
+        CHECK(std::string(e.what()) == "fft_sub_size (which is  4) is smaller than minimum (which is 8)", "msg: %s",
+              e.what());
+    }
+    fft_unit<float>  u(256);
+    vector<float, 8> v(128);
+    try {
+        u(v);
+        CHECK(false, "size mismatch accepted");
+    } catch (const std::invalid_argument& e) {
+        CHECK(std::string(e.what()) == "input size (which is 128) is not equal to fft size (which is 256)", "msg: %s",
+              e.what());
+    }
+    vector<float, 16>  a(256);
+    vector<float, 256> b(256);
+    try {
+        u.fft_raw<16, 256>(a.data(), a.data());
+        CHECK(false, "in-place repack accepted");
+    } catch (const std::invalid_argument& e) {
+        CHECK(std::string(e.what()).find("cannot perform in-place repack") == 0, "msg: %s", e.what());
+    }
+}
+
+static void test_fxcorr() {
+    using namespace pcx_gpu;
+    // tests/test_fxcorr.cpp:100-139: Barker-13 kernel, f = i*(Barker at 0 and 109) + rand()%7 - 3
+    const double       bark[13] = {1, 1, 1, 1, 1, -1, -1, 1, 1, -1, 1, -1, 1};
+    size_t             size     = 128;
+    std::vector<cxf>   g(size, 0), f(size, 0);
+    for (int i = 0; i < 13; ++i) {
+        g[i] = (float)bark[i];
+        f[i] += (float)bark[i];
+        f[109 + i] += (float)bark[i];
+    }
+    orc_glibc_rand_t rs;
+    orc_glibc_srand(&rs, 1);
+    for (size_t i = 0; i < size; ++i)
+        f[i] = f[i] * cxf(0, 1) + (float)(orc_glibc_rand(&rs) % 7 - 3);
+    std::vector<double> fr(size), fi(size), gr(size), gi(size), outr(size), outi(size);
+    for (size_t i = 0; i < size; ++i)
+        fr[i] = f[i].real(), fi[i] = f[i].imag(), gr[i] = g[i].real(), gi[i] = g[i].imag();
+    orc_xcorr_naive(fr.data(), fi.data(), size, gr.data(), gi.data(), size, outr.data(), outi.data());
+    vector<float>        gv(g), fv(f);
+    fxcorr_unit<float>   xcr(gv, size * 2);
+    xcr(fv);
+    auto             got = fv.to_host();
+    std::vector<cxd> want(size);
+    for (size_t i = 0; i < size; ++i)
+        want[i] = {outr[i], outi[i]};
+    CHECK(rel_l2(got, want) < 1e-6, "fxcorr Barker case err=%g", rel_l2(got, want));
+    CHECK(std::abs(got[109].imag() - 13.f) < 0.5f && std::abs(got[0].imag() - 13.f) < 0.5f, "Barker peaks");
+
+    // multi-block, out of place (the reference overload is an empty stub)
+    size_t              L = 20000, M = 300, N = 1024;
+    std::vector<float>  sr(L), si(L), kr(M), ki(M);
+    orc_gen_normal(0x5EED0003, 0.7071067811865476, sr.data(), si.data(), L);
+    orc_gen_uniform(0x5EED0013, kr.data(), ki.data(), M);
+    std::vector<cxf> sig(L), ker(M);
+    for (size_t i = 0; i < L; ++i)
+        sig[i] = {sr[i], si[i]};
+    for (size_t i = 0; i < M; ++i)
+        ker[i] = std::polar(1.0f, 3.14159265f * kr[i]);
+    std::vector<double> a(L), b(L), c(M), d(M), o1(L), o2(L);
+    for (size_t i = 0; i < L; ++i)
+        a[i] = sig[i].real(), b[i] = sig[i].imag();
+    for (size_t i = 0; i < M; ++i)
+        c[i] = ker[i].real(), d[i] = ker[i].imag();
+    orc_xcorr_naive(a.data(), b.data(), L, c.data(), d.data(), M, o1.data(), o2.data());
+    vector<float>      sv(sig), out(L), kv(ker);
+    fxcorr_unit<float> xc2(kv, N);
+    xc2(out, sv);
+    auto got2 = out.to_host();
+    std::vector<cxd> want2(L);
+    for (size_t i = 0; i < L; ++i)
+        want2[i] = {o1[i], o2[i]};
+    CHECK(rel_l2(got2, want2) < 4e-6, "fxcorr multi-block err=%g", rel_l2(got2, want2));
+    CHECK(xc2.step() == (N - (M - 1)) / 8 * 8, "fxcorr step");
+}
+
+int main() {
+    for (size_t n = 64; n <= 16384; n *= 2) { // test_fft.cpp:779-798 runs 2^6 .. 2^14
+        test_fft_float<8>(n);
+        test_fft_float<1>(n);
+        test_fftu_float<8>(n);
+    }
+    test_fft_float<8>(1 << 16);
+    test_fftu_float<8>(1 << 16);
+    test_zero_extension();
+    test_errors();
+    test_fxcorr();
+    std::printf("%s: %d failed checks\n", g_fail ? "FAIL" : "OK", g_fail);
+    return g_fail;
+}

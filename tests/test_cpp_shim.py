@@ -1,0 +1,27 @@
+"""Runs the C++ drop-in test (tests/cpp/test_fft_shim.cpp, built by __graft_entry__.build()):
+the pcx_gpu::fft_unit / fxcorr_unit classes with the reference's template signatures, checked
+against the FP64 oracle like the reference's own tests/test_fft.cpp and tests/test_fxcorr.cpp."""
+import os
+import subprocess
+
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+BIN = os.path.join(HERE, "cpp", "test_fft_shim")
+
+
+def test_shim_header_compiles_standalone(tmp_path):
+    """The header needs only pcx_gpu.h and a C++20 compiler (no HIP headers)."""
+    root = os.path.dirname(HERE)
+    src = tmp_path / "t.cpp"
+    src.write_text('#include "pcx_gpu/fft.hpp"\nint main(){ pcx_gpu::fft_unit<float> (*f)(size_t) = nullptr; (void)f; }\n')
+    subprocess.run(["g++", "-std=c++20", "-fsyntax-only", f"-I{root}/include", f"-I{root}/packed_amd/include",
+                    str(src)], check=True)
+
+
+@pytest.mark.gpu
+def test_cpp_drop_in():
+    assert os.path.exists(BIN), "build the C++ test first: python __graft_entry__.py"
+    res = subprocess.run([BIN], capture_output=True, text=True, timeout=300)
+    print(res.stdout[-4000:])
+    assert res.returncode == 0, res.stdout[-4000:] + res.stderr[-2000:]
