@@ -35,8 +35,12 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the host-core baseline")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--quick", action="store_true", help="time the forward step only (for profiler runs)")
     ap.add_argument("--log2n", type=int, default=N_LOG2)
     ap.add_argument("--batch", type=int, default=BATCH)
+    ap.add_argument("--config", type=int, default=2, choices=[1, 2, 3, 4, 5],
+                    help="2: the headline batched FFT (default); 1: N=1024 latency; 3: fxcorr; "
+                         "4: uneven batch; 5: split-precision MFMA")
     return ap.parse_args()
 
 
@@ -102,6 +106,9 @@ def main():
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported", file=sys.stderr)
         sys.exit(2)
+    if args.config != 2:
+        print(json.dumps(CONFIGS[args.config](args)))
+        return
     import numpy as np
     import torch
 
@@ -133,6 +140,9 @@ def main():
         fwd_br.batch(z.data_ptr(), x.data_ptr(), batch, n, 2 * n, n, 2 * n)
 
     ms = time_steps(step, args.steps, args.warmup)
+    if args.quick:
+        print(json.dumps({"ms_per_step": round(ms, 4)}))
+        return
     ms_inv = time_steps(step_inv, max(3, args.steps // 2), 1)
     ms_br = time_steps(step_br, max(3, args.steps // 2), 1)
 
@@ -189,6 +199,122 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(n, args.cpu_seconds)
     print(json.dumps(out))
+
+
+# ---------------------------------------------------------------------------- other workloads
+def run_config1(args):
+    """N=1024 single FP32 FFT, pack 8, forward then inverse: latency, order exactness, round trip."""
+    import numpy as np
+    import torch
+
+    import packed_amd as pa
+    from oracle import oracle as O
+
+    n = 1024
+    x = O.gen_uniform(0x5EED0001, n)
+    res = {"metric": "N=1024 single FFT forward+inverse latency (pack 8)", "unit": "us", "higher_is_better": False,
+           "config": {"workload": "single C2C FFT N=1024, pack 8, forward then inverse"}, "orders": {}}
+    for order in ("normal", "bit_reversed", "unordered"):
+        unit = pa.fft_unit(n, order=order)
+        v = pa.vector.from_complex(x, 8)
+        w = pa.vector(n, 8)
+        unit(w, v)
+        torch.cuda.synchronize()
+        _, want = O.fft_unit(order, False, True, n, O.to_packed(x, 8), 8, 8)
+        err = O.rel_l2(w.to_complex(), want)
+        unit.ifft(v, w)
+        torch.cuda.synchronize()
+        rt = O.rel_l2(v.to_complex(), x)
+
+        def pair():
+            unit(w, v)
+            unit.ifft(v, w)
+
+        us = time_steps(pair, 2000, 50) * 1e3
+        res["orders"][order] = {"fwd_inv_us": round(us, 2), "rel_l2_vs_fp64": float(f"{err:.3e}"),
+                                "round_trip_rel_l2": float(f"{rt:.3e}")}
+    res["value"] = res["orders"]["normal"]["fwd_inv_us"]
+    return res
+
+
+def run_config3(args):
+    """Overlap-save cross-correlation: 2^24 complex samples, 16 copies of a 4096-tap unit-modulus
+    kernel at SNR 0 dB, FFT block 16384 (step 12288, 1366 blocks), in place and out of place."""
+    import numpy as np
+    import torch
+
+    import packed_amd as pa
+    from oracle import oracle as O
+
+    L, M, N = 1 << 24, 4096, 16384
+    f = O.gen_normal(0x5EED0003, L, math.sqrt(0.5))
+    ph = O.gen_uniform(0x5EED0013, M)
+    g = np.exp(1j * np.pi * (np.real(ph).astype(np.float64) + 1.0)).astype(np.complex64)
+    rng = np.random.default_rng(0x5EED0003)
+    offs = np.sort(rng.choice(np.arange(0, L - M, 2 * M), 16, replace=False))
+    for o in offs:
+        f[o:o + M] += g
+    gv = pa.vector.from_complex(g, 8)
+    xc = pa.fxcorr_unit(gv, N)
+    src = pa.vector.from_complex(f, 8)
+    work = pa.vector(L, 8)
+    out = pa.vector(L, 8)
+    step = xc.step(8)
+    blocks = -(-L // step)
+
+    def inplace():
+        work.data.copy_(src.data)
+        xc(work)
+
+    def copy_only():
+        work.data.copy_(src.data)
+
+    def oop():
+        xc(src, out)
+
+    ms_oop = time_steps(oop, max(5, args.steps), 2)
+    ms_in = time_steps(inplace, max(5, args.steps), 2) - time_steps(copy_only, max(5, args.steps), 2)
+    res_out = out.to_complex()
+    top = np.sort(np.argsort(np.abs(res_out))[-16:])
+    want = O.xcorr_naive(f[: 1 << 16].astype(np.complex128), g.astype(np.complex128))[: (1 << 16) - M]
+    err = O.rel_l2(res_out[: (1 << 16) - M], want)
+    inplace()
+    torch.cuda.synchronize()
+    same = float(np.max(np.abs(work.to_complex() - res_out)))
+    flops = blocks * (2 * 5 * N * math.log2(N) + 6 * N)
+    algo_bytes = 16 * L + 8 * N
+    # host-core baseline: the oracle's FP32 port on a 2^20-sample slice of the same signal
+    threads = max(1, min(os.cpu_count() or 1, 16))
+    fs = f[: 1 << 20]
+    O.port_fxcorr(fs, g, N, threads)
+    t = time.perf_counter()
+    _, nb = O.port_fxcorr(fs, g, N, threads)
+    dt = time.perf_counter() - t
+    cpu_flops = nb * (2 * 5 * N * math.log2(N) + 6 * N)
+    return {
+        "metric": "fxcorr GFLOP/s (blocks*(2*5N*log2N+6N)), L=2^24, M=4096, N=16384",
+        "value": round(flops / (ms_oop * 1e-3) / 1e9, 1), "unit": "GFLOP/s", "higher_is_better": True,
+        "ms_out_of_place": round(ms_oop, 4), "ms_in_place": round(ms_in, 4), "blocks": blocks, "step": step,
+        "msamples_per_s": round(L / (ms_oop * 1e-3) / 1e6, 1),
+        "roofline": {"bound": "hbm", "achieved": round(algo_bytes / (ms_oop * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(algo_bytes / (ms_oop * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+        "peaks_index_exact": bool(np.array_equal(top, offs)), "rel_l2_vs_fp64_first_2^16": float(f"{err:.3e}"),
+        "in_place_equals_out_of_place_maxabs": same,
+        "cpu_baseline": {"value": round(cpu_flops / dt / 1e9, 2), "unit": "GFLOP/s", "cores": threads, "kind": "port",
+                         "sample": f"2^20-sample slice ({nb} blocks), same kernel and block size, {threads} threads"},
+        "config": {"workload": "overlap-save cross-correlation, L=2^24, M=4096, N=16384, pack 8"},
+    }
+
+
+def run_config4(args):
+    raise SystemExit("config 4 (uneven batch) is not implemented yet")
+
+
+def run_config5(args):
+    raise SystemExit("config 5 (split-precision MFMA) is not implemented yet")
+
+
+CONFIGS = {1: run_config1, 3: run_config3, 4: run_config4, 5: run_config5}
 
 
 if __name__ == "__main__":

@@ -8,6 +8,9 @@
 //
 // Inverse transforms run as conj(FFT(conj(x))): conj_in / conj_out negate imaginary parts at
 // load and store, so every kernel only implements the forward butterflies.
+//
+// Global addressing: one 64-bit base pointer per row (real part) and one for the imaginary part
+// (base + P), then 32-bit packed offsets i + (i / P) * P (element_access.hpp:8-11).
 #pragma once
 #include "stockham.hpp"
 
@@ -24,13 +27,14 @@ struct RowArgs {
     int64_t       nrows;              // rows in the launch (transforms << rows_lg)
     int           src_lp, dst_lp;     // log2 of the pack size
     int           rows_lg;            // log2(rows per transform): 0 single pass, log2(N2) four-step
-    int           load_kind, store_kind, order;
+    int           order;
     int           conj_in, conj_out;
     float         scale;              // applied at the store
     const float2* twA;                // W_A, A = row length
-    const float2* twl;                // optional post-FFT twiddle W_N^{a*row} = twl[e & lmask] * twh[e >> ls]
+    const float2* twl;                // post-FFT twiddle W_N^{a*row} = twl[e & lmask] * twh[e >> ls]
     const float2* twh;
-    int           tw_ls;
+    int           tw_ls, tw_lg;       // ls and log2(N)
+    int           nt_load, nt_store;  // non-temporal (streaming) global loads / stores
 };
 
 struct ColArgs {
@@ -44,26 +48,103 @@ struct ColArgs {
     int           conj_in, conj_out;
     float         scale;
     const float2* twA;    // W_A, A = column length N2
-    const float2* twl;    // optional post-FFT twiddle W_N^{n1*a}
+    const float2* twl;    // post-FFT twiddle W_N^{n1*a}
     const float2* twh;
-    int           tw_ls;
+    int           tw_ls, tw_lg;
+    int           nt_load, nt_store;
 };
 
-__device__ __forceinline__ cf ld_cx(const float* __restrict__ base, int64_t i, int lp) {
-    int64_t o = pidx(i, lp);
-    return {base[o], base[o + ((int64_t)1 << lp)]};
+// 32-bit packed offset; lp is clamped so the shift stays defined (offsets are < 2^31).
+__device__ __forceinline__ uint32_t poff(uint32_t i, int lp) {
+    int l = lp < 30 ? lp : 30;
+    return i + ((i >> l) << l);
 }
-__device__ __forceinline__ void st_cx(float* __restrict__ base, int64_t i, int lp, cf v) {
-    int64_t o                       = pidx(i, lp);
-    base[o]                         = v.x;
-    base[o + ((int64_t)1 << lp)]    = v.y;
-}
-__device__ __forceinline__ cf tw2(const float2* __restrict__ twl, const float2* __restrict__ twh, int ls,
-                                  uint32_t e) {
-    return cmul(tw_load(twl, e & ((1u << ls) - 1)), tw_load(twh, e >> ls));
+__device__ __forceinline__ float ldf(const float* p, int nt) { return nt ? __builtin_nontemporal_load(p) : *p; }
+__device__ __forceinline__ void  stf(float* p, float v, int nt) {
+    if (nt)
+        __builtin_nontemporal_store(v, p);
+    else
+        *p = v;
 }
 
-// Default rows per workgroup: enough rows for 256 threads.
+// Re/im views of one packed vector.
+struct CxPtr {
+    const float* re;
+    const float* im;
+    __device__ CxPtr(const float* base, int lp) : re(base), im(base + ((int64_t)1 << lp)) {}
+    __device__ cf ld(uint32_t o, int nt) const { return {ldf(re + o, nt), ldf(im + o, nt)}; }
+};
+struct CxPtrW {
+    float* re;
+    float* im;
+    __device__ CxPtrW(float* base, int lp) : re(base), im(base + ((int64_t)1 << lp)) {}
+    __device__ void st(uint32_t o, cf v, int nt) const {
+        stf(re + o, v.x, nt);
+        stf(im + o, v.y, nt);
+    }
+};
+
+// Buffer-resource views (wave-uniform base in SGPRs, 32-bit byte offsets): one VGPR per address.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(const float* base) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(base), 0, 0x7FFFFFFF, 0x00020000);
+}
+struct BufR {
+    __amdgpu_buffer_rsrc_t re, im;
+    __device__ BufR(const float* base, int lp) : re(rsrc(base)), im(rsrc(base + ((int64_t)1 << lp))) {}
+    __device__ cf ld(uint32_t o, int) const {
+        return {__uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(re, o * 4u, 0, 0)),
+                __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(im, o * 4u, 0, 0))};
+    }
+};
+struct BufW {
+    __amdgpu_buffer_rsrc_t re, im;
+    __device__ BufW(float* base, int lp) : re(rsrc(base)), im(rsrc(base + ((int64_t)1 << lp))) {}
+    __device__ void st(uint32_t o, cf v, int) const {
+        __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v.x), re, o * 4u, 0, 0);
+        __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v.y), im, o * 4u, 0, 0);
+    }
+};
+template<bool UNI>
+using RdView = typename std::conditional<UNI, BufR, CxPtr>::type;
+template<bool UNI>
+using WrView = typename std::conditional<UNI, BufW, CxPtrW>::type;
+
+// LDS copy of the two-level W_N table for four-step pass twiddles, TWN complex entries
+// (lo + hi, TWN >= 4); TWN == 2 reads both levels from global memory instead.
+template<int TWN>
+struct PassTw {
+    static constexpr bool in_lds = TWN > 2;
+    __device__ static void fill(float* t, const float2* __restrict__ twl, const float2* __restrict__ twh, int ls,
+                                int lg, int tid, int nthreads) {
+        if constexpr (in_lds) {
+            int nl = 1 << ls, nh = 1 << (lg - ls);
+            for (int e = tid; e < nl; e += nthreads) {
+                float2 w  = twl[e];
+                t[e]      = w.x;
+                t[nl + e] = w.y;
+            }
+            for (int h = tid; h < nh; h += nthreads) {
+                float2 w           = twh[h];
+                t[2 * nl + h]      = w.x;
+                t[2 * nl + nh + h] = w.y;
+            }
+        }
+    }
+    __device__ static cf get(const float* t, const float2* __restrict__ twl, const float2* __restrict__ twh, int ls,
+                             int lg, uint32_t e) {
+        uint32_t lo = e & ((1u << ls) - 1), hi = e >> ls;
+        if constexpr (in_lds) {
+            int nl = 1 << ls, nh = 1 << (lg - ls);
+            return cmul(cf{t[lo], t[nl + lo]}, cf{t[2 * nl + hi], t[2 * nl + nh + hi]});
+        } else {
+            float2 a = twl[lo], b = twh[hi];
+            return cmul(cf{a.x, a.y}, cf{b.x, b.y});
+        }
+    }
+};
+// pass-twiddle table capacity for a transform of 2^lg points (0: none needed)
+constexpr int pass_tw_entries(int lg) { return (1 << ((lg + 1) / 2)) + (1 << (lg / 2)); }
+
 template<int LA>
 constexpr int default_rows() {
     return Plan<LA>::TA >= 256 ? 1 : 256 / Plan<LA>::TA;
@@ -76,72 +157,124 @@ struct RowShape {
     static constexpr int NT = G * TA;
     // odd row stride: rows of a strided access land in distinct banks
     static constexpr int STRIDE = (1 << LA) + ((1 << LA) >> 5) + 1;
+    static constexpr int DATA   = 2 * G * STRIDE;
 };
 
-template<int LA, int G_>
-__global__ __launch_bounds__(G_ * Plan<LA>::TA) void k_row(RowArgs p) {
-    using P             = Plan<LA>;
-    using S             = RowShape<LA, G_>;
-    constexpr int A     = P::A;
-    constexpr int TA    = P::TA;
-    constexpr int PPT   = P::PPT;
-    constexpr int G     = S::G;
-    constexpr int NT    = S::NT;
-    constexpr int STR   = S::STRIDE;
-    constexpr int R0    = P::first_radix;
-    constexpr int RL    = P::last_radix;
-    __shared__ float lre[G * STR];
-    __shared__ float lim[G * STR];
+// Waves per SIMD that the LDS footprint allows (argument of __launch_bounds__).
+constexpr int waves_per_simd(int threads, int lds_bytes) {
+    int wg = (160 * 1024) / (lds_bytes > 0 ? lds_bytes : 1);
+    if (wg > 8)
+        wg = 8;
+    if (wg < 1)
+        wg = 1;
+    int w = wg * ((threads + 63) / 64) / 4;
+    return w < 1 ? 1 : (w > 8 ? 8 : w);
+}
 
-    const int     t       = threadIdx.x;
-    const int     g       = t / TA;
-    const int     jt      = t % TA;
-    const int64_t row0    = (int64_t)blockIdx.x * G;
-    const int64_t row     = row0 + g;
-    const bool    row_ok  = row < p.nrows;
-    const int64_t rmask   = ((int64_t)1 << p.rows_lg) - 1;
-    const int64_t bt      = row >> p.rows_lg;
-    const int64_t k2      = row & rmask;
-    auto          addr    = [&](int a) { return g * STR + a + (a >> 5); };
-    const float*  sbase   = p.src + bt * p.src_dist;
-    float*        dbase   = p.dst + bt * p.dst_dist;
+// One workgroup's group of G rows (rows wg*G .. wg*G + G - 1).
+// UNI: the rows of a wavefront share one transform (TA >= 64, or all rows of the workgroup lie
+// in one transform), so its base address is wave-uniform and buffer addressing applies.
+// Element held by local slot s = jt + u*TA of a permuted (bit_reversed / unordered) row.  When a
+// thread's slots differ only in their high bits (TA >= 32: q touches the low five bits, which
+// belong to jt) this is base(jt) + brev(u), a compile-time offset from one address.
+template<int LA, int ORD>
+__device__ __forceinline__ int perm_base(int jt) {
+    using P           = Plan<LA>;
+    constexpr int LT  = ilog2c(P::TA);
+    constexpr int LPP = ilog2c(P::PPT);
+    uint32_t      j   = ORD == ORD_UNORD ? uq((uint32_t)jt) : (uint32_t)jt;
+    return (int)(brev_bits(j, LT) << LPP);
+}
+template<int LA, int ORD>
+__device__ __forceinline__ int perm_elem(int jt, int u) {
+    using P = Plan<LA>;
+    if constexpr (P::TA >= 32)
+        return perm_base<LA, ORD>(jt) + brevc(u, ilog2c(P::PPT));
+    else
+        return (int)slot_freq(ORD, (uint32_t)(jt + u * P::TA), LA);
+}
+
+template<int LA, int G, int LD, int ST, int TW, bool UNI, int ORD>
+__device__ __forceinline__ void row_group(const RowArgs& p, int64_t wg, float* lre, float* lim, const float* ltw,
+                                          const float* lptw) {
+    using P           = Plan<LA>;
+    using S           = RowShape<LA, G>;
+    constexpr int TA  = P::TA;
+    constexpr int PPT = P::PPT;
+    constexpr int NT  = S::NT;
+    constexpr int STR = S::STRIDE;
+    constexpr int R0  = P::first_radix;
+    constexpr int RL  = P::last_radix;
+
+    const int      t      = threadIdx.x;
+    const int      g      = t / TA;
+    const int      jt     = t % TA;
+    const int64_t  row0   = wg * G;
+    // a wavefront holds one row when TA >= 64: keep the row (and its base address) in SGPRs
+    const int64_t  row    = TA >= 64 ? row0 + __builtin_amdgcn_readfirstlane(g) : row0 + g;
+    const bool     row_ok = row < p.nrows;
+    const int64_t  rmask  = ((int64_t)1 << p.rows_lg) - 1;
+    const int64_t  bt     = row >> p.rows_lg;
+    const uint32_t k2     = (uint32_t)(row & rmask);
+    // transform of this thread's row; with UNI the same for the whole wavefront
+    const int64_t  btu    = (UNI && TA < 64) ? (row0 >> p.rows_lg) : bt;
+    auto           addr   = [&](int b, int o) { return g * STR + b + (b >> 5) + o + (o >> 5); };
 
     cf v[PPT];
     // ------------------------------------------------------------------ load
-    if (p.load_kind == LD_CONTIG) {
+    if constexpr (LD == LD_CONTIG) {
+        const RdView<UNI> src(p.src + btu * p.src_dist, p.src_lp);
+        if (row_ok && (int64_t)(k2 + 1) << LA <= p.src_size) { // whole row inside the source
 #pragma unroll
-        for (int m = 0; m < PPT / R0; ++m)
+            for (int m = 0; m < PPT / R0; ++m)
 #pragma unroll
-            for (int r = 0; r < R0; ++r) {
-                int64_t i        = in_elem<LA>(jt, m, r) + (k2 << LA);
-                v[m * R0 + r]    = (row_ok && i < p.src_size) ? ld_cx(sbase, i, p.src_lp) : cf{0.f, 0.f};
-            }
+                for (int r = 0; r < R0; ++r) {
+                    uint32_t i    = (uint32_t)in_elem<LA>(jt, m, r) + (k2 << LA);
+                    v[m * R0 + r] = src.ld(poff(i, p.src_lp), p.nt_load);
+                }
+        } else {
+#pragma unroll
+            for (int m = 0; m < PPT / R0; ++m)
+#pragma unroll
+                for (int r = 0; r < R0; ++r) {
+                    uint32_t i    = (uint32_t)in_elem<LA>(jt, m, r) + (k2 << LA);
+                    bool     ok   = row_ok && i < p.src_size;
+                    cf       x    = src.ld(ok ? poff(i, p.src_lp) : 0u, p.nt_load);
+                    v[m * R0 + r] = ok ? x : cf{0.f, 0.f};
+                }
+        }
     } else {
-        if (p.load_kind == LD_PERM) {
+        if constexpr (LD == LD_PERM) {
             // this row's spectrum occupies slots [A*brev(k2), A*brev(k2) + A); local slot s holds
             // element slot_freq(order, s)
-            int64_t base = (int64_t)brev_bits((uint32_t)k2, p.rows_lg) << LA;
+            const RdView<UNI> src(p.src + btu * p.src_dist, p.src_lp);
+            uint32_t          base = brev_bits(k2, p.rows_lg) << LA;
 #pragma unroll
             for (int u = 0; u < PPT; ++u) {
-                int s     = jt + u * TA;
-                int e     = (int)slot_freq(p.order, (uint32_t)s, LA);
-                cf  x     = row_ok ? ld_cx(sbase, base + s, p.src_lp) : cf{0.f, 0.f};
-                int o     = addr(e);
-                lre[o]    = x.x;
-                lim[o]    = x.y;
+                int s  = jt + u * TA;
+                cf  x  = src.ld(row_ok ? poff(base + s, p.src_lp) : 0u, p.nt_load);
+                x      = row_ok ? x : cf{0.f, 0.f};
+                int o;
+                if constexpr (TA >= 32)
+                    o = addr(perm_base<LA, ORD>(jt), brevc(u, ilog2c(PPT)));
+                else
+                    o = addr(perm_elem<LA, ORD>(jt, u), 0);
+                lre[o] = x.x;
+                lim[o] = x.y;
             }
         } else { // LD_STRIDED: element k1 of row k2 sits at slot k2 + (k1 << rows_lg)
+            // (the rows of a workgroup lie in one transform: G divides the rows per transform)
+            const RdView<UNI> src(p.src + (row0 >> p.rows_lg) * p.src_dist, p.src_lp);
 #pragma unroll
             for (int u = 0; u < PPT; ++u) {
-                int     idx  = t + u * NT;
-                int     gg   = idx % G;
-                int     k1   = idx / G;
-                int64_t rw   = row0 + gg;
-                cf      x    = {0.f, 0.f};
-                if (rw < p.nrows) {
-                    const float* b = p.src + (rw >> p.rows_lg) * p.src_dist;
-                    x              = ld_cx(b, (rw & rmask) + ((int64_t)k1 << p.rows_lg), p.src_lp);
-                }
+                int     idx = t + u * NT;
+                int     gg  = idx % G;
+                int     k1  = idx / G;
+                int64_t rw  = row0 + gg;
+                bool    ok  = rw < p.nrows;
+                cf      x   = src.ld(ok ? poff((uint32_t)(rw & rmask) + ((uint32_t)k1 << p.rows_lg), p.src_lp) : 0u,
+                                     p.nt_load);
+                x           = ok ? x : cf{0.f, 0.f};
                 int o  = gg * STR + k1 + (k1 >> 5);
                 lre[o] = x.x;
                 lim[o] = x.y;
@@ -152,7 +285,7 @@ __global__ __launch_bounds__(G_ * Plan<LA>::TA) void k_row(RowArgs p) {
         for (int m = 0; m < PPT / R0; ++m)
 #pragma unroll
             for (int r = 0; r < R0; ++r) {
-                int o         = addr(in_elem<LA>(jt, m, r));
+                int o         = addr(jt + m * TA, r * (P::A / R0));
                 v[m * R0 + r] = cf{lre[o], lim[o]};
             }
     }
@@ -163,127 +296,210 @@ __global__ __launch_bounds__(G_ * Plan<LA>::TA) void k_row(RowArgs p) {
     }
     // ------------------------------------------------------------------ transform
     // (the barrier before the first stage's LDS writes also orders them after the reads above)
-    stockham<LA, false>(v, lre, lim, jt, addr, p.twA);
+    stockham<LA, false>(v, lre, lim, jt, addr, ltw);
 
-    if (p.twl) {
+    if constexpr (TW > 0) {
 #pragma unroll
         for (int m = 0; m < PPT / RL; ++m)
 #pragma unroll
             for (int r = 0; r < RL; ++r) {
-                uint32_t e    = (uint32_t)out_elem<LA>(jt, m, r) * (uint32_t)k2;
-                v[m * RL + r] = cmul(v[m * RL + r], tw2(p.twl, p.twh, p.tw_ls, e));
+                uint32_t e    = (uint32_t)out_elem<LA>(jt, m, r) * k2;
+                v[m * RL + r] = cmul(v[m * RL + r], PassTw<TW>::get(lptw, p.twl, p.twh, p.tw_ls, p.tw_lg, e));
+                __builtin_amdgcn_sched_barrier(0); // one element's table reads in flight at a time
             }
     }
-    {
-        float sy = p.conj_out ? -p.scale : p.scale;
+    if (p.scale != 1.0f) {
 #pragma unroll
         for (int q = 0; q < PPT; ++q) {
             v[q].x *= p.scale;
-            v[q].y *= sy;
+            v[q].y *= p.scale;
         }
     }
+    if (p.conj_out) {
+#pragma unroll
+        for (int q = 0; q < PPT; ++q)
+            v[q].y = -v[q].y;
+    }
     // ------------------------------------------------------------------ store
-    if (p.store_kind == ST_CONTIG) {
+    if constexpr (ST == ST_CONTIG) {
         if (row_ok) {
+            const WrView<UNI> dst(p.dst + btu * p.dst_dist, p.dst_lp);
 #pragma unroll
             for (int m = 0; m < PPT / RL; ++m)
 #pragma unroll
                 for (int r = 0; r < RL; ++r) {
-                    int64_t i = out_elem<LA>(jt, m, r) + (k2 << LA);
-                    st_cx(dbase, i, p.dst_lp, v[m * RL + r]);
+                    uint32_t i = (uint32_t)out_elem<LA>(jt, m, r) + (k2 << LA);
+                    dst.st(poff(i, p.dst_lp), v[m * RL + r], p.nt_store);
                 }
         }
-        return;
-    }
-    __syncthreads(); // the last stage's LDS reads are done
+    } else {
+        __syncthreads(); // the last stage's LDS reads are done
 #pragma unroll
-    for (int m = 0; m < PPT / RL; ++m)
+        for (int m = 0; m < PPT / RL; ++m)
 #pragma unroll
-        for (int r = 0; r < RL; ++r) {
-            int o  = addr(out_elem<LA>(jt, m, r));
-            lre[o] = v[m * RL + r].x;
-            lim[o] = v[m * RL + r].y;
-        }
-    __syncthreads();
-    if (p.store_kind == ST_PERM) {
-        if (row_ok) {
-            int64_t base = (int64_t)brev_bits((uint32_t)k2, p.rows_lg) << LA;
+            for (int r = 0; r < RL; ++r) {
+                int o  = addr(jt + m * TA, r * (P::A / RL));
+                lre[o] = v[m * RL + r].x;
+                lim[o] = v[m * RL + r].y;
+            }
+        __syncthreads();
+        if constexpr (ST == ST_PERM) {
+            if (row_ok) {
+                const WrView<UNI> dst(p.dst + btu * p.dst_dist, p.dst_lp);
+                uint32_t          base = brev_bits(k2, p.rows_lg) << LA;
+#pragma unroll
+                for (int u = 0; u < PPT; ++u) {
+                    int s = jt + u * TA;
+                    int o;
+                    if constexpr (TA >= 32)
+                        o = addr(perm_base<LA, ORD>(jt), brevc(u, ilog2c(PPT)));
+                    else
+                        o = addr(perm_elem<LA, ORD>(jt, u), 0);
+                    dst.st(poff(base + s, p.dst_lp), cf{lre[o], lim[o]}, p.nt_store);
+                }
+            }
+        } else { // ST_STRIDED: element k1 of row k2 goes to slot k2 + (k1 << rows_lg)
+            const WrView<UNI> dst(p.dst + (row0 >> p.rows_lg) * p.dst_dist, p.dst_lp);
 #pragma unroll
             for (int u = 0; u < PPT; ++u) {
-                int s = jt + u * TA;
-                int o = addr((int)slot_freq(p.order, (uint32_t)s, LA));
-                st_cx(dbase, base + s, p.dst_lp, cf{lre[o], lim[o]});
-            }
-        }
-    } else { // ST_STRIDED: element k1 of row k2 goes to slot k2 + (k1 << rows_lg)
-#pragma unroll
-        for (int u = 0; u < PPT; ++u) {
-            int     idx = t + u * NT;
-            int     gg  = idx % G;
-            int     k1  = idx / G;
-            int64_t rw  = row0 + gg;
-            if (rw < p.nrows) {
-                int o    = gg * STR + k1 + (k1 >> 5);
-                float* b = p.dst + (rw >> p.rows_lg) * p.dst_dist;
-                st_cx(b, (rw & rmask) + ((int64_t)k1 << p.rows_lg), p.dst_lp, cf{lre[o], lim[o]});
+                int     idx = t + u * NT;
+                int     gg  = idx % G;
+                int     k1  = idx / G;
+                int64_t rw  = row0 + gg;
+                if (rw < p.nrows) {
+                    int o = gg * STR + k1 + (k1 >> 5);
+                    dst.st(poff((uint32_t)(rw & rmask) + ((uint32_t)k1 << p.rows_lg), p.dst_lp), cf{lre[o], lim[o]},
+                           p.nt_store);
+                }
             }
         }
     }
 }
 
-template<int LA, int C>
-__global__ __launch_bounds__(C * Plan<LA>::TA) void k_col(ColArgs p) {
+template<int LA, int G, int TW>
+constexpr int row_lds_floats() {
+    return RowShape<LA, G>::DATA + TwTable<LA>::FLOATS + 2 * TW;
+}
+
+// Grid-stride over row groups: gridDim.x may be smaller than the number of groups.
+template<int LA, int G, int LD, int ST, int TW, bool UNI, int ORD>
+__global__ __launch_bounds__(G * Plan<LA>::TA, waves_per_simd(G * Plan<LA>::TA, 4 * row_lds_floats<LA, G, TW>()))
+void k_row(RowArgs p) {
+    constexpr int    NT = G * Plan<LA>::TA;
+    __shared__ float lds[row_lds_floats<LA, G, TW>()];
+    float*           lre  = lds;
+    float*           lim  = lds + RowShape<LA, G>::DATA / 2;
+    float*           ltw  = lds + RowShape<LA, G>::DATA;
+    float*           lptw = ltw + TwTable<LA>::FLOATS;
+    TwTable<LA>::fill(ltw, p.twA, threadIdx.x, NT);
+    if constexpr (TW > 0)
+        PassTw<TW>::fill(lptw, p.twl, p.twh, p.tw_ls, p.tw_lg, threadIdx.x, NT);
+    __syncthreads();
+    const int64_t nwg = (p.nrows + G - 1) / G;
+    for (int64_t wg = blockIdx.x; wg < nwg; wg += gridDim.x) {
+        row_group<LA, G, LD, ST, TW, UNI, ORD>(p, wg, lre, lim, ltw, lptw);
+        __syncthreads(); // LDS is reused by the next group
+    }
+}
+
+template<int LA, int C, int TW>
+__device__ __forceinline__ void col_tile(const ColArgs& p, int64_t tile, float* lre, float* lim, const float* ltw,
+                                         const float* lptw) {
     using P           = Plan<LA>;
     constexpr int PPT = P::PPT;
     constexpr int R0  = P::first_radix;
     constexpr int RL  = P::last_radix;
-    constexpr int A   = P::A;
-    __shared__ float lre[A * C];
-    __shared__ float lim[A * C];
 
-    const int     t     = threadIdx.x;
-    const int     c     = t % C;
-    const int     jt    = t / C;
-    const int     ctl   = p.n1_lg - ilog2c(C); // log2(column tiles per transform)
-    const int64_t tile  = blockIdx.x;
-    const int64_t bt    = tile >> ctl;
-    const int64_t n1    = ((tile & (((int64_t)1 << ctl) - 1)) * C) + c;
-    auto          addr  = [&](int a) { return a * C + c; };
-    const float*  sbase = p.src + bt * p.src_dist;
-    float*        dbase = p.dst + bt * p.dst_dist;
+    const int      t    = threadIdx.x;
+    const int      c    = t % C;
+    const int      jt   = t / C;
+    const int      ctl  = p.n1_lg - ilog2c(C); // log2(column tiles per transform)
+    const int64_t  bt   = tile >> ctl;
+    const uint32_t n1   = (uint32_t)((tile & (((int64_t)1 << ctl) - 1)) * C) + c;
+    auto           addr = [&](int b, int o) { return (b + o) * C + c; };
+    const BufR     src(p.src + bt * p.src_dist, p.src_lp);
+    const BufW     dst(p.dst + bt * p.dst_dist, p.dst_lp);
 
     cf v[PPT];
+    if (((int64_t)1 << (p.n1_lg + LA)) <= p.src_size) { // the whole transform is inside the source
 #pragma unroll
-    for (int m = 0; m < PPT / R0; ++m)
+        for (int m = 0; m < PPT / R0; ++m)
 #pragma unroll
-        for (int r = 0; r < R0; ++r) {
-            int64_t i     = n1 + ((int64_t)in_elem<LA>(jt, m, r) << p.n1_lg);
-            v[m * R0 + r] = i < p.src_size ? ld_cx(sbase, i, p.src_lp) : cf{0.f, 0.f};
-        }
+            for (int r = 0; r < R0; ++r) {
+                uint32_t i    = n1 + ((uint32_t)in_elem<LA>(jt, m, r) << p.n1_lg);
+                v[m * R0 + r] = src.ld(poff(i, p.src_lp), p.nt_load);
+            }
+    } else {
+#pragma unroll
+        for (int m = 0; m < PPT / R0; ++m)
+#pragma unroll
+            for (int r = 0; r < R0; ++r) {
+                uint32_t i    = n1 + ((uint32_t)in_elem<LA>(jt, m, r) << p.n1_lg);
+                bool     ok   = i < p.src_size;
+                cf       x    = src.ld(ok ? poff(i, p.src_lp) : 0u, p.nt_load);
+                v[m * R0 + r] = ok ? x : cf{0.f, 0.f};
+            }
+    }
     if (p.conj_in) {
 #pragma unroll
         for (int q = 0; q < PPT; ++q)
             v[q].y = -v[q].y;
     }
-    stockham<LA, false>(v, lre, lim, jt, addr, p.twA);
-    if (p.twl) {
+    stockham<LA, false>(v, lre, lim, jt, addr, ltw);
+    if constexpr (TW > 0) {
 #pragma unroll
         for (int m = 0; m < PPT / RL; ++m)
 #pragma unroll
             for (int r = 0; r < RL; ++r) {
-                uint32_t e    = (uint32_t)n1 * (uint32_t)out_elem<LA>(jt, m, r);
-                v[m * RL + r] = cmul(v[m * RL + r], tw2(p.twl, p.twh, p.tw_ls, e));
+                uint32_t e    = n1 * (uint32_t)out_elem<LA>(jt, m, r);
+                v[m * RL + r] = cmul(v[m * RL + r], PassTw<TW>::get(lptw, p.twl, p.twh, p.tw_ls, p.tw_lg, e));
+                __builtin_amdgcn_sched_barrier(0); // one element's table reads in flight at a time
             }
     }
-    float sy = p.conj_out ? -p.scale : p.scale;
+    if (p.scale != 1.0f) {
+#pragma unroll
+        for (int q = 0; q < PPT; ++q) {
+            v[q].x *= p.scale;
+            v[q].y *= p.scale;
+        }
+    }
+    if (p.conj_out) {
+#pragma unroll
+        for (int q = 0; q < PPT; ++q)
+            v[q].y = -v[q].y;
+    }
 #pragma unroll
     for (int m = 0; m < PPT / RL; ++m)
 #pragma unroll
         for (int r = 0; r < RL; ++r) {
-            cf      x = v[m * RL + r];
-            int64_t i = n1 + ((int64_t)out_elem<LA>(jt, m, r) << p.n1_lg);
-            st_cx(dbase, i, p.dst_lp, cf{x.x * p.scale, x.y * sy});
+            uint32_t i = n1 + ((uint32_t)out_elem<LA>(jt, m, r) << p.n1_lg);
+            dst.st(poff(i, p.dst_lp), v[m * RL + r], p.nt_store);
         }
+}
+
+template<int LA, int C, int TW>
+constexpr int col_lds_floats() {
+    return 2 * (1 << LA) * C + TwTable<LA>::FLOATS + 2 * TW;
+}
+
+template<int LA, int C, int TW>
+__global__ __launch_bounds__(C * Plan<LA>::TA, waves_per_simd(C * Plan<LA>::TA, 4 * col_lds_floats<LA, C, TW>()))
+void k_col(ColArgs p) {
+    constexpr int    NT   = C * Plan<LA>::TA;
+    constexpr int    DATA = 2 * (1 << LA) * C;
+    __shared__ float lds[col_lds_floats<LA, C, TW>()];
+    float*           lre  = lds;
+    float*           lim  = lds + DATA / 2;
+    float*           ltw  = lds + DATA;
+    float*           lptw = ltw + TwTable<LA>::FLOATS;
+    TwTable<LA>::fill(ltw, p.twA, threadIdx.x, NT);
+    if constexpr (TW > 0)
+        PassTw<TW>::fill(lptw, p.twl, p.twh, p.tw_ls, p.tw_lg, threadIdx.x, NT);
+    __syncthreads();
+    for (int64_t tile = blockIdx.x; tile < p.ntiles; tile += gridDim.x) {
+        col_tile<LA, C, TW>(p, tile, lre, lim, ltw, lptw);
+        __syncthreads(); // LDS is reused by the next tile
+    }
 }
 
 // t[e] = W_M^(e*stride) = exp(-2*pi*i*e*stride/M) for e < count, computed in FP64, rounded to FP32.
@@ -322,7 +538,9 @@ __global__ void k_generate(float* dst, int lp, int64_t n, uint64_t seed, int dis
         re = (float)(sigma * r * c);
         im = (float)(sigma * r * s);
     }
-    st_cx(dst, i, lp, cf{re, im});
+    int64_t o                   = i + ((i >> lp) << lp);
+    dst[o]                      = re;
+    dst[o + ((int64_t)1 << lp)] = im;
 }
 
 } // namespace pcxk

@@ -14,6 +14,17 @@
 
 namespace pcxk {
 
+// 64-bit packed element access (signals may exceed 2^31 floats).
+__device__ __forceinline__ cf ld_cx64(const float* __restrict__ base, int64_t i, int lp) {
+    int64_t o = pidx(i, lp);
+    return {base[o], base[o + ((int64_t)1 << lp)]};
+}
+__device__ __forceinline__ void st_cx64(float* __restrict__ base, int64_t i, int lp, cf v) {
+    int64_t o                    = pidx(i, lp);
+    base[o]                      = v.x;
+    base[o + ((int64_t)1 << lp)] = v.y;
+}
+
 struct XcArgs {
     const float* src;   // signal, pack src_lp
     float*       dst;   // output, pack dst_lp (may equal src: then halo holds the overlap)
@@ -30,7 +41,14 @@ struct XcArgs {
 };
 
 template<int LA, int G>
-__global__ __launch_bounds__(G * Plan<LA>::TA) void k_fxcorr(XcArgs p) {
+constexpr int xc_lds_floats() {
+    return RowShape<LA, G>::DATA + TwTable<LA>::FLOATS;
+}
+
+// B32: the signal fits 32-bit buffer offsets (one SGPR descriptor, one VGPR per address).
+template<int LA, int G, bool B32>
+__global__ __launch_bounds__(G * Plan<LA>::TA, waves_per_simd(G * Plan<LA>::TA, 4 * xc_lds_floats<LA, G>()))
+void k_fxcorr(XcArgs p) {
     using PF           = Plan<LA, false>;
     using PI           = Plan<LA, true>;
     constexpr int TA   = PF::TA;
@@ -40,8 +58,12 @@ __global__ __launch_bounds__(G * Plan<LA>::TA) void k_fxcorr(XcArgs p) {
     constexpr int RLI  = PI::last_radix;
     constexpr int STR  = RowShape<LA, G>::STRIDE;
     static_assert(PI::first_radix == RL, "fused multiply needs matching register maps");
-    __shared__ float lre[G * STR];
-    __shared__ float lim[G * STR];
+    __shared__ float lds[xc_lds_floats<LA, G>()];
+    float*           lre = lds;
+    float*           lim = lds + RowShape<LA, G>::DATA / 2;
+    float*           ltw = lds + RowShape<LA, G>::DATA;
+    TwTable<LA>::fill(ltw, p.twA, threadIdx.x, G * TA);
+    __syncthreads();
 
     const int     t    = threadIdx.x;
     const int     g    = t / TA;
@@ -49,7 +71,7 @@ __global__ __launch_bounds__(G * Plan<LA>::TA) void k_fxcorr(XcArgs p) {
     const int64_t b    = (int64_t)blockIdx.x * G + g;
     const bool    ok   = b < p.nblocks;
     const int64_t off  = b * p.step;
-    auto          addr = [&](int a) { return g * STR + a + (a >> 5); };
+    auto          addr = [&](int b, int o) { return g * STR + b + (b >> 5) + o + (o >> 5); };
 
     cf v[PPT];
 #pragma unroll
@@ -63,22 +85,24 @@ __global__ __launch_bounds__(G * Plan<LA>::TA) void k_fxcorr(XcArgs p) {
                 if (p.hre && a >= p.step) {
                     int64_t h = b * p.hlen + (a - p.step);
                     x         = {p.hre[h], p.him[h]};
+                } else if constexpr (B32) {
+                    x = BufR(p.src, p.src_lp).ld(poff((uint32_t)i, p.src_lp), 0);
                 } else {
-                    x = ld_cx(p.src, i, p.src_lp);
+                    x = ld_cx64(p.src, i, p.src_lp);
                 }
             }
             v[m * R0 + r] = x;
         }
-    stockham<LA, false>(v, lre, lim, jt, addr, p.twA);
+    stockham<LA, false>(v, lre, lim, jt, addr, ltw);
 #pragma unroll
     for (int m = 0; m < PPT / RL; ++m)
 #pragma unroll
         for (int r = 0; r < RL; ++r) {
             int e         = out_elem<LA, false>(jt, m, r);
-            cf  k         = {p.kre[e], p.kim[e]};
+            cf  k         = BufR(p.kre, LA).ld((uint32_t)e, 0); // split spectrum: re[N], im[N]
             v[m * RL + r] = cmul(cconj(v[m * RL + r]), k);
         }
-    stockham<LA, true>(v, lre, lim, jt, addr, p.twA);
+    stockham<LA, true>(v, lre, lim, jt, addr, ltw);
     if (ok) {
         int64_t keep = p.len - off < p.step ? p.len - off : p.step;
 #pragma unroll
@@ -86,8 +110,12 @@ __global__ __launch_bounds__(G * Plan<LA>::TA) void k_fxcorr(XcArgs p) {
 #pragma unroll
             for (int r = 0; r < RLI; ++r) {
                 int a = out_elem<LA, true>(jt, m, r);
-                if (a < keep)
-                    st_cx(p.dst, off + a, p.dst_lp, cconj(v[m * RLI + r]));
+                if (a < keep) {
+                    if constexpr (B32)
+                        BufW(p.dst, p.dst_lp).st(poff((uint32_t)(off + a), p.dst_lp), cconj(v[m * RLI + r]), 0);
+                    else
+                        st_cx64(p.dst, off + a, p.dst_lp, cconj(v[m * RLI + r]));
+                }
             }
     }
 }
@@ -101,7 +129,7 @@ __global__ void k_fxcorr_halo(const float* src, int src_lp, int64_t len, int64_t
         return;
     int64_t b = idx / hlen, h = idx % hlen;
     int64_t i = b * step + step + h;
-    cf      x = i < len ? ld_cx(src, i, src_lp) : cf{0.f, 0.f};
+    cf      x = i < len ? ld_cx64(src, i, src_lp) : cf{0.f, 0.f};
     hre[idx]  = x.x;
     him[idx]  = x.y;
 }

@@ -38,7 +38,8 @@ int fail(int code, std::string msg) {
 constexpr int kMaxSinglePass = 14; // N <= 16384 runs in one LDS-resident pass
 constexpr int kMaxLog2       = 24; // four-step limit: N2 <= 1024 columns, N1 <= 16384
 
-bool is_pow2(int64_t v) { return v > 0 && (v & (v - 1)) == 0; }
+bool    is_pow2(int64_t v) { return v > 0 && (v & (v - 1)) == 0; }
+int64_t pidx64(int64_t i, int lp) { return i + ((i >> lp) << lp); }
 int  ilog2(int64_t v) {
     int l = 0;
     while (((int64_t)1 << l) < v)
@@ -76,79 +77,115 @@ hipError_t gen_twiddles(float2* t, int64_t M, int64_t stride, int64_t count) {
 }
 
 // ---------------------------------------------------------------- kernel dispatch tables
-using RowLaunch = void (*)(const RowArgs&, int64_t nrows, hipStream_t);
-using ColLaunch = void (*)(const ColArgs&, hipStream_t);
+using RowLaunch = void (*)(const RowArgs&, int64_t nrows, int64_t grid_cap, hipStream_t);
+using ColLaunch = void (*)(const ColArgs&, int64_t grid_cap, hipStream_t);
 using XcLaunch  = void (*)(const XcArgs&, hipStream_t);
 
-template<int LA, int G>
-void launch_row(const RowArgs& a, int64_t nrows, hipStream_t st) {
+// grid_cap > 0 bounds the grid; the kernels then loop over their work (grid-stride).
+template<int LA, int G, int LD, int ST, int TW, bool UNI = (Plan<LA>::TA >= 64), int ORD = 0>
+void launch_row(const RowArgs& a, int64_t nrows, int64_t grid_cap, hipStream_t st) {
     int64_t nwg = (nrows + G - 1) / G;
-    hipLaunchKernelGGL((k_row<LA, G>), dim3((unsigned)nwg), dim3(G * Plan<LA>::TA), 0, st, a);
+    if (grid_cap > 0 && nwg > grid_cap)
+        nwg = grid_cap;
+    hipLaunchKernelGGL((k_row<LA, G, LD, ST, TW, UNI, ORD>), dim3((unsigned)nwg), dim3(G * Plan<LA>::TA), 0, st, a);
 }
-template<int LA, int C>
-void launch_col(const ColArgs& a, hipStream_t st) {
-    hipLaunchKernelGGL((k_col<LA, C>), dim3((unsigned)a.ntiles), dim3(C * Plan<LA>::TA), 0, st, a);
+template<int LA, int C, int TW>
+void launch_col(const ColArgs& a, int64_t grid_cap, hipStream_t st) {
+    int64_t nwg = a.ntiles;
+    if (grid_cap > 0 && nwg > grid_cap)
+        nwg = grid_cap;
+    hipLaunchKernelGGL((k_col<LA, C, TW>), dim3((unsigned)nwg), dim3(C * Plan<LA>::TA), 0, st, a);
 }
 template<int LA>
 void launch_xc(const XcArgs& a, hipStream_t st) {
     constexpr int G   = default_rows<LA>();
     int64_t       nwg = (a.nblocks + G - 1) / G;
-    hipLaunchKernelGGL((k_fxcorr<LA, G>), dim3((unsigned)nwg), dim3(G * Plan<LA>::TA), 0, st, a);
+    // 32-bit buffer offsets cover the signal when its packed footprint stays below 2^29 floats
+    bool small = pidx64(a.len + (int64_t)G * (1 << LA), a.src_lp > a.dst_lp ? a.src_lp : a.dst_lp) < ((int64_t)1 << 29) &&
+                 a.src_lp < 29 && a.dst_lp < 29;
+    if (small)
+        hipLaunchKernelGGL((k_fxcorr<LA, G, true>), dim3((unsigned)nwg), dim3(G * Plan<LA>::TA), 0, st, a);
+    else
+        hipLaunchKernelGGL((k_fxcorr<LA, G, false>), dim3((unsigned)nwg), dim3(G * Plan<LA>::TA), 0, st, a);
 }
 
+// Single-pass variants: 0 natural in/out; 1, 2 permuted store (forward, bit_reversed /
+// unordered); 3, 4 permuted load (inverse, bit_reversed / unordered).
+using RowSet = std::array<RowLaunch, 5>;
+template<int LA>
+constexpr RowSet single_set() {
+    constexpr int  G   = default_rows<LA>();
+    constexpr bool UNI = Plan<LA>::TA >= 64;
+    return {&launch_row<LA, G, LD_CONTIG, ST_CONTIG, 0, UNI, 0>, &launch_row<LA, G, LD_CONTIG, ST_PERM, 0, UNI, ORD_BITREV>,
+            &launch_row<LA, G, LD_CONTIG, ST_PERM, 0, UNI, ORD_UNORD>, &launch_row<LA, G, LD_PERM, ST_CONTIG, 0, UNI, ORD_BITREV>,
+            &launch_row<LA, G, LD_PERM, ST_CONTIG, 0, UNI, ORD_UNORD>};
+}
+int single_variant(int inverse, int order) { return order == 0 ? 0 : (inverse ? 2 : 0) + order; }
 template<int... L>
-constexpr auto row_table(std::integer_sequence<int, L...>) {
-    return std::array<RowLaunch, sizeof...(L)>{&launch_row<L + 1, default_rows<L + 1>()>...};
+constexpr auto single_table(std::integer_sequence<int, L...>) {
+    return std::array<RowSet, sizeof...(L)>{single_set<L + 1>()...};
 }
 template<int... L>
 constexpr auto xc_table(std::integer_sequence<int, L...>) {
     return std::array<XcLaunch, sizeof...(L)>{&launch_xc<L + 1>...};
 }
 // index LA-1 for LA in [1, 14]
-const auto kRow = row_table(std::make_integer_sequence<int, kMaxSinglePass>{});
-const auto kXc  = xc_table(std::make_integer_sequence<int, kMaxSinglePass>{});
+const auto kSingle = single_table(std::make_integer_sequence<int, kMaxSinglePass>{});
+const auto kXc     = xc_table(std::make_integer_sequence<int, kMaxSinglePass>{});
 
-// Pass-B / pass-1 rows of the four-step (N1 = 2^LA, G rows per workgroup, G * N1 / 16 <= 1024
-// threads).  Strided stores write G consecutive floats per run.
-template<int LA, int... Gs>
-RowLaunch pick_row(int g) {
-    RowLaunch f = nullptr;
-    ((g == Gs ? (f = &launch_row<LA, Gs>, 0) : 0), ...);
-    return f;
+using PassSet = std::array<RowLaunch, 6>;
+// Pass-twiddle table capacity in LDS for N = 2^LG (2: too large, read from global memory).
+template<int LG>
+constexpr int tw_cap() {
+    return pass_tw_entries(LG) <= 2048 ? pass_tw_entries(LG) : 2;
 }
-RowLaunch row_pass_launch(int la, int g) {
-    switch (la) {
-    case 6: return pick_row<6, 16, 32, 64, 128>(g);
-    case 7: return pick_row<7, 16, 32, 64, 128>(g);
-    case 8: return pick_row<8, 8, 16, 32, 64>(g);
-    case 9: return pick_row<9, 8, 16, 32>(g);
-    case 10: return pick_row<10, 4, 8, 16>(g);
-    case 11: return pick_row<11, 2, 4, 8>(g);
-    case 12: return pick_row<12, 1, 2, 4>(g);
-    case 13: return pick_row<13, 1, 2>(g);
-    case 14: return pick_row<14, 1>(g);
-    default: return nullptr;
-    }
+// Four-step row-pass variants: forward with strided store (natural order) or permuted store
+// (bit_reversed, unordered); inverse with strided or permuted load and the pass twiddle.
+// The G rows of a workgroup lie in one transform (G <= N2), so buffer addressing applies.
+template<int LA, int G, int LG>
+constexpr PassSet pass_set() {
+    constexpr int TW = tw_cap<LG>();
+    return {&launch_row<LA, G, LD_CONTIG, ST_STRIDED, 0, true, 0>,
+            &launch_row<LA, G, LD_CONTIG, ST_PERM, 0, true, ORD_BITREV>,
+            &launch_row<LA, G, LD_CONTIG, ST_PERM, 0, true, ORD_UNORD>,
+            &launch_row<LA, G, LD_STRIDED, ST_CONTIG, TW, true, 0>,
+            &launch_row<LA, G, LD_PERM, ST_CONTIG, TW, true, ORD_BITREV>,
+            &launch_row<LA, G, LD_PERM, ST_CONTIG, TW, true, ORD_UNORD>};
 }
-int default_row_g(int la) { return la <= 7 ? 64 : la == 8 ? 32 : la == 9 ? 32 : la == 10 ? 16 : la == 11 ? 8 : la == 12 ? 4 : la == 13 ? 2 : 1; }
-// Column passes (N2 = 2^LA rows per column, C columns per tile, C * N2 <= 16384 for LDS).
-template<int LA, int... Cs>
-ColLaunch pick_col(int c) {
-    ColLaunch f = nullptr;
-    ((c == Cs ? (f = &launch_col<LA, Cs>, 0) : 0), ...);
-    return f;
+template<int LA, int C, int LG>
+constexpr std::array<ColLaunch, 2> col_set() {
+    return {&launch_col<LA, C, 0>, &launch_col<LA, C, tw_cap<LG>()>};
 }
-ColLaunch col_pass_launch(int la, int c) {
-    switch (la) {
-    case 6: return pick_col<6, 16, 32, 64>(c);
-    case 7: return pick_col<7, 16, 32, 64>(c);
-    case 8: return pick_col<8, 16, 32, 64>(c);
-    case 9: return pick_col<9, 16, 32>(c);
-    case 10: return pick_col<10, 16>(c);
-    default: return nullptr;
-    }
+
+// The four-step shapes this build instantiates: N = 2^lg = N1 * N2 with N1 = 2^n1 (row length,
+// G rows per workgroup, G * N1 / PPT <= 1024 threads) and N2 = 2^n2 (column length, C columns
+// per tile, C * N2 <= 16384 for LDS).
+struct FourStep {
+    int     lg, n1, n2, g, c;
+    PassSet rows;
+    std::array<ColLaunch, 2> cols;
+};
+template<int LG, int N1, int N2, int G, int C>
+FourStep shape() {
+    static_assert(N1 + N2 == LG, "shape");
+    return {LG, N1, N2, G, C, pass_set<N1, G, LG>(), col_set<N2, C, LG>()};
 }
-int default_col_c(int la) { return la <= 8 ? 64 : la == 9 ? 32 : 16; }
+const std::array<FourStep, 20> kShapes = {
+    // defaults (first match per lg): measured best on MI355X for 2^16 is N1 = 512, N2 = 128
+    shape<15, 8, 7, 32, 64>(), shape<16, 9, 7, 32, 64>(), shape<16, 8, 8, 32, 64>(), shape<17, 9, 8, 32, 64>(), shape<18, 9, 9, 32, 32>(),
+    shape<19, 10, 9, 16, 32>(), shape<20, 10, 10, 16, 16>(), shape<21, 11, 10, 8, 16>(), shape<22, 12, 10, 4, 16>(),
+    shape<23, 13, 10, 2, 16>(), shape<24, 14, 10, 1, 16>(),
+    // alternatives for N = 2^16 explored by the tuning sweeps
+    shape<16, 8, 8, 16, 64>(), shape<16, 8, 8, 64, 64>(), shape<16, 8, 8, 32, 32>(), shape<16, 8, 8, 16, 32>(),
+    shape<16, 10, 6, 16, 64>(), shape<16, 7, 9, 64, 32>(), shape<16, 6, 10, 64, 16>(),
+    shape<16, 9, 7, 32, 32>(), shape<16, 10, 6, 16, 32>(),
+};
+const FourStep* find_shape(int lg, int n2, int g, int c) {
+    for (const auto& f : kShapes)
+        if (f.lg == lg && (n2 == 0 || f.n2 == n2) && (g == 0 || f.g == g) && (c == 0 || f.c == c))
+            return &f;
+    return nullptr;
+}
 
 } // namespace
 
@@ -167,8 +204,10 @@ struct pcx_plan_s {
     float*  scratch      = nullptr;
     int64_t scratch_cap  = 0; // transforms the scratch holds
     int64_t chunk        = 0; // transforms per four-step chunk
-    int     col_c        = 0; // columns per column-pass tile
-    int     row_g        = 0; // rows per row-pass workgroup
+    const FourStep* shape = nullptr; // four-step kernels and tile sizes
+    int64_t grid_col     = 1024; // grid caps of the two passes (persistent loop; 0: one workgroup per tile)
+    int64_t grid_row     = 1024;
+    int     nt_flags     = 0; // bit 0/1: streaming load/store of src/dst, bit 2/3: of the scratch
     std::mutex mu;
 };
 
@@ -202,14 +241,26 @@ extern "C" int pcx_plan_create(int64_t fft_size, int64_t sub_size, int order, pc
             he = gen_twiddles(p->tw_row, fft_size, 1, fft_size);
     } else {
         p->two_pass = true;
-        p->n2_lg    = lg / 2 > 10 ? 10 : lg / 2;
-        if (const char* env = std::getenv("PCX_N2_LOG2")) { // column-length override for tuning sweeps
-            int v = std::atoi(env);
-            if (v >= 6 && v <= 10 && lg - v >= 6 && lg - v <= 14)
-                p->n2_lg = v;
+        // shape: N2 = 2^min(lg/2, 10) columns-length by default; PCX_N2_LOG2 and PCX_TILE ("C,G")
+        // select one of the other instantiated shapes (tuning sweeps)
+        int n2s = 0, c = 0, g = 0; // 0: the default (first listed) shape for this size
+        if (const char* env = std::getenv("PCX_N2_LOG2"))
+            n2s = std::atoi(env);
+        if (const char* env = std::getenv("PCX_TILE"))
+            if (std::sscanf(env, "%d,%d", &c, &g) != 2)
+                c = g = 0;
+        p->shape = find_shape(lg, n2s, g, c);
+        if (!p->shape)
+            p->shape = find_shape(lg, n2s, 0, 0);
+        if (!p->shape)
+            p->shape = find_shape(lg, 0, 0, 0);
+        if (!p->shape) {
+            delete p;
+            return fail(PCX_ERR_UNSUPPORTED, "no four-step kernels for fft_size " + s(fft_size));
         }
-        p->n1_lg    = lg - p->n2_lg;
-        p->tw_ls    = (lg + 1) / 2;
+        p->n2_lg = p->shape->n2;
+        p->n1_lg = p->shape->n1;
+        p->tw_ls = (lg + 1) / 2;
         int64_t n1 = (int64_t)1 << p->n1_lg, n2 = (int64_t)1 << p->n2_lg;
         int64_t nl = (int64_t)1 << p->tw_ls, nh = fft_size >> p->tw_ls;
         he = hipMalloc(&p->tw_row, sizeof(float2) * n1);
@@ -234,15 +285,13 @@ extern "C" int pcx_plan_create(int64_t fft_size, int64_t sub_size, int order, pc
         p->chunk = chunk_bytes / (fft_size * 8);
         if (p->chunk < 1)
             p->chunk = 1;
-        p->col_c = default_col_c(p->n2_lg);
-        p->row_g = default_row_g(p->n1_lg);
-        if (const char* env = std::getenv("PCX_TILE")) { // "C,G" override for tuning sweeps
-            int c = 0, g = 0;
-            if (std::sscanf(env, "%d,%d", &c, &g) == 2 && col_pass_launch(p->n2_lg, c) && row_pass_launch(p->n1_lg, g)) {
-                p->col_c = c;
-                p->row_g = g;
-            }
+        if (const char* env = std::getenv("PCX_GRID")) { // "col,row" grid caps for tuning sweeps
+            long long ga = 0, gb = 0;
+            if (std::sscanf(env, "%lld,%lld", &ga, &gb) == 2)
+                p->grid_col = ga, p->grid_row = gb;
         }
+        if (const char* env = std::getenv("PCX_NT"))
+            p->nt_flags = std::atoi(env);
     }
     if (he == hipSuccess)
         he = hipDeviceSynchronize();
@@ -312,11 +361,9 @@ int run_batch(pcx_plan_t p, int inverse, int normalized, float* dst, int64_t dst
         RowArgs a{};
         a.src = src, a.dst = dst, a.src_dist = src_dist, a.dst_dist = dst_dist;
         a.src_size = src_size, a.nrows = batch, a.src_lp = src_lp, a.dst_lp = dst_lp, a.rows_lg = 0;
-        a.load_kind  = (inverse && permuted) ? LD_PERM : LD_CONTIG;
-        a.store_kind = (!inverse && permuted) ? ST_PERM : ST_CONTIG;
         a.order = p->order, a.conj_in = inverse, a.conj_out = inverse, a.scale = scale;
         a.twA = p->tw_row, a.twl = nullptr, a.twh = nullptr, a.tw_ls = 0;
-        kRow[p->lg - 1](a, batch, st);
+        kSingle[p->lg - 1][single_variant(inverse, p->order)](a, batch, 0, st);
         HIPCHK(hipGetLastError());
         return PCX_SUCCESS;
     }
@@ -335,11 +382,9 @@ int run_batch(pcx_plan_t p, int inverse, int normalized, float* dst, int64_t dst
             return fail(PCX_ERR_OUT_OF_MEMORY, "scratch allocation of " + s(8 * n * chunk) + " bytes failed");
         p->scratch_cap = chunk;
     }
-    const int C    = p->col_c;
-    ColLaunch colf = col_pass_launch(p->n2_lg, C);
-    RowLaunch rowf = row_pass_launch(p->n1_lg, p->row_g);
-    if (!colf || !rowf)
-        return fail(PCX_ERR_UNSUPPORTED, "no four-step kernels for fft_size " + s(n));
+    const int C    = p->shape->c;
+    ColLaunch colf = p->shape->cols[inverse ? 0 : 1];
+    RowLaunch rowf = p->shape->rows[(inverse ? 3 : 0) + p->order];
     const int64_t n1 = (int64_t)1 << p->n1_lg, n2 = (int64_t)1 << p->n2_lg;
     for (int64_t b0 = 0; b0 < batch; b0 += chunk) {
         int64_t      cb = batch - b0 < chunk ? batch - b0 : chunk;
@@ -349,34 +394,37 @@ int run_batch(pcx_plan_t p, int inverse, int normalized, float* dst, int64_t dst
         RowArgs      ra{};
         ca.twA = p->tw_col, ca.n1_lg = p->n1_lg, ca.ntiles = cb * (n1 / C);
         ra.twA = p->tw_row, ra.rows_lg = p->n2_lg, ra.nrows = cb * n2, ra.order = p->order;
+        ca.tw_lg = p->lg, ra.tw_lg = p->lg;
         if (!inverse) {
             // pass A: columns of src -> scratch, times W_N^(n1*k2)
             ca.src = cs, ca.src_dist = src_dist, ca.src_lp = src_lp, ca.src_size = src_size;
             ca.dst = p->scratch, ca.dst_dist = 2 * n, ca.dst_lp = p->lg;
             ca.conj_in = 0, ca.conj_out = 0, ca.scale = 1.0f;
             ca.twl = p->twl, ca.twh = p->twh, ca.tw_ls = p->tw_ls;
-            colf(ca, st);
+            ca.nt_load = p->nt_flags & 1, ca.nt_store = (p->nt_flags >> 2) & 1;
+            colf(ca, p->grid_col, st);
             // pass B: rows of scratch -> dst in the plan's order
             ra.src = p->scratch, ra.src_dist = 2 * n, ra.src_lp = p->lg, ra.src_size = n;
             ra.dst = cd, ra.dst_dist = dst_dist, ra.dst_lp = dst_lp;
-            ra.load_kind = LD_CONTIG, ra.store_kind = permuted ? ST_PERM : ST_STRIDED;
             ra.conj_in = 0, ra.conj_out = 0, ra.scale = 1.0f;
             ra.twl = nullptr, ra.twh = nullptr;
-            rowf(ra, ra.nrows, st);
+            ra.nt_load = (p->nt_flags >> 3) & 1, ra.nt_store = (p->nt_flags >> 1) & 1;
+            rowf(ra, ra.nrows, p->grid_row, st);
         } else {
             // pass 1: rows (over k1) of the spectrum -> scratch, times W_N^(n1*k2)
             ra.src = cs, ra.src_dist = src_dist, ra.src_lp = src_lp, ra.src_size = n;
             ra.dst = p->scratch, ra.dst_dist = 2 * n, ra.dst_lp = p->lg;
-            ra.load_kind = permuted ? LD_PERM : LD_STRIDED, ra.store_kind = ST_CONTIG;
             ra.conj_in = 1, ra.conj_out = 0, ra.scale = 1.0f;
             ra.twl = p->twl, ra.twh = p->twh, ra.tw_ls = p->tw_ls;
-            rowf(ra, ra.nrows, st);
+            ra.nt_load = p->nt_flags & 1, ra.nt_store = (p->nt_flags >> 2) & 1;
+            rowf(ra, ra.nrows, p->grid_row, st);
             // pass 2: columns (over k2) of scratch -> dst, natural order
             ca.src = p->scratch, ca.src_dist = 2 * n, ca.src_lp = p->lg, ca.src_size = n;
             ca.dst = cd, ca.dst_dist = dst_dist, ca.dst_lp = dst_lp;
             ca.conj_in = 0, ca.conj_out = 1, ca.scale = scale;
             ca.twl = nullptr, ca.twh = nullptr;
-            colf(ca, st);
+            ca.nt_load = (p->nt_flags >> 3) & 1, ca.nt_store = (p->nt_flags >> 1) & 1;
+            colf(ca, p->grid_col, st);
         }
         HIPCHK(hipGetLastError());
     }
@@ -469,10 +517,10 @@ extern "C" int pcx_fxcorr_create(const float* g, int64_t g_pack, int64_t g_size,
     // kernel spectrum: FFT of g zero-extended to N, scaled by 1/N (fxcorr.hpp:67-68)
     RowArgs a{};
     a.src = g, a.dst = x->kspec, a.src_dist = 0, a.dst_dist = 0, a.src_size = g_size, a.nrows = 1;
-    a.src_lp = ilog2(g_pack), a.dst_lp = lg, a.rows_lg = 0, a.load_kind = LD_CONTIG, a.store_kind = ST_CONTIG;
+    a.src_lp = ilog2(g_pack), a.dst_lp = lg, a.rows_lg = 0;
     a.order = 0, a.conj_in = 0, a.conj_out = 0, a.scale = (float)(1.0 / (double)fft_size);
     a.twA = x->tw;
-    kRow[lg - 1](a, 1, st);
+    kSingle[lg - 1][0](a, 1, 0, st);
     he = hipGetLastError();
     if (he == hipSuccess)
         he = hipStreamSynchronize(st);

@@ -10,7 +10,14 @@
 // Entry: v[m*R0 + r] holds x[jt + m*TA + r*A/R0] (loaded by the caller, from HBM or LDS).
 // Exit:  v[m*RL + r] holds X[jt + m*TA + r*A/RL] in natural order (RL = last radix).
 // Between stages the data goes through LDS; `addr` maps an element index to a float offset of
-// this thread's transform in the split real/imaginary arrays `lre`/`lim`.
+// this thread's transform in the split real/imaginary arrays `lre`/`lim`.  It is called as
+// addr(base, off) with a compile-time offset, so LDS accesses use immediate offsets from one
+// address per butterfly (the padded layouts are carry-free for every plan: pad(b + o) =
+// pad(b) + pad(o) for the offsets a stage adds).
+//
+// Stage twiddles come from an LDS copy of W_A (TwTable): the whole table for A <= 1024, else a
+// two-level split W_A^e = lo[e mod 64] * hi[e / 64] (one extra complex multiply, 2.5 KiB at
+// A = 16384).  The table is filled once per workgroup from the FP32 table in global memory.
 #pragma once
 #include "dev_common.hpp"
 
@@ -22,7 +29,8 @@ struct Plan {
     static constexpr int NFULL   = LA / 4;           // radix-16 stages
     static constexpr int REM     = LA % 4;           // bits left for the small-radix stage
     static constexpr int NSTAGES = NFULL + (REM ? 1 : 0);
-    static constexpr int PPT     = LA >= 4 ? 16 : A; // points per thread
+    // points per thread: 32 for the largest sizes keeps a transform within 512 threads
+    static constexpr int PPT     = LA >= 13 ? 32 : (LA >= 4 ? 16 : A);
     static constexpr int TA      = A / PPT;          // threads per transform
     static constexpr int radix(int s) {
         if (REM == 0)
@@ -36,15 +44,42 @@ struct Plan {
     static constexpr int last_radix  = radix(NSTAGES - 1);
 };
 
-// Twiddle tables hold W_M^e = exp(-2*pi*i*e/M), e in [0, M), as interleaved (re, im) floats.
-__device__ __forceinline__ cf tw_load(const float2* __restrict__ t, uint32_t e) {
-    float2 w = t[e];
-    return {w.x, w.y};
-}
+// LDS twiddle table of W_A^e = exp(-2*pi*i*e/A).
+template<int LA>
+struct TwTable {
+    static constexpr bool direct = LA <= 10;
+    static constexpr int  LOB    = direct ? LA : 6;
+    static constexpr int  NLO    = 1 << LOB;
+    static constexpr int  NHI    = direct ? 0 : 1 << (LA - LOB);
+    static constexpr int  FLOATS = 2 * (NLO + NHI); // lo re, lo im, hi re, hi im
+
+    // tw: global W_A table (interleaved re, im); nthreads threads cooperate.
+    __device__ static void fill(float* t, const float2* __restrict__ tw, int tid, int nthreads) {
+        for (int e = tid; e < NLO; e += nthreads) {
+            float2 w       = tw[e];
+            t[e]           = w.x;
+            t[NLO + e]     = w.y;
+        }
+        for (int h = tid; h < NHI; h += nthreads) {
+            float2 w                = tw[h << LOB];
+            t[2 * NLO + h]          = w.x;
+            t[2 * NLO + NHI + h]    = w.y;
+        }
+    }
+    __device__ static cf get(const float* t, uint32_t e) {
+        if constexpr (direct) {
+            return {t[e], t[NLO + e]};
+        } else {
+            uint32_t lo = e & (NLO - 1), hi = e >> LOB;
+            cf       a  = {t[lo], t[NLO + lo]};
+            cf       b  = {t[2 * NLO + hi], t[2 * NLO + NHI + hi]};
+            return cmul(a, b);
+        }
+    }
+};
 
 template<int LA, bool REV, class Addr>
-__device__ __forceinline__ void stockham(cf* v, float* lre, float* lim, int jt, const Addr& addr,
-                                         const float2* __restrict__ twA) {
+__device__ __forceinline__ void stockham(cf* v, float* lre, float* lim, int jt, const Addr& addr, const float* ltw) {
     using P           = Plan<LA, REV>;
     constexpr int A   = P::A;
     constexpr int TA  = P::TA;
@@ -61,24 +96,25 @@ __device__ __forceinline__ void stockham(cf* v, float* lre, float* lim, int jt, 
                 int j = jt + m * TA;
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
-                    int o        = addr(j + r * (A / R));
+                    int o        = addr(j, r * (A / R));
                     v[m * R + r] = cf{lre[o], lim[o]};
                 }
             }
         }
-        if constexpr (Ns > 1) {
-            constexpr int shift = LA - ilog2c(Ns * R);
+        // twiddle and butterfly one radix-R group at a time, so only one group's twiddles are live
 #pragma unroll
-            for (int m = 0; m < NB; ++m) {
-                uint32_t k = (uint32_t)(jt + m * TA) & (Ns - 1);
+        for (int m = 0; m < NB; ++m) {
+            if constexpr (Ns > 1) {
+                constexpr int shift = LA - ilog2c(Ns * R);
+                uint32_t      k     = (uint32_t)(jt + m * TA) & (Ns - 1);
 #pragma unroll
                 for (int r = 1; r < R; ++r)
-                    v[m * R + r] = cmul(v[m * R + r], tw_load(twA, (uint32_t)(r * k) << shift));
+                    v[m * R + r] = cmul(v[m * R + r], TwTable<LA>::get(ltw, (uint32_t)(r * k) << shift));
             }
-        }
-#pragma unroll
-        for (int m = 0; m < NB; ++m)
             dft<R>(v + m * R);
+            if constexpr (NB > 1)
+                __builtin_amdgcn_sched_barrier(0);
+        }
         if constexpr (s + 1 < P::NSTAGES) {
             __syncthreads(); // every read of this stage is done before LDS is overwritten
 #pragma unroll
@@ -87,7 +123,7 @@ __device__ __forceinline__ void stockham(cf* v, float* lre, float* lim, int jt, 
                 int hi = (j / Ns) * Ns * R + (j & (Ns - 1));
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
-                    int o  = addr(hi + r * Ns);
+                    int o  = addr(hi, r * Ns);
                     lre[o] = v[m * R + r].x;
                     lim[o] = v[m * R + r].y;
                 }
