@@ -61,6 +61,11 @@ int     pcx_plan_destroy(pcx_plan_t plan);
 /* fft.hpp:620-622 size() */
 int64_t pcx_plan_size(pcx_plan_t plan);
 int     pcx_plan_order(pcx_plan_t plan);
+/* Arithmetic of the plan's radix-16 butterflies: PCX_PREC_FP32 (default) or PCX_PREC_BF16X3
+ * (matrix cores, three-term BF16 split operands, FP32 accumulation; normal order, N in
+ * [2^4, 2^20]).  No reference counterpart: the reference computes in FP32 only. */
+int     pcx_plan_set_precision(pcx_plan_t plan, int precision);
+int     pcx_plan_precision(pcx_plan_t plan);
 
 /* Forward FFT.  Replaces fft_raw<PD,PS>(dst, src[, src_size]) and operator()(dst, src)
  * (fft.hpp:624-687, 728-820).  In place when dst == src.  src_size <= fft_size values are read
@@ -95,6 +100,32 @@ int pcx_fxcorr_destroy(pcx_fxcorr_t xc);
 int pcx_fxcorr_exec(pcx_fxcorr_t xc, float* dst, int64_t dst_pack, const float* src, int64_t src_pack,
                     int64_t len, void* stream);
 int64_t pcx_fxcorr_step(pcx_fxcorr_t xc, int64_t src_pack);
+
+/* ------------------------------------------------------------------ uneven batches
+ * A work queue of transforms of uneven power-of-two sizes and mixed directions, natural order,
+ * each vector in split layout (pack = its size: re[N] then im[N]) at its own offset.  The
+ * reference runs one fft_unit per size and one call per vector; this is its batched form.
+ * Sizes up to 4096 run in one persistent launch that pulls work items from a device counter;
+ * larger sizes run as grouped single-pass or four-step launches.  pcx_queue_capture records the
+ * whole pass into a hipGraph for fixed (dst, src); pcx_queue_autotune times the launch shapes
+ * on the caller's data and keeps the fastest (optionally persisted in a cache file). */
+typedef struct {
+    int64_t log2n;      /* transform size 2^log2n, 1 <= log2n <= 24 */
+    int32_t inverse;    /* 0 forward, 1 inverse */
+    int32_t reserved;   /* 0 */
+    int64_t src_offset; /* floats from the source base */
+    int64_t dst_offset; /* floats from the destination base */
+} pcx_job;
+typedef struct pcx_queue_s* pcx_queue_t;
+
+int pcx_queue_create(const pcx_job* jobs, int64_t njobs, int normalized, pcx_queue_t* queue);
+int pcx_queue_destroy(pcx_queue_t queue);
+int pcx_queue_exec(pcx_queue_t queue, float* dst, const float* src, void* stream);
+int pcx_queue_autotune(pcx_queue_t queue, float* dst, const float* src, void* stream, const char* cache_path);
+int pcx_queue_capture(pcx_queue_t queue, float* dst, const float* src, void* stream);
+int pcx_queue_launch(pcx_queue_t queue, void* stream);
+/* algorithmic work of one pass: sum 5*N*log2(N) flops and 16*N bytes */
+int pcx_queue_work(pcx_queue_t queue, double* flops, double* bytes);
 
 /* ------------------------------------------------------------------ memory and utilities
  * The reference's Allocator template parameter (allocators.hpp:11-48) decides where buffers

@@ -24,6 +24,14 @@ ERR_HIP = 8
 ERR_OUT_OF_MEMORY = 9
 
 ORDER = {"normal": 0, "bit_reversed": 1, "unordered": 2}
+PRECISION = {"fp32": 0, "bf16x3": 1}
+
+class Job(C.Structure):
+    """pcx_job (include/pcx_gpu.h)."""
+
+    _fields_ = [("log2n", C.c_int64), ("inverse", C.c_int32), ("reserved", C.c_int32),
+                ("src_offset", C.c_int64), ("dst_offset", C.c_int64)]
+
 
 # Every symbol include/pcx_gpu.h declares, with (restype, argtypes).
 _vp, _i64, _int, _sz, _u64, _dbl = C.c_void_p, C.c_int64, C.c_int, C.c_size_t, C.c_uint64, C.c_double
@@ -34,6 +42,8 @@ SIGNATURES = {
     "pcx_plan_destroy": (_int, [_vp]),
     "pcx_plan_size": (_i64, [_vp]),
     "pcx_plan_order": (_int, [_vp]),
+    "pcx_plan_set_precision": (_int, [_vp, _int]),
+    "pcx_plan_precision": (_int, [_vp]),
     "pcx_fft": (_int, [_vp, _vp, _i64, _vp, _i64, _i64, _vp]),
     "pcx_ifft": (_int, [_vp, _int, _vp, _i64, _vp, _i64, _vp]),
     "pcx_fft_batch": (_int, [_vp, _int, _int, _vp, _i64, _i64, _vp, _i64, _i64, _i64, _i64, _vp]),
@@ -41,6 +51,13 @@ SIGNATURES = {
     "pcx_fxcorr_destroy": (_int, [_vp]),
     "pcx_fxcorr_exec": (_int, [_vp, _vp, _i64, _vp, _i64, _i64, _vp]),
     "pcx_fxcorr_step": (_i64, [_vp, _i64]),
+    "pcx_queue_create": (_int, [C.POINTER(Job), _i64, _int, C.POINTER(_vp)]),
+    "pcx_queue_destroy": (_int, [_vp]),
+    "pcx_queue_exec": (_int, [_vp, _vp, _vp, _vp]),
+    "pcx_queue_autotune": (_int, [_vp, _vp, _vp, _vp, C.c_char_p]),
+    "pcx_queue_capture": (_int, [_vp, _vp, _vp, _vp]),
+    "pcx_queue_launch": (_int, [_vp, _vp]),
+    "pcx_queue_work": (_int, [_vp, C.POINTER(_dbl), C.POINTER(_dbl)]),
     "pcx_malloc": (_int, [C.POINTER(_vp), _sz]),
     "pcx_free": (_int, [_vp]),
     "pcx_memcpy_htod": (_int, [_vp, _vp, _sz]),

@@ -35,6 +35,8 @@ struct RowArgs {
     const float2* twh;
     int           tw_ls, tw_lg;       // ls and log2(N)
     int           nt_load, nt_store;  // non-temporal (streaming) global loads / stores
+    const int64_t* src_offs;          // optional per-transform float offsets (instead of b * dist)
+    const int64_t* dst_offs;
 };
 
 struct ColArgs {
@@ -52,7 +54,17 @@ struct ColArgs {
     const float2* twh;
     int           tw_ls, tw_lg;
     int           nt_load, nt_store;
+    const int64_t* src_offs;
+    const int64_t* dst_offs;
 };
+
+// Base of transform b: b * dist floats, or offs[b] when a per-transform offset table is given.
+__device__ __forceinline__ const float* tbase(const float* base, const int64_t* offs, int64_t dist, int64_t b) {
+    return offs ? base + offs[b] : base + b * dist;
+}
+__device__ __forceinline__ float* tbase(float* base, const int64_t* offs, int64_t dist, int64_t b) {
+    return offs ? base + offs[b] : base + b * dist;
+}
 
 // 32-bit packed offset; lp is clamped so the shift stays defined (offsets are < 2^31).
 __device__ __forceinline__ uint32_t poff(uint32_t i, int lp) {
@@ -194,9 +206,9 @@ __device__ __forceinline__ int perm_elem(int jt, int u) {
         return (int)slot_freq(ORD, (uint32_t)(jt + u * P::TA), LA);
 }
 
-template<int LA, int G, int LD, int ST, int TW, bool UNI, int ORD>
+template<int LA, int G, int LD, int ST, int TW, bool UNI, int ORD, int PREC = PREC_FP32>
 __device__ __forceinline__ void row_group(const RowArgs& p, int64_t wg, float* lre, float* lim, const float* ltw,
-                                          const float* lptw) {
+                                          const float* lptw, const MfmaDft16* mf = nullptr) {
     using P           = Plan<LA>;
     using S           = RowShape<LA, G>;
     constexpr int TA  = P::TA;
@@ -223,7 +235,7 @@ __device__ __forceinline__ void row_group(const RowArgs& p, int64_t wg, float* l
     cf v[PPT];
     // ------------------------------------------------------------------ load
     if constexpr (LD == LD_CONTIG) {
-        const RdView<UNI> src(p.src + btu * p.src_dist, p.src_lp);
+        const RdView<UNI> src(tbase(p.src, p.src_offs, p.src_dist, btu), p.src_lp);
         if (row_ok && (int64_t)(k2 + 1) << LA <= p.src_size) { // whole row inside the source
 #pragma unroll
             for (int m = 0; m < PPT / R0; ++m)
@@ -247,7 +259,7 @@ __device__ __forceinline__ void row_group(const RowArgs& p, int64_t wg, float* l
         if constexpr (LD == LD_PERM) {
             // this row's spectrum occupies slots [A*brev(k2), A*brev(k2) + A); local slot s holds
             // element slot_freq(order, s)
-            const RdView<UNI> src(p.src + btu * p.src_dist, p.src_lp);
+            const RdView<UNI> src(tbase(p.src, p.src_offs, p.src_dist, btu), p.src_lp);
             uint32_t          base = brev_bits(k2, p.rows_lg) << LA;
 #pragma unroll
             for (int u = 0; u < PPT; ++u) {
@@ -264,7 +276,7 @@ __device__ __forceinline__ void row_group(const RowArgs& p, int64_t wg, float* l
             }
         } else { // LD_STRIDED: element k1 of row k2 sits at slot k2 + (k1 << rows_lg)
             // (the rows of a workgroup lie in one transform: G divides the rows per transform)
-            const RdView<UNI> src(p.src + (row0 >> p.rows_lg) * p.src_dist, p.src_lp);
+            const RdView<UNI> src(tbase(p.src, p.src_offs, p.src_dist, row0 >> p.rows_lg), p.src_lp);
 #pragma unroll
             for (int u = 0; u < PPT; ++u) {
                 int     idx = t + u * NT;
@@ -296,7 +308,7 @@ __device__ __forceinline__ void row_group(const RowArgs& p, int64_t wg, float* l
     }
     // ------------------------------------------------------------------ transform
     // (the barrier before the first stage's LDS writes also orders them after the reads above)
-    stockham<LA, false>(v, lre, lim, jt, addr, ltw);
+    stockham<LA, false, decltype(addr), PREC>(v, lre, lim, jt, addr, ltw, mf);
 
     if constexpr (TW > 0) {
 #pragma unroll
@@ -323,7 +335,7 @@ __device__ __forceinline__ void row_group(const RowArgs& p, int64_t wg, float* l
     // ------------------------------------------------------------------ store
     if constexpr (ST == ST_CONTIG) {
         if (row_ok) {
-            const WrView<UNI> dst(p.dst + btu * p.dst_dist, p.dst_lp);
+            const WrView<UNI> dst(tbase(p.dst, p.dst_offs, p.dst_dist, btu), p.dst_lp);
 #pragma unroll
             for (int m = 0; m < PPT / RL; ++m)
 #pragma unroll
@@ -345,7 +357,7 @@ __device__ __forceinline__ void row_group(const RowArgs& p, int64_t wg, float* l
         __syncthreads();
         if constexpr (ST == ST_PERM) {
             if (row_ok) {
-                const WrView<UNI> dst(p.dst + btu * p.dst_dist, p.dst_lp);
+                const WrView<UNI> dst(tbase(p.dst, p.dst_offs, p.dst_dist, btu), p.dst_lp);
                 uint32_t          base = brev_bits(k2, p.rows_lg) << LA;
 #pragma unroll
                 for (int u = 0; u < PPT; ++u) {
@@ -359,7 +371,7 @@ __device__ __forceinline__ void row_group(const RowArgs& p, int64_t wg, float* l
                 }
             }
         } else { // ST_STRIDED: element k1 of row k2 goes to slot k2 + (k1 << rows_lg)
-            const WrView<UNI> dst(p.dst + (row0 >> p.rows_lg) * p.dst_dist, p.dst_lp);
+            const WrView<UNI> dst(tbase(p.dst, p.dst_offs, p.dst_dist, row0 >> p.rows_lg), p.dst_lp);
 #pragma unroll
             for (int u = 0; u < PPT; ++u) {
                 int     idx = t + u * NT;
@@ -382,8 +394,12 @@ constexpr int row_lds_floats() {
 }
 
 // Grid-stride over row groups: gridDim.x may be smaller than the number of groups.
-template<int LA, int G, int LD, int ST, int TW, bool UNI, int ORD>
-__global__ __launch_bounds__(G * Plan<LA>::TA, waves_per_simd(G * Plan<LA>::TA, 4 * row_lds_floats<LA, G, TW>()))
+// The matrix-core variants keep the split operands in registers: they run at half the waves.
+constexpr int mfma_waves(int w, int prec) { return prec == PREC_FP32 ? w : (w > 2 ? w / 2 : 1); }
+
+template<int LA, int G, int LD, int ST, int TW, bool UNI, int ORD, int PREC = PREC_FP32>
+__global__ __launch_bounds__(G * Plan<LA>::TA,
+                             mfma_waves(waves_per_simd(G * Plan<LA>::TA, 4 * row_lds_floats<LA, G, TW>()), PREC))
 void k_row(RowArgs p) {
     constexpr int    NT = G * Plan<LA>::TA;
     __shared__ float lds[row_lds_floats<LA, G, TW>()];
@@ -395,16 +411,19 @@ void k_row(RowArgs p) {
     if constexpr (TW > 0)
         PassTw<TW>::fill(lptw, p.twl, p.twh, p.tw_ls, p.tw_lg, threadIdx.x, NT);
     __syncthreads();
+    MfmaDft16 mf;
+    if constexpr (PREC == PREC_BF16X3)
+        mf.init();
     const int64_t nwg = (p.nrows + G - 1) / G;
     for (int64_t wg = blockIdx.x; wg < nwg; wg += gridDim.x) {
-        row_group<LA, G, LD, ST, TW, UNI, ORD>(p, wg, lre, lim, ltw, lptw);
+        row_group<LA, G, LD, ST, TW, UNI, ORD, PREC>(p, wg, lre, lim, ltw, lptw, &mf);
         __syncthreads(); // LDS is reused by the next group
     }
 }
 
-template<int LA, int C, int TW>
+template<int LA, int C, int TW, int PREC = PREC_FP32>
 __device__ __forceinline__ void col_tile(const ColArgs& p, int64_t tile, float* lre, float* lim, const float* ltw,
-                                         const float* lptw) {
+                                         const float* lptw, const MfmaDft16* mf = nullptr) {
     using P           = Plan<LA>;
     constexpr int PPT = P::PPT;
     constexpr int R0  = P::first_radix;
@@ -417,8 +436,8 @@ __device__ __forceinline__ void col_tile(const ColArgs& p, int64_t tile, float* 
     const int64_t  bt   = tile >> ctl;
     const uint32_t n1   = (uint32_t)((tile & (((int64_t)1 << ctl) - 1)) * C) + c;
     auto           addr = [&](int b, int o) { return (b + o) * C + c; };
-    const BufR     src(p.src + bt * p.src_dist, p.src_lp);
-    const BufW     dst(p.dst + bt * p.dst_dist, p.dst_lp);
+    const BufR     src(tbase(p.src, p.src_offs, p.src_dist, bt), p.src_lp);
+    const BufW     dst(tbase(p.dst, p.dst_offs, p.dst_dist, bt), p.dst_lp);
 
     cf v[PPT];
     if (((int64_t)1 << (p.n1_lg + LA)) <= p.src_size) { // the whole transform is inside the source
@@ -445,7 +464,7 @@ __device__ __forceinline__ void col_tile(const ColArgs& p, int64_t tile, float* 
         for (int q = 0; q < PPT; ++q)
             v[q].y = -v[q].y;
     }
-    stockham<LA, false>(v, lre, lim, jt, addr, ltw);
+    stockham<LA, false, decltype(addr), PREC>(v, lre, lim, jt, addr, ltw, mf);
     if constexpr (TW > 0) {
 #pragma unroll
         for (int m = 0; m < PPT / RL; ++m)
@@ -482,8 +501,9 @@ constexpr int col_lds_floats() {
     return 2 * (1 << LA) * C + TwTable<LA>::FLOATS + 2 * TW;
 }
 
-template<int LA, int C, int TW>
-__global__ __launch_bounds__(C * Plan<LA>::TA, waves_per_simd(C * Plan<LA>::TA, 4 * col_lds_floats<LA, C, TW>()))
+template<int LA, int C, int TW, int PREC = PREC_FP32>
+__global__ __launch_bounds__(C * Plan<LA>::TA,
+                             mfma_waves(waves_per_simd(C * Plan<LA>::TA, 4 * col_lds_floats<LA, C, TW>()), PREC))
 void k_col(ColArgs p) {
     constexpr int    NT   = C * Plan<LA>::TA;
     constexpr int    DATA = 2 * (1 << LA) * C;
@@ -496,8 +516,11 @@ void k_col(ColArgs p) {
     if constexpr (TW > 0)
         PassTw<TW>::fill(lptw, p.twl, p.twh, p.tw_ls, p.tw_lg, threadIdx.x, NT);
     __syncthreads();
+    MfmaDft16 mf;
+    if constexpr (PREC == PREC_BF16X3)
+        mf.init();
     for (int64_t tile = blockIdx.x; tile < p.ntiles; tile += gridDim.x) {
-        col_tile<LA, C, TW>(p, tile, lre, lim, ltw, lptw);
+        col_tile<LA, C, TW, PREC>(p, tile, lre, lim, ltw, lptw, &mf);
         __syncthreads(); // LDS is reused by the next tile
     }
 }

@@ -6,6 +6,7 @@
 
 #include "fft_kernels.hpp"
 #include "fxcorr_kernels.hpp"
+#include "queue_kernels.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -16,6 +17,8 @@
 #include <mutex>
 #include <string>
 #include <utility>
+#include <vector>
+#include <algorithm>
 
 using namespace pcxk;
 
@@ -82,19 +85,20 @@ using ColLaunch = void (*)(const ColArgs&, int64_t grid_cap, hipStream_t);
 using XcLaunch  = void (*)(const XcArgs&, hipStream_t);
 
 // grid_cap > 0 bounds the grid; the kernels then loop over their work (grid-stride).
-template<int LA, int G, int LD, int ST, int TW, bool UNI = (Plan<LA>::TA >= 64), int ORD = 0>
+template<int LA, int G, int LD, int ST, int TW, bool UNI = (Plan<LA>::TA >= 64), int ORD = 0, int PREC = PREC_FP32>
 void launch_row(const RowArgs& a, int64_t nrows, int64_t grid_cap, hipStream_t st) {
     int64_t nwg = (nrows + G - 1) / G;
     if (grid_cap > 0 && nwg > grid_cap)
         nwg = grid_cap;
-    hipLaunchKernelGGL((k_row<LA, G, LD, ST, TW, UNI, ORD>), dim3((unsigned)nwg), dim3(G * Plan<LA>::TA), 0, st, a);
+    hipLaunchKernelGGL((k_row<LA, G, LD, ST, TW, UNI, ORD, PREC>), dim3((unsigned)nwg), dim3(G * Plan<LA>::TA), 0, st,
+                       a);
 }
-template<int LA, int C, int TW>
+template<int LA, int C, int TW, int PREC = PREC_FP32>
 void launch_col(const ColArgs& a, int64_t grid_cap, hipStream_t st) {
     int64_t nwg = a.ntiles;
     if (grid_cap > 0 && nwg > grid_cap)
         nwg = grid_cap;
-    hipLaunchKernelGGL((k_col<LA, C, TW>), dim3((unsigned)nwg), dim3(C * Plan<LA>::TA), 0, st, a);
+    hipLaunchKernelGGL((k_col<LA, C, TW, PREC>), dim3((unsigned)nwg), dim3(C * Plan<LA>::TA), 0, st, a);
 }
 template<int LA>
 void launch_xc(const XcArgs& a, hipStream_t st) {
@@ -131,6 +135,13 @@ constexpr auto xc_table(std::integer_sequence<int, L...>) {
 }
 // index LA-1 for LA in [1, 14]
 const auto kSingle = single_table(std::make_integer_sequence<int, kMaxSinglePass>{});
+// split-BF16 matrix-core variants (natural order), index LA-4 for LA in [4, 14]
+template<int... L>
+constexpr auto single_mfma_table(std::integer_sequence<int, L...>) {
+    return std::array<RowLaunch, sizeof...(L)>{
+        &launch_row<L + 4, default_rows<L + 4>(), LD_CONTIG, ST_CONTIG, 0, (Plan<L + 4>::TA >= 64), 0, PREC_BF16X3>...};
+}
+const auto kSingleMfma = single_mfma_table(std::make_integer_sequence<int, kMaxSinglePass - 3>{});
 const auto kXc     = xc_table(std::make_integer_sequence<int, kMaxSinglePass>{});
 
 using PassSet = std::array<RowLaunch, 6>;
@@ -180,6 +191,31 @@ const std::array<FourStep, 20> kShapes = {
     shape<16, 10, 6, 16, 64>(), shape<16, 7, 9, 64, 32>(), shape<16, 6, 10, 64, 16>(),
     shape<16, 9, 7, 32, 32>(), shape<16, 10, 6, 16, 32>(),
 };
+// split-BF16 matrix-core variants of the default shapes for 2^15 .. 2^20 (natural order):
+// rows {forward, inverse}, cols {inverse, forward}
+struct FourStepMfma {
+    int                      lg;
+    std::array<RowLaunch, 2> rows;
+    std::array<ColLaunch, 2> cols;
+};
+template<int LG, int N1, int N2, int G, int C>
+FourStepMfma shape_mfma() {
+    return {LG,
+            {&launch_row<N1, G, LD_CONTIG, ST_STRIDED, 0, true, 0, PREC_BF16X3>,
+             &launch_row<N1, G, LD_STRIDED, ST_CONTIG, tw_cap<LG>(), true, 0, PREC_BF16X3>},
+            {&launch_col<N2, C, 0, PREC_BF16X3>, &launch_col<N2, C, tw_cap<LG>(), PREC_BF16X3>}};
+}
+const std::array<FourStepMfma, 6> kShapesMfma = {shape_mfma<15, 8, 7, 32, 64>(),  shape_mfma<16, 9, 7, 32, 64>(),
+                                                 shape_mfma<17, 9, 8, 32, 64>(),  shape_mfma<18, 9, 9, 32, 32>(),
+                                                 shape_mfma<19, 10, 9, 16, 32>(), shape_mfma<20, 10, 10, 16, 16>()};
+const FourStep* find_shape(int lg, int n2, int g, int c);
+const FourStepMfma* find_mfma(int lg, const FourStep* sh) {
+    for (const auto& f : kShapesMfma)
+        if (f.lg == lg && sh == find_shape(lg, 0, 0, 0))
+            return &f;
+    return nullptr;
+}
+
 const FourStep* find_shape(int lg, int n2, int g, int c) {
     for (const auto& f : kShapes)
         if (f.lg == lg && (n2 == 0 || f.n2 == n2) && (g == 0 || f.g == g) && (c == 0 || f.c == c))
@@ -205,6 +241,7 @@ struct pcx_plan_s {
     int64_t scratch_cap  = 0; // transforms the scratch holds
     int64_t chunk        = 0; // transforms per four-step chunk
     const FourStep* shape = nullptr; // four-step kernels and tile sizes
+    int     precision    = PCX_PREC_FP32;
     int64_t grid_col     = 1024; // grid caps of the two passes (persistent loop; 0: one workgroup per tile)
     int64_t grid_row     = 1024;
     int     nt_flags     = 0; // bit 0/1: streaming load/store of src/dst, bit 2/3: of the scratch
@@ -316,13 +353,57 @@ extern "C" int pcx_plan_destroy(pcx_plan_t p) {
 }
 
 extern "C" int64_t pcx_plan_size(pcx_plan_t p) { return p ? p->n : 0; }
+
+extern "C" int pcx_plan_set_precision(pcx_plan_t p, int precision) {
+    if (!p)
+        return fail(PCX_ERR_INVALID_ARGUMENT, "plan is null");
+    if (precision == PCX_PREC_FP32) {
+        p->precision = precision;
+        return PCX_SUCCESS;
+    }
+    if (precision != PCX_PREC_BF16X3)
+        return fail(PCX_ERR_UNSUPPORTED, "precision " + s(precision) + " is not implemented");
+    if (p->order != PCX_ORDER_NORMAL)
+        return fail(PCX_ERR_UNSUPPORTED, "split-BF16 transforms are built for the normal order only");
+    if (p->lg < 4 || p->lg > 20 || (p->two_pass && !find_mfma(p->lg, p->shape)))
+        return fail(PCX_ERR_UNSUPPORTED, "split-BF16 transforms cover 2^4 .. 2^20 (default four-step shapes)");
+    p->precision = precision;
+    return PCX_SUCCESS;
+}
+extern "C" int pcx_plan_precision(pcx_plan_t p) { return p ? p->precision : -1; }
 extern "C" int     pcx_plan_order(pcx_plan_t p) { return p ? p->order : -1; }
 
 namespace {
 
+// Switches a four-step plan to another instantiated shape, rebuilding the W_N1 / W_N2 tables
+// when the split changes (the pass tables W_N do not depend on the split).
+int plan_apply_shape(pcx_plan_t p, const FourStep* sh) {
+    if (!p->two_pass || !sh || sh->lg != p->lg)
+        return fail(PCX_ERR_INVALID_ARGUMENT, "shape does not match the plan");
+    if (sh->n1 != p->n1_lg || sh->n2 != p->n2_lg) {
+        HIPCHK(hipDeviceSynchronize());
+        HIPCHK(hipFree(p->tw_row));
+        HIPCHK(hipFree(p->tw_col));
+        p->tw_row = p->tw_col = nullptr;
+        int64_t n1 = (int64_t)1 << sh->n1, n2 = (int64_t)1 << sh->n2;
+        HIPCHK(hipMalloc(&p->tw_row, sizeof(float2) * n1));
+        HIPCHK(hipMalloc(&p->tw_col, sizeof(float2) * n2));
+        HIPCHK(gen_twiddles(p->tw_row, n1, 1, n1));
+        HIPCHK(gen_twiddles(p->tw_col, n2, 1, n2));
+        HIPCHK(hipDeviceSynchronize());
+    }
+    p->shape = sh, p->n1_lg = sh->n1, p->n2_lg = sh->n2;
+    return PCX_SUCCESS;
+}
+
+} // namespace
+
+namespace {
+
+// src_offs / dst_offs (optional, device arrays of `batch` float offsets) replace b * dist.
 int run_batch(pcx_plan_t p, int inverse, int normalized, float* dst, int64_t dst_pack, int64_t dst_dist,
               const float* src, int64_t src_pack, int64_t src_dist, int64_t src_size, int64_t batch,
-              hipStream_t st) {
+              hipStream_t st, const int64_t* src_offs = nullptr, const int64_t* dst_offs = nullptr) {
     if (!p)
         return fail(PCX_ERR_INVALID_ARGUMENT, "plan is null");
     if (!dst || !src)
@@ -363,7 +444,11 @@ int run_batch(pcx_plan_t p, int inverse, int normalized, float* dst, int64_t dst
         a.src_size = src_size, a.nrows = batch, a.src_lp = src_lp, a.dst_lp = dst_lp, a.rows_lg = 0;
         a.order = p->order, a.conj_in = inverse, a.conj_out = inverse, a.scale = scale;
         a.twA = p->tw_row, a.twl = nullptr, a.twh = nullptr, a.tw_ls = 0;
-        kSingle[p->lg - 1][single_variant(inverse, p->order)](a, batch, 0, st);
+        a.src_offs = src_offs, a.dst_offs = dst_offs;
+        if (p->precision == PCX_PREC_BF16X3)
+            kSingleMfma[p->lg - 4](a, batch, 0, st);
+        else
+            kSingle[p->lg - 1][single_variant(inverse, p->order)](a, batch, 0, st);
         HIPCHK(hipGetLastError());
         return PCX_SUCCESS;
     }
@@ -385,11 +470,20 @@ int run_batch(pcx_plan_t p, int inverse, int normalized, float* dst, int64_t dst
     const int C    = p->shape->c;
     ColLaunch colf = p->shape->cols[inverse ? 0 : 1];
     RowLaunch rowf = p->shape->rows[(inverse ? 3 : 0) + p->order];
+    if (p->precision == PCX_PREC_BF16X3) {
+        const FourStepMfma* fm = find_mfma(p->lg, p->shape);
+        if (!fm)
+            return fail(PCX_ERR_UNSUPPORTED, "no split-BF16 kernels for this four-step shape");
+        colf = fm->cols[inverse ? 0 : 1];
+        rowf = fm->rows[inverse ? 1 : 0];
+    }
     const int64_t n1 = (int64_t)1 << p->n1_lg, n2 = (int64_t)1 << p->n2_lg;
     for (int64_t b0 = 0; b0 < batch; b0 += chunk) {
         int64_t      cb = batch - b0 < chunk ? batch - b0 : chunk;
-        const float* cs = src + b0 * src_dist;
-        float*       cd = dst + b0 * dst_dist;
+        const float*   cs  = src_offs ? src : src + b0 * src_dist;
+        float*         cd  = dst_offs ? dst : dst + b0 * dst_dist;
+        const int64_t* cso = src_offs ? src_offs + b0 : nullptr;
+        const int64_t* cdo = dst_offs ? dst_offs + b0 : nullptr;
         ColArgs      ca{};
         RowArgs      ra{};
         ca.twA = p->tw_col, ca.n1_lg = p->n1_lg, ca.ntiles = cb * (n1 / C);
@@ -397,7 +491,7 @@ int run_batch(pcx_plan_t p, int inverse, int normalized, float* dst, int64_t dst
         ca.tw_lg = p->lg, ra.tw_lg = p->lg;
         if (!inverse) {
             // pass A: columns of src -> scratch, times W_N^(n1*k2)
-            ca.src = cs, ca.src_dist = src_dist, ca.src_lp = src_lp, ca.src_size = src_size;
+            ca.src = cs, ca.src_dist = src_dist, ca.src_lp = src_lp, ca.src_size = src_size, ca.src_offs = cso;
             ca.dst = p->scratch, ca.dst_dist = 2 * n, ca.dst_lp = p->lg;
             ca.conj_in = 0, ca.conj_out = 0, ca.scale = 1.0f;
             ca.twl = p->twl, ca.twh = p->twh, ca.tw_ls = p->tw_ls;
@@ -405,14 +499,14 @@ int run_batch(pcx_plan_t p, int inverse, int normalized, float* dst, int64_t dst
             colf(ca, p->grid_col, st);
             // pass B: rows of scratch -> dst in the plan's order
             ra.src = p->scratch, ra.src_dist = 2 * n, ra.src_lp = p->lg, ra.src_size = n;
-            ra.dst = cd, ra.dst_dist = dst_dist, ra.dst_lp = dst_lp;
+            ra.dst = cd, ra.dst_dist = dst_dist, ra.dst_lp = dst_lp, ra.dst_offs = cdo;
             ra.conj_in = 0, ra.conj_out = 0, ra.scale = 1.0f;
             ra.twl = nullptr, ra.twh = nullptr;
             ra.nt_load = (p->nt_flags >> 3) & 1, ra.nt_store = (p->nt_flags >> 1) & 1;
             rowf(ra, ra.nrows, p->grid_row, st);
         } else {
             // pass 1: rows (over k1) of the spectrum -> scratch, times W_N^(n1*k2)
-            ra.src = cs, ra.src_dist = src_dist, ra.src_lp = src_lp, ra.src_size = n;
+            ra.src = cs, ra.src_dist = src_dist, ra.src_lp = src_lp, ra.src_size = n, ra.src_offs = cso;
             ra.dst = p->scratch, ra.dst_dist = 2 * n, ra.dst_lp = p->lg;
             ra.conj_in = 1, ra.conj_out = 0, ra.scale = 1.0f;
             ra.twl = p->twl, ra.twh = p->twh, ra.tw_ls = p->tw_ls;
@@ -420,7 +514,7 @@ int run_batch(pcx_plan_t p, int inverse, int normalized, float* dst, int64_t dst
             rowf(ra, ra.nrows, p->grid_row, st);
             // pass 2: columns (over k2) of scratch -> dst, natural order
             ca.src = p->scratch, ca.src_dist = 2 * n, ca.src_lp = p->lg, ca.src_size = n;
-            ca.dst = cd, ca.dst_dist = dst_dist, ca.dst_lp = dst_lp;
+            ca.dst = cd, ca.dst_dist = dst_dist, ca.dst_lp = dst_lp, ca.dst_offs = cdo;
             ca.conj_in = 0, ca.conj_out = 1, ca.scale = scale;
             ca.twl = nullptr, ca.twh = nullptr;
             ca.nt_load = (p->nt_flags >> 3) & 1, ca.nt_store = (p->nt_flags >> 1) & 1;
@@ -648,5 +742,375 @@ extern "C" int pcx_generate(float* dst, int64_t pack, int64_t n, uint64_t seed, 
     hipLaunchKernelGGL(k_generate, dim3((unsigned)nb), dim3(256), 0, (hipStream_t)stream, dst, ilog2(pack), n, seed,
                        dist, sigma);
     HIPCHK(hipGetLastError());
+    return PCX_SUCCESS;
+}
+
+// ---------------------------------------------------------------- uneven-batch work queue
+namespace {
+
+struct QGroup { // transforms of one size and direction outside the persistent kernel
+    int      lg   = 0;
+    int      inv  = 0;
+    int64_t  count = 0;
+    int64_t* src_offs = nullptr; // device
+    int64_t* dst_offs = nullptr;
+    pcx_plan_t plan = nullptr;
+};
+
+} // namespace
+
+struct pcx_queue_s {
+    int                 normalized = 1;
+    int64_t             njobs      = 0;
+    double              flops = 0, bytes = 0;
+    // persistent part
+    int32_t             nitems    = 0;
+    QItem*              items     = nullptr; // device
+    int64_t*            q_src_offs = nullptr;
+    int64_t*            q_dst_offs = nullptr;
+    uint32_t*           counter   = nullptr;
+    float2*             tw        = nullptr; // W_A for A = 2 .. 4096
+    int                 grid      = 1024;
+    // grouped part
+    std::vector<QGroup> groups;
+    // captured graph
+    hipGraph_t          graph = nullptr;
+    hipGraphExec_t      exec  = nullptr;
+};
+
+namespace {
+
+int queue_run(pcx_queue_t q, float* dst, const float* src, hipStream_t st) {
+    if (q->nitems > 0) {
+        hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, st, q->counter);
+        QueueArgs a{};
+        a.src = src, a.dst = dst, a.src_offs = q->q_src_offs, a.dst_offs = q->q_dst_offs;
+        a.items = q->items, a.nitems = q->nitems, a.normalized = q->normalized, a.counter = q->counter, a.tw = q->tw;
+        int grid = q->grid < q->nitems ? q->grid : q->nitems;
+        hipLaunchKernelGGL(k_queue, dim3(grid), dim3(kQueueThreads), 0, st, a);
+        HIPCHK(hipGetLastError());
+    }
+    for (auto& g : q->groups) {
+        int64_t n = (int64_t)1 << g.lg;
+        if (int e = run_batch(g.plan, g.inv, q->normalized, dst, n, 2 * n, src, n, 2 * n, n, g.count, st, g.src_offs,
+                              g.dst_offs))
+            return e;
+    }
+    return PCX_SUCCESS;
+}
+
+void queue_free(pcx_queue_t q) {
+    if (q->exec)
+        (void)hipGraphExecDestroy(q->exec);
+    if (q->graph)
+        (void)hipGraphDestroy(q->graph);
+    (void)hipFree(q->items);
+    (void)hipFree(q->q_src_offs);
+    (void)hipFree(q->q_dst_offs);
+    (void)hipFree(q->counter);
+    (void)hipFree(q->tw);
+    for (auto& g : q->groups) {
+        (void)hipFree(g.src_offs);
+        (void)hipFree(g.dst_offs);
+        pcx_plan_destroy(g.plan);
+    }
+    delete q;
+}
+
+} // namespace
+
+extern "C" int pcx_queue_create(const pcx_job* jobs, int64_t njobs, int normalized, pcx_queue_t* out) {
+    if (!out)
+        return fail(PCX_ERR_INVALID_ARGUMENT, "queue output pointer is null");
+    *out = nullptr;
+    if (njobs < 0 || (njobs > 0 && !jobs))
+        return fail(PCX_ERR_INVALID_ARGUMENT, "job list is null or negative");
+    for (int64_t i = 0; i < njobs; ++i) {
+        if (jobs[i].log2n < 1 || jobs[i].log2n > kMaxLog2)
+            return fail(PCX_ERR_INVALID_SIZE, "fft_size (which is  2^" + s(jobs[i].log2n) + ") is not supported");
+        if (jobs[i].inverse != 0 && jobs[i].inverse != 1)
+            return fail(PCX_ERR_INVALID_ARGUMENT, "job " + s(i) + ": inverse must be 0 or 1");
+        if (jobs[i].src_offset < 0 || jobs[i].dst_offset < 0)
+            return fail(PCX_ERR_INVALID_ARGUMENT, "job " + s(i) + ": negative offset");
+    }
+    auto* q       = new pcx_queue_s;
+    q->normalized = normalized ? 1 : 0;
+    q->njobs      = njobs;
+    // bucket by (size, direction)
+    std::vector<std::vector<int64_t>> bucket((kMaxLog2 + 1) * 2);
+    for (int64_t i = 0; i < njobs; ++i) {
+        bucket[jobs[i].log2n * 2 + jobs[i].inverse].push_back(i);
+        double n = (double)((int64_t)1 << jobs[i].log2n);
+        q->flops += 5.0 * n * (double)jobs[i].log2n;
+        q->bytes += 16.0 * n;
+    }
+    std::vector<QItem>   items;
+    std::vector<int64_t> qs, qd;
+    hipError_t           he = hipSuccess;
+    // persistent items, largest size first (longest-processing-time order)
+    for (int lg = kQueueMaxLog2; lg >= 1; --lg) {
+        int per = kQueueThreads / (lg >= 4 ? (1 << lg) / 16 : 1);
+        for (int inv = 0; inv < 2; ++inv) {
+            auto& b = bucket[lg * 2 + inv];
+            for (size_t k = 0; k < b.size(); k += per) {
+                int cnt = (int)std::min<size_t>(per, b.size() - k);
+                items.push_back({lg, inv, (int32_t)qs.size(), cnt});
+                for (int c = 0; c < cnt; ++c) {
+                    qs.push_back(jobs[b[k + c]].src_offset);
+                    qd.push_back(jobs[b[k + c]].dst_offset);
+                }
+            }
+        }
+    }
+    q->nitems = (int32_t)items.size();
+    if (!items.empty()) {
+        he = hipMalloc(&q->items, sizeof(QItem) * items.size());
+        if (he == hipSuccess)
+            he = hipMalloc(&q->q_src_offs, sizeof(int64_t) * qs.size());
+        if (he == hipSuccess)
+            he = hipMalloc(&q->q_dst_offs, sizeof(int64_t) * qd.size());
+        if (he == hipSuccess)
+            he = hipMalloc(&q->counter, sizeof(uint32_t));
+        if (he == hipSuccess)
+            he = hipMalloc(&q->tw, sizeof(float2) * ((1 << (kQueueMaxLog2 + 1)) - 2));
+        if (he == hipSuccess)
+            he = hipMemcpy(q->items, items.data(), sizeof(QItem) * items.size(), hipMemcpyHostToDevice);
+        if (he == hipSuccess)
+            he = hipMemcpy(q->q_src_offs, qs.data(), sizeof(int64_t) * qs.size(), hipMemcpyHostToDevice);
+        if (he == hipSuccess)
+            he = hipMemcpy(q->q_dst_offs, qd.data(), sizeof(int64_t) * qd.size(), hipMemcpyHostToDevice);
+        for (int lg = 1; lg <= kQueueMaxLog2 && he == hipSuccess; ++lg)
+            he = gen_twiddles(q->tw + ((1 << lg) - 2), (int64_t)1 << lg, 1, (int64_t)1 << lg);
+    }
+    // grouped launches for the larger sizes, largest first
+    for (int lg = kMaxLog2; lg > kQueueMaxLog2 && he == hipSuccess; --lg) {
+        for (int inv = 0; inv < 2; ++inv) {
+            auto& b = bucket[lg * 2 + inv];
+            if (b.empty())
+                continue;
+            QGroup g;
+            g.lg = lg, g.inv = inv, g.count = (int64_t)b.size();
+            std::vector<int64_t> so(b.size()), dof(b.size());
+            for (size_t k = 0; k < b.size(); ++k)
+                so[k] = jobs[b[k]].src_offset, dof[k] = jobs[b[k]].dst_offset;
+            he = hipMalloc(&g.src_offs, sizeof(int64_t) * b.size());
+            if (he == hipSuccess)
+                he = hipMalloc(&g.dst_offs, sizeof(int64_t) * b.size());
+            if (he == hipSuccess)
+                he = hipMemcpy(g.src_offs, so.data(), sizeof(int64_t) * b.size(), hipMemcpyHostToDevice);
+            if (he == hipSuccess)
+                he = hipMemcpy(g.dst_offs, dof.data(), sizeof(int64_t) * b.size(), hipMemcpyHostToDevice);
+            if (he == hipSuccess) {
+                if (int e = pcx_plan_create((int64_t)1 << lg, 2048, PCX_ORDER_NORMAL, &g.plan)) {
+                    q->groups.push_back(g);
+                    queue_free(q);
+                    return e;
+                }
+            }
+            q->groups.push_back(g);
+        }
+    }
+    if (he == hipSuccess)
+        he = hipDeviceSynchronize();
+    if (he != hipSuccess) {
+        queue_free(q);
+        return fail(PCX_ERR_HIP, std::string("queue setup: ") + hipGetErrorString(he));
+    }
+    *out = q;
+    return PCX_SUCCESS;
+}
+
+extern "C" int pcx_queue_destroy(pcx_queue_t q) {
+    if (q)
+        queue_free(q);
+    return PCX_SUCCESS;
+}
+
+extern "C" int pcx_queue_exec(pcx_queue_t q, float* dst, const float* src, void* stream) {
+    if (!q)
+        return fail(PCX_ERR_INVALID_ARGUMENT, "queue is null");
+    if (!dst || !src)
+        return fail(PCX_ERR_INVALID_ARGUMENT, "data pointer is null");
+    return queue_run(q, dst, src, (hipStream_t)stream);
+}
+
+extern "C" int pcx_queue_capture(pcx_queue_t q, float* dst, const float* src, void* stream) {
+    if (!q)
+        return fail(PCX_ERR_INVALID_ARGUMENT, "queue is null");
+    if (!dst || !src)
+        return fail(PCX_ERR_INVALID_ARGUMENT, "data pointer is null");
+    hipStream_t st = (hipStream_t)stream;
+    if (!st)
+        return fail(PCX_ERR_INVALID_ARGUMENT, "graph capture needs a non-default stream");
+    // make sure every group's scratch exists before capture (allocation is not capturable)
+    for (auto& g : q->groups) {
+        int64_t n = (int64_t)1 << g.lg;
+        if (g.plan->two_pass && g.plan->scratch_cap < std::min(g.count, g.plan->chunk)) {
+            if (int e = run_batch(g.plan, g.inv, q->normalized, dst, n, 2 * n, src, n, 2 * n, n, g.count, st,
+                                  g.src_offs, g.dst_offs))
+                return e;
+        }
+    }
+    HIPCHK(hipStreamSynchronize(st));
+    if (q->exec) {
+        HIPCHK(hipGraphExecDestroy(q->exec));
+        q->exec = nullptr;
+    }
+    if (q->graph) {
+        HIPCHK(hipGraphDestroy(q->graph));
+        q->graph = nullptr;
+    }
+    HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    int        e  = queue_run(q, dst, src, st);
+    hipGraph_t gr = nullptr;
+    hipError_t he = hipStreamEndCapture(st, &gr);
+    if (e)
+        return e;
+    if (he != hipSuccess)
+        return fail(PCX_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(he));
+    q->graph = gr;
+    HIPCHK(hipGraphInstantiate(&q->exec, q->graph, nullptr, nullptr, 0));
+    return PCX_SUCCESS;
+}
+
+extern "C" int pcx_queue_launch(pcx_queue_t q, void* stream) {
+    if (!q || !q->exec)
+        return fail(PCX_ERR_INVALID_ARGUMENT, "queue has no captured graph");
+    HIPCHK(hipGraphLaunch(q->exec, (hipStream_t)stream));
+    return PCX_SUCCESS;
+}
+
+extern "C" int pcx_queue_work(pcx_queue_t q, double* flops, double* bytes) {
+    if (!q)
+        return fail(PCX_ERR_INVALID_ARGUMENT, "queue is null");
+    if (flops)
+        *flops = q->flops;
+    if (bytes)
+        *bytes = q->bytes;
+    return PCX_SUCCESS;
+}
+
+namespace {
+
+// Times fn (which enqueues work on st) with events: best of `reps` after one warm-up.
+template<class F>
+float time_ms(hipStream_t st, F&& fn, int reps = 3) {
+    hipEvent_t a, b;
+    if (hipEventCreate(&a) != hipSuccess)
+        return 1e30f;
+    if (hipEventCreate(&b) != hipSuccess) {
+        (void)hipEventDestroy(a);
+        return 1e30f;
+    }
+    float best = 1e30f;
+    if (fn() != PCX_SUCCESS)
+        best = -1.f;
+    for (int r = 0; r < reps && best >= 0.f; ++r) {
+        float ms = 0.f;
+        (void)hipEventRecord(a, st);
+        if (fn() != PCX_SUCCESS) {
+            best = -1.f;
+            break;
+        }
+        (void)hipEventRecord(b, st);
+        (void)hipEventSynchronize(b);
+        (void)hipEventElapsedTime(&ms, a, b);
+        best = ms < best ? ms : best;
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return best < 0.f ? 1e30f : best;
+}
+
+} // namespace
+
+// Autotuner: persistent grid size, then per group the four-step shape and grid caps.  Winners are
+// written to cache_path (one "key value..." line each) and read back from it when present.
+extern "C" int pcx_queue_autotune(pcx_queue_t q, float* dst, const float* src, void* stream, const char* cache_path) {
+    if (!q)
+        return fail(PCX_ERR_INVALID_ARGUMENT, "queue is null");
+    if (!dst || !src)
+        return fail(PCX_ERR_INVALID_ARGUMENT, "data pointer is null");
+    hipStream_t st = (hipStream_t)stream;
+    if (cache_path) { // reuse earlier decisions
+        if (FILE* f = std::fopen(cache_path, "r")) {
+            char key[32];
+            int  a = 0, b = 0, c = 0, d = 0, e = 0;
+            long long gc = 0, gr = 0;
+            while (std::fscanf(f, "%31s", key) == 1) {
+                if (!std::strcmp(key, "queue_grid") && std::fscanf(f, "%d", &a) == 1)
+                    q->grid = a;
+                else if (!std::strcmp(key, "group") && std::fscanf(f, "%d %d %d %d %d %lld %lld", &a, &b, &c, &d, &e, &gc, &gr) == 7) {
+                    for (auto& g : q->groups)
+                        if (g.lg == a && g.inv == b && g.plan->two_pass) {
+                            if (const FourStep* sh = find_shape(a, c, d, e)) {
+                                if (int err = plan_apply_shape(g.plan, sh)) {
+                                    std::fclose(f);
+                                    return err;
+                                }
+                                g.plan->grid_col = gc, g.plan->grid_row = gr;
+                            }
+                        }
+                } else {
+                    break;
+                }
+            }
+            std::fclose(f);
+            return PCX_SUCCESS;
+        }
+    }
+    if (q->nitems > 0) {
+        float best = 1e30f;
+        int   bg   = q->grid;
+        for (int grid : {256, 512, 1024, 2048, 4096}) {
+            q->grid  = grid;
+            float ms = time_ms(st, [&] {
+                std::vector<QGroup> keep;
+                keep.swap(q->groups); // time the persistent part alone
+                int e = queue_run(q, dst, src, st);
+                keep.swap(q->groups);
+                return e;
+            });
+            if (ms < best)
+                best = ms, bg = grid;
+        }
+        q->grid = bg;
+    }
+    for (auto& g : q->groups) {
+        if (!g.plan->two_pass)
+            continue;
+        int64_t         n    = (int64_t)1 << g.lg;
+        const FourStep* best = g.plan->shape;
+        int64_t         bgc = g.plan->grid_col, bgr = g.plan->grid_row;
+        float           bms = 1e30f;
+        for (const auto& sh : kShapes) {
+            if (sh.lg != g.lg)
+                continue;
+            if (int err = plan_apply_shape(g.plan, &sh))
+                return err;
+            for (int64_t gc : {(int64_t)0, (int64_t)1024, (int64_t)2048}) {
+                g.plan->grid_col = g.plan->grid_row = gc;
+                float ms = time_ms(st, [&] {
+                    return run_batch(g.plan, g.inv, q->normalized, dst, n, 2 * n, src, n, 2 * n, n, g.count, st,
+                                     g.src_offs, g.dst_offs);
+                });
+                if (ms < bms)
+                    bms = ms, best = &sh, bgc = gc, bgr = gc;
+            }
+        }
+        if (int err = plan_apply_shape(g.plan, best))
+            return err;
+        g.plan->grid_col = bgc, g.plan->grid_row = bgr;
+    }
+    if (cache_path) {
+        if (FILE* f = std::fopen(cache_path, "w")) {
+            std::fprintf(f, "queue_grid %d\n", q->grid);
+            for (auto& g : q->groups)
+                if (g.plan->two_pass)
+                    std::fprintf(f, "group %d %d %d %d %d %lld %lld\n", g.lg, g.inv, g.plan->shape->n2, g.plan->shape->g,
+                                 g.plan->shape->c, (long long)g.plan->grid_col, (long long)g.plan->grid_row);
+            std::fclose(f);
+        }
+    }
     return PCX_SUCCESS;
 }

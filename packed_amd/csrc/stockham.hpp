@@ -20,6 +20,7 @@
 // A = 16384).  The table is filled once per workgroup from the FP32 table in global memory.
 #pragma once
 #include "dev_common.hpp"
+#include "mfma_dft.hpp"
 
 namespace pcxk {
 
@@ -78,8 +79,10 @@ struct TwTable {
     }
 };
 
-template<int LA, bool REV, class Addr>
-__device__ __forceinline__ void stockham(cf* v, float* lre, float* lim, int jt, const Addr& addr, const float* ltw) {
+// PREC_BF16X3 runs the radix-16 butterflies on the matrix cores (mf holds the operands).
+template<int LA, bool REV, class Addr, int PREC = PREC_FP32>
+__device__ __forceinline__ void stockham(cf* v, float* lre, float* lim, int jt, const Addr& addr, const float* ltw,
+                                         const MfmaDft16* mf = nullptr) {
     using P           = Plan<LA, REV>;
     constexpr int A   = P::A;
     constexpr int TA  = P::TA;
@@ -111,7 +114,10 @@ __device__ __forceinline__ void stockham(cf* v, float* lre, float* lim, int jt, 
                 for (int r = 1; r < R; ++r)
                     v[m * R + r] = cmul(v[m * R + r], TwTable<LA>::get(ltw, (uint32_t)(r * k) << shift));
             }
-            dft<R>(v + m * R);
+            if constexpr (R == 16 && PREC == PREC_BF16X3)
+                mf->dft16(v + m * R);
+            else
+                dft<R>(v + m * R);
             if constexpr (NB > 1)
                 __builtin_amdgcn_sched_barrier(0);
         }

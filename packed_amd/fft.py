@@ -17,9 +17,9 @@ from __future__ import annotations
 import numpy as np
 
 from . import _lib
-from ._lib import ORDER, PcxError, check, lib
+from ._lib import ORDER, PRECISION, PcxError, check, lib
 
-__all__ = ["vector", "fft_unit", "fxcorr_unit", "PcxError", "real_size", "pidx"]
+__all__ = ["vector", "fft_unit", "fxcorr_unit", "fft_queue", "PcxError", "real_size", "pidx"]
 
 
 def _torch():
@@ -92,7 +92,7 @@ class vector:
 class fft_unit:
     """fft.hpp:565-915 fft_unit<float, Order> on the GPU."""
 
-    def __init__(self, fft_size: int, sub_size: int = 2048, order: str = "normal"):
+    def __init__(self, fft_size: int, sub_size: int = 2048, order: str = "normal", precision: str = "fp32"):
         if order not in ORDER:
             raise PcxError(_lib.ERR_INVALID_ARGUMENT, f"unknown fft_order {order!r}")
         import ctypes as C
@@ -101,6 +101,9 @@ class fft_unit:
         check(lib().pcx_plan_create(int(fft_size), int(sub_size), ORDER[order], C.byref(h)))
         self._h = h
         self.order = order
+        if precision != "fp32":
+            check(lib().pcx_plan_set_precision(h, PRECISION[precision]))
+        self.precision = precision
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -207,3 +210,45 @@ class fxcorr_unit:
                                                    f"the signal (which is {f.size()})")
         check(lib().pcx_fxcorr_exec(self._h, dst.data_ptr(), dst.pack, f.data_ptr(), f.pack, f.size(),
                                     _stream_ptr(stream)))
+
+
+class fft_queue:
+    """Work queue of uneven power-of-two transforms (pcx_queue_*): jobs = [(log2n, inverse,
+    src_offset, dst_offset)], each vector in split layout (re[N] then im[N]) at its float offset."""
+
+    def __init__(self, jobs, normalized: bool = True):
+        import ctypes as C
+
+        arr = (_lib.Job * len(jobs))()
+        for k, (lg, inv, so, do) in enumerate(jobs):
+            arr[k].log2n, arr[k].inverse, arr[k].reserved = int(lg), int(inv), 0
+            arr[k].src_offset, arr[k].dst_offset = int(so), int(do)
+        h = C.c_void_p()
+        check(lib().pcx_queue_create(arr, len(jobs), int(normalized), C.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value and _lib is not None and _lib._lib is not None:
+            _lib._lib.pcx_queue_destroy(h)
+            self._h = None
+
+    def exec(self, dst: int, src: int, stream=None) -> None:
+        check(lib().pcx_queue_exec(self._h, dst, src, _stream_ptr(stream)))
+
+    def autotune(self, dst: int, src: int, cache_path: str | None = None, stream=None) -> None:
+        check(lib().pcx_queue_autotune(self._h, dst, src, _stream_ptr(stream),
+                                       cache_path.encode() if cache_path else None))
+
+    def capture(self, dst: int, src: int, stream) -> None:
+        check(lib().pcx_queue_capture(self._h, dst, src, _stream_ptr(stream)))
+
+    def launch(self, stream) -> None:
+        check(lib().pcx_queue_launch(self._h, _stream_ptr(stream)))
+
+    def work(self):
+        import ctypes as C
+
+        f, b = C.c_double(), C.c_double()
+        check(lib().pcx_queue_work(self._h, C.byref(f), C.byref(b)))
+        return f.value, b.value
