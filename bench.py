@@ -398,7 +398,75 @@ def run_config4(args):
 
 
 def run_config5(args):
-    raise SystemExit("config 5 (split-precision MFMA) is not implemented yet")
+    """Split-precision matrix-core form (three-term BF16 operands, FP32 accumulation) against the
+    plain FP32 path: batched C2C N in {2^12, 2^16, 2^20} filling 2 GiB, plus 512-channel
+    correlation of 2^20 samples with a 1024-tap kernel (block size chosen by timing)."""
+    import numpy as np
+    import torch
+
+    import packed_amd as pa
+    from packed_amd._lib import check, lib
+    from oracle import oracle as O
+
+    pts = 1 << 28
+    x = torch.empty(2 * pts, dtype=torch.float32, device="cuda")
+    y = torch.empty(2 * pts, dtype=torch.float32, device="cuda")
+    st = torch.cuda.current_stream().cuda_stream
+    check(lib().pcx_generate(x.data_ptr(), 1, pts, 0x5EED0005, 1, 1.0, st))
+    res = {"metric": "split-BF16 MFMA vs FP32 FFT, GFLOP/s (5N*log2N), 2 GiB batches", "unit": "GFLOP/s",
+           "higher_is_better": True, "sizes": {},
+           "config": {"workload": "batched C2C FFT forward, N in {2^12, 2^16, 2^20}, 2^28 points, split re/im"}}
+    for lg in (12, 16, 20):
+        n = 1 << lg
+        batch = pts // n
+        row = {}
+        for prec in ("fp32", "bf16x3"):
+            u = pa.fft_unit(n, precision=prec)
+            ms = time_steps(lambda: u.batch(y.data_ptr(), x.data_ptr(), batch, n, 2 * n, n, 2 * n), 10, 2)
+            xin = x[:2 * n].cpu().numpy()
+            z = xin[:n].astype(np.float64) + 1j * xin[n:]
+            out = y[:2 * n].cpu().numpy()
+            err = O.rel_l2(out[:n] + 1j * out[n:].astype(np.float64), np.fft.fft(z))
+            row[prec] = {"ms": round(ms, 4), "gflops": round(5 * n * lg * batch / (ms * 1e-3) / 1e9, 1),
+                         "rel_l2_vs_fp64": float(f"{err:.3e}")}
+        row["speedup_bf16x3_over_fp32"] = round(row["fp32"]["ms"] / row["bf16x3"]["ms"], 3)
+        res["sizes"][f"2^{lg}"] = row
+    del x, y
+    torch.cuda.empty_cache()
+    # 512 channels x 2^20 samples, 1024-tap kernel
+    C, L, M = 512, 1 << 20, 1024
+    pack = 8
+    dist = 2 * L
+    sig = torch.empty(C * dist, dtype=torch.float32, device="cuda")
+    out = torch.empty_like(sig)
+    check(lib().pcx_generate(sig.data_ptr(), pack, C * L, 0x5EED0015, 1, math.sqrt(0.5), st))
+    g = np.exp(1j * np.pi * (np.real(O.gen_uniform(0x5EED0016, M)).astype(np.float64) + 1.0)).astype(np.complex64)
+    gv = pa.vector.from_complex(g, pack)
+    xres = {}
+    for prec in ("fp32", "bf16x3"):
+        best = None
+        for N in (4096, 8192, 16384):
+            xc = pa.fxcorr_unit(gv, N, precision=prec)
+            ms = time_steps(lambda: xc.batch(out.data_ptr(), sig.data_ptr(), L, C, pack, dist), 5, 1)
+            step = xc.step(pack)
+            blocks = C * (-(-L // step))
+            flops = blocks * (2 * 5 * N * math.log2(N) + 6 * N)
+            cand = {"fft_size": N, "ms": round(ms, 4), "gflops": round(flops / (ms * 1e-3) / 1e9, 1),
+                    "msamples_per_s": round(C * L / (ms * 1e-3) / 1e6, 1)}
+            if best is None or ms < best["ms"]:
+                best = cand
+                bestxc = xc
+        bestxc.batch(out.data_ptr(), sig.data_ptr(), L, C, pack, dist)
+        torch.cuda.synchronize()
+        s0 = sig[:dist].cpu().numpy()
+        f0 = O.from_packed(s0, 1 << 16, pack)
+        want = O.xcorr_naive(f0, g.astype(np.complex128))[: (1 << 16) - M]
+        got = O.from_packed(out[:dist].cpu().numpy(), (1 << 16) - M, pack)
+        best["rel_l2_vs_fp64_ch0_first_2^16"] = float(f"{O.rel_l2(got, want):.3e}")
+        xres[prec] = best
+    res["fxcorr_512ch_2^20_M1024"] = xres
+    res["value"] = res["sizes"]["2^16"]["bf16x3"]["gflops"]
+    return res
 
 
 CONFIGS = {1: run_config1, 3: run_config3, 4: run_config4, 5: run_config5}

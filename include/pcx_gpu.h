@@ -100,6 +100,13 @@ int pcx_fxcorr_destroy(pcx_fxcorr_t xc);
 int pcx_fxcorr_exec(pcx_fxcorr_t xc, float* dst, int64_t dst_pack, const float* src, int64_t src_pack,
                     int64_t len, void* stream);
 int64_t pcx_fxcorr_step(pcx_fxcorr_t xc, int64_t src_pack);
+/* Multi-channel form: `channels` signals of `len` samples, channel c at src + c*src_dist and
+ * dst + c*dst_dist floats, all correlated with the same kernel spectrum in one launch. */
+int pcx_fxcorr_exec_batch(pcx_fxcorr_t xc, float* dst, int64_t dst_pack, int64_t dst_dist, const float* src,
+                          int64_t src_pack, int64_t src_dist, int64_t len, int64_t channels, void* stream);
+/* PCX_PREC_FP32 (default) or PCX_PREC_BF16X3 (matrix-core radix-16 butterflies in both transforms;
+ * no reference counterpart). */
+int pcx_fxcorr_set_precision(pcx_fxcorr_t xc, int precision);
 
 /* ------------------------------------------------------------------ uneven batches
  * A work queue of transforms of uneven power-of-two sizes and mixed directions, natural order,

@@ -51,6 +51,8 @@ SIGNATURES = {
     "pcx_fxcorr_destroy": (_int, [_vp]),
     "pcx_fxcorr_exec": (_int, [_vp, _vp, _i64, _vp, _i64, _i64, _vp]),
     "pcx_fxcorr_step": (_i64, [_vp, _i64]),
+    "pcx_fxcorr_exec_batch": (_int, [_vp, _vp, _i64, _i64, _vp, _i64, _i64, _i64, _i64, _vp]),
+    "pcx_fxcorr_set_precision": (_int, [_vp, _int]),
     "pcx_queue_create": (_int, [C.POINTER(Job), _i64, _int, C.POINTER(_vp)]),
     "pcx_queue_destroy": (_int, [_vp]),
     "pcx_queue_exec": (_int, [_vp, _vp, _vp, _vp]),

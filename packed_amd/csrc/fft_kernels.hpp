@@ -394,8 +394,9 @@ constexpr int row_lds_floats() {
 }
 
 // Grid-stride over row groups: gridDim.x may be smaller than the number of groups.
-// The matrix-core variants keep the split operands in registers: they run at half the waves.
-constexpr int mfma_waves(int w, int prec) { return prec == PREC_FP32 ? w : (w > 2 ? w / 2 : 1); }
+// The matrix-core variants keep the split operands in registers: they run at half the waves,
+// but at least two, so the register budget (256) never pushes accumulators into AGPRs.
+constexpr int mfma_waves(int w, int prec) { return prec == PREC_FP32 ? w : (w > 4 ? w / 2 : 2); }
 
 template<int LA, int G, int LD, int ST, int TW, bool UNI, int ORD, int PREC = PREC_FP32>
 __global__ __launch_bounds__(G * Plan<LA>::TA,

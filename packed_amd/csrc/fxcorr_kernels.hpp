@@ -38,6 +38,8 @@ struct XcArgs {
     int64_t      nblocks;
     int64_t      hlen;  // N - step
     int          src_lp, dst_lp;
+    int64_t      src_dist, dst_dist; // floats between channels
+    int64_t      channels;           // blocks of channel c are nblocks * c .. nblocks * (c + 1) - 1
 };
 
 template<int LA, int G>
@@ -46,8 +48,13 @@ constexpr int xc_lds_floats() {
 }
 
 // B32: the signal fits 32-bit buffer offsets (one SGPR descriptor, one VGPR per address).
-template<int LA, int G, bool B32>
-__global__ __launch_bounds__(G * Plan<LA>::TA, waves_per_simd(G * Plan<LA>::TA, 4 * xc_lds_floats<LA, G>()))
+// PREC_BF16X3 runs the radix-16 butterflies of both transforms on the matrix cores.
+template<int LA, int G, bool B32, int PREC = PREC_FP32>
+__global__ __launch_bounds__(G * Plan<LA>::TA,
+                             PREC == PREC_FP32 ? waves_per_simd(G * Plan<LA>::TA, 4 * xc_lds_floats<LA, G>())
+                                               : (waves_per_simd(G * Plan<LA>::TA, 4 * xc_lds_floats<LA, G>()) > 4
+                                                      ? waves_per_simd(G * Plan<LA>::TA, 4 * xc_lds_floats<LA, G>()) / 2
+                                                      : 2))
 void k_fxcorr(XcArgs p) {
     using PF           = Plan<LA, false>;
     using PI           = Plan<LA, true>;
@@ -65,12 +72,19 @@ void k_fxcorr(XcArgs p) {
     TwTable<LA>::fill(ltw, p.twA, threadIdx.x, G * TA);
     __syncthreads();
 
+    MfmaDft16 mf;
+    if constexpr (PREC == PREC_BF16X3)
+        mf.init();
     const int     t    = threadIdx.x;
     const int     g    = t / TA;
     const int     jt   = t % TA;
-    const int64_t b    = (int64_t)blockIdx.x * G + g;
-    const bool    ok   = b < p.nblocks;
+    const int64_t bg   = (int64_t)blockIdx.x * G + g; // block over all channels
+    const bool    ok   = bg < p.nblocks * p.channels;
+    const int64_t ch   = ok ? bg / p.nblocks : 0;
+    const int64_t b    = ok ? bg - ch * p.nblocks : 0;
     const int64_t off  = b * p.step;
+    const float*  src  = p.src + ch * p.src_dist;
+    float*        dst  = p.dst + ch * p.dst_dist;
     auto          addr = [&](int b, int o) { return g * STR + b + (b >> 5) + o + (o >> 5); };
 
     cf v[PPT];
@@ -83,17 +97,17 @@ void k_fxcorr(XcArgs p) {
             cf      x = {0.f, 0.f};
             if (ok && i < p.len) {
                 if (p.hre && a >= p.step) {
-                    int64_t h = b * p.hlen + (a - p.step);
+                    int64_t h = bg * p.hlen + (a - p.step);
                     x         = {p.hre[h], p.him[h]};
                 } else if constexpr (B32) {
-                    x = BufR(p.src, p.src_lp).ld(poff((uint32_t)i, p.src_lp), 0);
+                    x = BufR(src, p.src_lp).ld(poff((uint32_t)i, p.src_lp), 0);
                 } else {
-                    x = ld_cx64(p.src, i, p.src_lp);
+                    x = ld_cx64(src, i, p.src_lp);
                 }
             }
             v[m * R0 + r] = x;
         }
-    stockham<LA, false>(v, lre, lim, jt, addr, ltw);
+    stockham<LA, false, decltype(addr), PREC>(v, lre, lim, jt, addr, ltw, &mf);
 #pragma unroll
     for (int m = 0; m < PPT / RL; ++m)
 #pragma unroll
@@ -102,7 +116,7 @@ void k_fxcorr(XcArgs p) {
             cf  k         = BufR(p.kre, LA).ld((uint32_t)e, 0); // split spectrum: re[N], im[N]
             v[m * RL + r] = cmul(cconj(v[m * RL + r]), k);
         }
-    stockham<LA, true>(v, lre, lim, jt, addr, ltw);
+    stockham<LA, true, decltype(addr), PREC>(v, lre, lim, jt, addr, ltw, &mf);
     if (ok) {
         int64_t keep = p.len - off < p.step ? p.len - off : p.step;
 #pragma unroll
@@ -112,9 +126,9 @@ void k_fxcorr(XcArgs p) {
                 int a = out_elem<LA, true>(jt, m, r);
                 if (a < keep) {
                     if constexpr (B32)
-                        BufW(p.dst, p.dst_lp).st(poff((uint32_t)(off + a), p.dst_lp), cconj(v[m * RLI + r]), 0);
+                        BufW(dst, p.dst_lp).st(poff((uint32_t)(off + a), p.dst_lp), cconj(v[m * RLI + r]), 0);
                     else
-                        st_cx64(p.dst, off + a, p.dst_lp, cconj(v[m * RLI + r]));
+                        st_cx64(dst, off + a, p.dst_lp, cconj(v[m * RLI + r]));
                 }
             }
     }
@@ -122,14 +136,15 @@ void k_fxcorr(XcArgs p) {
 
 // Saves, for every block b, the samples [b*step + step, b*step + N) that later blocks overwrite
 // when the correlation runs in place.
-__global__ void k_fxcorr_halo(const float* src, int src_lp, int64_t len, int64_t step, int64_t hlen,
-                              int64_t nblocks, float* hre, float* him) {
+__global__ void k_fxcorr_halo(const float* src, int src_lp, int64_t src_dist, int64_t len, int64_t step, int64_t hlen,
+                              int64_t nblocks, int64_t channels, float* hre, float* him) {
     int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= nblocks * hlen)
+    if (idx >= nblocks * channels * hlen)
         return;
-    int64_t b = idx / hlen, h = idx % hlen;
-    int64_t i = b * step + step + h;
-    cf      x = i < len ? ld_cx64(src, i, src_lp) : cf{0.f, 0.f};
+    int64_t bg = idx / hlen, h = idx % hlen;
+    int64_t ch = bg / nblocks, b = bg % nblocks;
+    int64_t i  = b * step + step + h;
+    cf      x  = i < len ? ld_cx64(src + ch * src_dist, i, src_lp) : cf{0.f, 0.f};
     hre[idx]  = x.x;
     him[idx]  = x.y;
 }

@@ -100,17 +100,17 @@ void launch_col(const ColArgs& a, int64_t grid_cap, hipStream_t st) {
         nwg = grid_cap;
     hipLaunchKernelGGL((k_col<LA, C, TW, PREC>), dim3((unsigned)nwg), dim3(C * Plan<LA>::TA), 0, st, a);
 }
-template<int LA>
+template<int LA, int PREC = PREC_FP32>
 void launch_xc(const XcArgs& a, hipStream_t st) {
     constexpr int G   = default_rows<LA>();
-    int64_t       nwg = (a.nblocks + G - 1) / G;
-    // 32-bit buffer offsets cover the signal when its packed footprint stays below 2^29 floats
-    bool small = pidx64(a.len + (int64_t)G * (1 << LA), a.src_lp > a.dst_lp ? a.src_lp : a.dst_lp) < ((int64_t)1 << 29) &&
-                 a.src_lp < 29 && a.dst_lp < 29;
+    int64_t       nwg = (a.nblocks * a.channels + G - 1) / G;
+    // 32-bit buffer offsets cover a channel when its packed footprint stays below 2^29 floats
+    int  lp    = a.src_lp > a.dst_lp ? a.src_lp : a.dst_lp;
+    bool small = pidx64(a.len + (int64_t)(1 << LA), lp) < ((int64_t)1 << 29) && a.src_lp < 29 && a.dst_lp < 29;
     if (small)
-        hipLaunchKernelGGL((k_fxcorr<LA, G, true>), dim3((unsigned)nwg), dim3(G * Plan<LA>::TA), 0, st, a);
+        hipLaunchKernelGGL((k_fxcorr<LA, G, true, PREC>), dim3((unsigned)nwg), dim3(G * Plan<LA>::TA), 0, st, a);
     else
-        hipLaunchKernelGGL((k_fxcorr<LA, G, false>), dim3((unsigned)nwg), dim3(G * Plan<LA>::TA), 0, st, a);
+        hipLaunchKernelGGL((k_fxcorr<LA, G, false, PREC>), dim3((unsigned)nwg), dim3(G * Plan<LA>::TA), 0, st, a);
 }
 
 // Single-pass variants: 0 natural in/out; 1, 2 permuted store (forward, bit_reversed /
@@ -143,6 +143,11 @@ constexpr auto single_mfma_table(std::integer_sequence<int, L...>) {
 }
 const auto kSingleMfma = single_mfma_table(std::make_integer_sequence<int, kMaxSinglePass - 3>{});
 const auto kXc     = xc_table(std::make_integer_sequence<int, kMaxSinglePass>{});
+template<int... L>
+constexpr auto xc_mfma_table(std::integer_sequence<int, L...>) {
+    return std::array<XcLaunch, sizeof...(L)>{&launch_xc<L + 4, PREC_BF16X3>...};
+}
+const auto kXcMfma = xc_mfma_table(std::make_integer_sequence<int, kMaxSinglePass - 3>{}); // LA 4 .. 14
 
 using PassSet = std::array<RowLaunch, 6>;
 // Pass-twiddle table capacity in LDS for N = 2^LG (2: too large, read from global memory).
@@ -570,6 +575,7 @@ extern "C" int pcx_fft_batch(pcx_plan_t p, int inverse, int normalized, float* d
 struct pcx_fxcorr_s {
     int64_t n = 0, m = 0;
     int     lg = 0;
+    int     precision = PCX_PREC_FP32;
     float2* tw = nullptr;
     float*  kspec = nullptr; // FFT(g)/N, split re[N] im[N]
     float*  halo  = nullptr;
@@ -646,8 +652,19 @@ extern "C" int64_t pcx_fxcorr_step(pcx_fxcorr_t x, int64_t src_pack) {
     return (x->n - overlap) / src_pack * src_pack;
 }
 
-extern "C" int pcx_fxcorr_exec(pcx_fxcorr_t x, float* dst, int64_t dst_pack, const float* src, int64_t src_pack,
-                               int64_t len, void* stream) {
+extern "C" int pcx_fxcorr_set_precision(pcx_fxcorr_t x, int precision) {
+    if (!x)
+        return fail(PCX_ERR_INVALID_ARGUMENT, "fxcorr handle is null");
+    if (precision != PCX_PREC_FP32 && precision != PCX_PREC_BF16X3)
+        return fail(PCX_ERR_UNSUPPORTED, "precision " + s(precision) + " is not implemented");
+    if (precision == PCX_PREC_BF16X3 && x->lg < 4)
+        return fail(PCX_ERR_UNSUPPORTED, "split-BF16 correlation needs fft_size >= 16");
+    x->precision = precision;
+    return PCX_SUCCESS;
+}
+
+extern "C" int pcx_fxcorr_exec_batch(pcx_fxcorr_t x, float* dst, int64_t dst_pack, int64_t dst_dist, const float* src,
+                                     int64_t src_pack, int64_t src_dist, int64_t len, int64_t channels, void* stream) {
     if (!x)
         return fail(PCX_ERR_INVALID_ARGUMENT, "fxcorr handle is null");
     if (!dst || !src)
@@ -656,26 +673,29 @@ extern "C" int pcx_fxcorr_exec(pcx_fxcorr_t x, float* dst, int64_t dst_pack, con
         return e;
     if (int e = check_pack(src_pack, "source"))
         return e;
-    if (len < 0)
-        return fail(PCX_ERR_SIZE_MISMATCH, "signal length (which is " + s(len) + ") is negative");
+    if (len < 0 || channels < 0)
+        return fail(PCX_ERR_SIZE_MISMATCH, "signal length (which is " + s(len) + ") or channel count is negative");
     int64_t step = pcx_fxcorr_step(x, src_pack);
     if (step <= 0)
         return fail(PCX_ERR_SIZE_MISMATCH, "fft size (which is " + s(x->n) + ") leaves no output per block for a kernel of " +
                                                s(x->m) + " taps at pack " + s(src_pack));
-    if ((const float*)dst == src && dst_pack != src_pack)
+    const bool inplace = (const float*)dst == src;
+    if (inplace && (dst_pack != src_pack || dst_dist != src_dist))
         return fail(PCX_ERR_INPLACE_REPACK, "cannot perform in-place repack from source pack size (which is " +
                                                 s(src_pack) + ") to destination pack size (which is " + s(dst_pack) + ")");
-    if (len == 0)
+    if (len == 0 || channels == 0)
         return PCX_SUCCESS;
-    hipStream_t st = (hipStream_t)stream;
+    hipStream_t st      = (hipStream_t)stream;
     int64_t     nblocks = (len + step - 1) / step;
     XcArgs      a{};
     a.src = src, a.dst = dst, a.kre = x->kspec, a.kim = x->kspec + x->n, a.twA = x->tw;
     a.len = len, a.step = step, a.nblocks = nblocks, a.hlen = x->n - step;
     a.src_lp = ilog2(src_pack), a.dst_lp = ilog2(dst_pack);
+    a.src_dist = src_dist, a.dst_dist = dst_dist, a.channels = channels;
     std::lock_guard<std::mutex> lock(x->mu);
-    if ((const float*)dst == src && nblocks > 1 && a.hlen > 0) {
-        int64_t need = nblocks * a.hlen;
+    if (inplace && nblocks > 1 && a.hlen > 0) {
+        // blocks overwrite the overlap that the next block still reads: save it first
+        int64_t need = nblocks * channels * a.hlen;
         if (x->halo_cap < need) {
             if (x->halo) {
                 HIPCHK(hipStreamSynchronize(st));
@@ -689,13 +709,21 @@ extern "C" int pcx_fxcorr_exec(pcx_fxcorr_t x, float* dst, int64_t dst_pack, con
         }
         a.hre = x->halo, a.him = x->halo + need;
         int64_t nb = (need + 255) / 256;
-        hipLaunchKernelGGL(k_fxcorr_halo, dim3((unsigned)nb), dim3(256), 0, st, src, a.src_lp, len, step, a.hlen,
-                           nblocks, x->halo, x->halo + need);
+        hipLaunchKernelGGL(k_fxcorr_halo, dim3((unsigned)nb), dim3(256), 0, st, src, a.src_lp, src_dist, len, step, a.hlen,
+                           nblocks, channels, x->halo, x->halo + need);
         HIPCHK(hipGetLastError());
     }
-    kXc[x->lg - 1](a, st);
+    if (x->precision == PCX_PREC_BF16X3)
+        kXcMfma[x->lg - 4](a, st);
+    else
+        kXc[x->lg - 1](a, st);
     HIPCHK(hipGetLastError());
     return PCX_SUCCESS;
+}
+
+extern "C" int pcx_fxcorr_exec(pcx_fxcorr_t x, float* dst, int64_t dst_pack, const float* src, int64_t src_pack,
+                               int64_t len, void* stream) {
+    return pcx_fxcorr_exec_batch(x, dst, dst_pack, 0, src, src_pack, 0, len, 1, stream);
 }
 
 // ---------------------------------------------------------------- memory and utilities

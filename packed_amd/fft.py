@@ -179,13 +179,15 @@ class fft_unit:
 class fxcorr_unit:
     """fxcorr.hpp:44-147 fxcorr_unit<float> on the GPU: out[i] = sum_m f[i+m] * conj(g[m])."""
 
-    def __init__(self, g: vector, fft_size: int, stream=None):
+    def __init__(self, g: vector, fft_size: int, stream=None, precision: str = "fp32"):
         import ctypes as C
 
         h = C.c_void_p()
         check(lib().pcx_fxcorr_create(g.data_ptr(), g.pack, g.size(), int(fft_size), _stream_ptr(stream),
                                       C.byref(h)))
         self._h = h
+        if precision != "fp32":
+            check(lib().pcx_fxcorr_set_precision(h, PRECISION[precision]))
         self.fft_size = int(fft_size)
         self.g_size = g.size()
 
@@ -210,6 +212,11 @@ class fxcorr_unit:
                                                    f"the signal (which is {f.size()})")
         check(lib().pcx_fxcorr_exec(self._h, dst.data_ptr(), dst.pack, f.data_ptr(), f.pack, f.size(),
                                     _stream_ptr(stream)))
+
+    def batch(self, dst: int, src: int, length: int, channels: int, pack: int, dist: int, stream=None) -> None:
+        """Multi-channel correlation on raw device pointers: channel c at c*dist floats."""
+        check(lib().pcx_fxcorr_exec_batch(self._h, dst, pack, dist, src, pack, dist, length, channels,
+                                          _stream_ptr(stream)))
 
 
 class fft_queue:

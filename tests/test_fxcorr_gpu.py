@@ -115,3 +115,44 @@ def test_peaks_index_exact(orc, pa):
     assert np.array_equal(top, offs)
     want = orc.xcorr_naive(f[: 1 << 16], g)[: (1 << 16) - M]
     assert orc.rel_l2(got[: (1 << 16) - M], want) < 4e-6
+
+
+@pytest.mark.parametrize("inplace", [True, False])
+def test_multi_channel(orc, pa, inplace):
+    """pcx_fxcorr_exec_batch: several channels correlated with one kernel spectrum in one launch."""
+    import torch
+
+    L, M, N, C = 20000, 300, 1024, 5
+    pack = 8
+    dist = orc.real_size(L, pack) + 64
+    g = np.exp(1j * np.pi * np.real(orc.gen_uniform(0x5EED0013, M)))
+    fs = [orc.gen_normal(0x5EED0030 + c, L, np.sqrt(0.5)).astype(np.complex128) for c in range(C)]
+    buf = np.zeros(dist * C, np.float32)
+    for c in range(C):
+        pb = orc.to_packed(fs[c].astype(np.complex64), pack)
+        buf[c * dist:c * dist + len(pb)] = pb
+    src = torch.from_numpy(buf).cuda()
+    dst = src if inplace else torch.zeros_like(src)
+    xc = pa.fxcorr_unit(pa.vector.from_complex(g.astype(np.complex64), pack), N)
+    xc.batch(dst.data_ptr(), src.data_ptr(), L, C, pack, dist)
+    torch.cuda.synchronize()
+    out = dst.cpu().numpy()
+    for c in range(C):
+        got = orc.from_packed(out[c * dist:], L, pack)
+        assert orc.rel_l2(got, orc.xcorr_naive(fs[c], g)) < 4e-6, c
+
+
+def test_bf16x3_correlation(orc, pa):
+    """Split-BF16 matrix-core form of the fused correlation."""
+    import torch
+
+    L, M, N = 70000, 4096, 16384
+    f = orc.gen_normal(0x5EED0003, L, np.sqrt(0.5)).astype(np.complex128)
+    g = np.exp(1j * np.pi * np.real(orc.gen_uniform(0x5EED0013, M)))
+    gv = pa.vector.from_complex(g.astype(np.complex64), 8)
+    xc = pa.fxcorr_unit(gv, N, precision="bf16x3")
+    fv = pa.vector.from_complex(f.astype(np.complex64), 8)
+    out = pa.vector(L, 8)
+    xc(fv, out)
+    torch.cuda.synchronize()
+    assert orc.rel_l2(out.to_complex(), orc.xcorr_naive(f, g)) < 4e-6

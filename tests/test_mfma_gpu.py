@@ -16,7 +16,7 @@ def pa():
     return packed_amd
 
 
-@pytest.mark.parametrize("lg", [4, 5, 8, 10, 12, 13, 14, 15, 16, 17, 18, 20])
+@pytest.mark.parametrize("lg", list(range(4, 21)))
 def test_bf16x3_forward_inverse(orc, pa, lg):
     import torch
 
