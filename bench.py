@@ -38,10 +38,24 @@ def parse():
     ap.add_argument("--quick", action="store_true", help="time the forward step only (for profiler runs)")
     ap.add_argument("--log2n", type=int, default=N_LOG2)
     ap.add_argument("--batch", type=int, default=BATCH)
+    ap.add_argument("--copy", action="store_true",
+                    help="time a plain device copy of the same bytes instead (calibrates the HBM counters)")
     ap.add_argument("--config", type=int, default=2, choices=[1, 2, 3, 4, 5],
                     help="2: the headline batched FFT (default); 1: N=1024 latency; 3: fxcorr; "
                          "4: uneven batch; 5: split-precision MFMA")
     return ap.parse_args()
+
+
+def measured_traffic(log2n: int, batch: int):
+    """HBM bytes per step from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this
+    workload (counters scaled by a plain copy of the same bytes); None for other shapes."""
+    path = os.path.join(ROOT, "profiles", f"fft_fourstep_n{1 << log2n}_b{batch}_hbm_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_step"):
+            return int(line.split("=")[1])
+    return None
 
 
 def time_steps(fn, steps, warmup):
@@ -109,6 +123,9 @@ def main():
     if args.config != 2:
         print(json.dumps(CONFIGS[args.config](args)))
         return
+    if args.copy:
+        print(json.dumps(run_copy(args)))
+        return
     import numpy as np
     import torch
 
@@ -166,6 +183,7 @@ def main():
     want_br = np.fft.fft(xin[:n].astype(np.float64) + 1j * xin[n:])[O.slot_freq("bit_reversed", n)]
     err_br = O.rel_l2(zin[:n] + 1j * zin[n:], want_br)
 
+    traffic = measured_traffic(args.log2n, batch)
     out = {
         "metric": "C2C FFT GFLOP/s (5N*log2N), N=2^16 batch 4096, FP32 split re/im",
         "value": round(gflops, 1),
@@ -188,8 +206,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
-            "kernel": "four-step column pass + row pass (per step)",
+            "traffic": traffic,
+            "kernel": "four-step column pass k_col + row pass k_row (one step = one launch of each)",
         },
         "inverse": {"ms": round(ms_inv, 4), "gflops": round(flops / (ms_inv * 1e-3) / 1e9, 1)},
         "bit_reversed": {"ms": round(ms_br, 4), "gflops": round(flops / (ms_br * 1e-3) / 1e9, 1),
@@ -202,6 +220,20 @@ def main():
 
 
 # ---------------------------------------------------------------------------- other workloads
+
+
+def run_copy(args):
+    """Plain device-to-device copy of the headline's bytes (2^28 complex points: read 2 GiB, write
+    2 GiB).  Its counter readings give the scale factors from FETCH_SIZE / WRITE_SIZE to bytes."""
+    import torch
+
+    pts = (1 << args.log2n) * args.batch
+    x = torch.ones(2 * pts, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    ms = time_steps(lambda: y.copy_(x), args.steps, args.warmup)
+    return {"workload": f"device copy of {2 * pts * 4} bytes", "ms_per_copy": round(ms, 4),
+            "gb_per_s": round(2 * 2 * pts * 4 / (ms * 1e-3) / 1e9, 1), "bytes_read": 2 * pts * 4,
+            "bytes_written": 2 * pts * 4}
 def run_config1(args):
     """N=1024 single FP32 FFT, pack 8, forward then inverse: latency, order exactness, round trip."""
     import numpy as np
