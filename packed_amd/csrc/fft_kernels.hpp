@@ -97,12 +97,16 @@ struct CxPtrW {
 };
 
 // Buffer-resource views (wave-uniform base in SGPRs, 32-bit byte offsets): one VGPR per address.
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(const float* base) {
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(base), 0, 0x7FFFFFFF, 0x00020000);
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(const float* base, uint32_t bytes = 0x7FFFFFFFu) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(base), 0, (int)bytes, 0x00020000);
 }
 struct BufR {
     __amdgpu_buffer_rsrc_t re, im;
     __device__ BufR(const float* base, int lp) : re(rsrc(base)), im(rsrc(base + ((int64_t)1 << lp))) {}
+    // Loads at float offsets >= nfl return 0 (the buffer range check).  With nfl = poff(lim),
+    // exactly the elements i >= lim read as zero, in both views (poff is increasing).
+    __device__ BufR(const float* base, int lp, uint32_t nfl)
+        : re(rsrc(base, nfl * 4u)), im(rsrc(base + ((int64_t)1 << lp), nfl * 4u)) {}
     __device__ cf ld(uint32_t o, int) const {
         return {__uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(re, o * 4u, 0, 0)),
                 __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(im, o * 4u, 0, 0))};
@@ -419,6 +423,140 @@ void k_row(RowArgs p) {
     for (int64_t wg = blockIdx.x; wg < nwg; wg += gridDim.x) {
         row_group<LA, G, LD, ST, TW, UNI, ORD, PREC>(p, wg, lre, lim, ltw, lptw, &mf);
         __syncthreads(); // LDS is reused by the next group
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// k_dif: one-pass transform of N = R * M points (R = 2^LR, M = 2^LM <= 16384) with no HBM
+// intermediate.  With n = n2 + M*n1 and k = R*k2 + k1 (one decimation-in-frequency step of
+// radix R):
+//   X[R*k2 + k1] = FFT_M(y_k1)[k2],   y_k1[n2] = W_N^(n2*k1) * sum_{n1<R} x[n2 + M*n1] W_R^(n1*k1).
+// Workgroup (b, k1) reads all of transform b (R segments of M points), forms y_k1 in registers
+// and runs the M-point Stockham transform in LDS.  The R workgroups of one transform read the
+// same bytes: they are numbered w = 8*(R*t + k1) + x for transform b = 8*t + x, so the
+// round-robin placement of workgroups on the 8 XCDs (w mod 8) puts them on one XCD, dispatched
+// back to back, and the repeated reads are served by that XCD's L2.  (Placement only affects
+// speed, never results.)  Stores: natural order puts k2 of workgroup k1 at R*k2 + k1 (the R
+// workgroups fill interleaved floats of each cache line; a line evicted before all R have written
+// costs a second HBM write); bit_reversed / unordered give workgroup k1 the contiguous block
+// brev_R(k1)*M + [0, M), permuted inside.  Out of place only: every workgroup reads the whole
+// transform.
+template<int LM, int LR>
+constexpr int dif_lds_floats() {
+    return RowShape<LM, 1>::DATA + TwTable<LM>::FLOATS + 2 * pass_tw_entries(LM + LR);
+}
+
+template<int LM, int LR, int ORD, int PREC = PREC_FP32>
+__global__ __launch_bounds__(Plan<LM>::TA,
+                             mfma_waves(waves_per_simd(Plan<LM>::TA, 4 * dif_lds_floats<LM, LR>()), PREC))
+void k_dif(RowArgs p) {
+    using P           = Plan<LM>;
+    constexpr int TA  = P::TA;
+    constexpr int PPT = P::PPT;
+    constexpr int R0  = P::first_radix;
+    constexpr int RL  = P::last_radix;
+    constexpr int M   = 1 << LM;
+    constexpr int R   = 1 << LR;
+    constexpr int TWN = pass_tw_entries(LM + LR);
+    static_assert(LM >= 5 && TA >= 32, "k_dif needs the block-permuted store layout (TA >= 32)");
+    __shared__ float lds[dif_lds_floats<LM, LR>()];
+    float*           lre  = lds;
+    float*           lim  = lds + RowShape<LM, 1>::DATA / 2;
+    float*           ltw  = lds + RowShape<LM, 1>::DATA;
+    float*           lptw = ltw + TwTable<LM>::FLOATS;
+
+    const uint32_t w   = blockIdx.x;
+    const int      k1  = (int)((w >> 3) & (R - 1));
+    const int64_t  b   = (int64_t)(w >> (3 + LR)) * 8 + (w & 7);
+    if (b >= p.nrows)
+        return; // uniform over the workgroup
+    const int jt = threadIdx.x;
+    TwTable<LM>::fill(ltw, p.twA, jt, TA);
+    PassTw<TWN>::fill(lptw, p.twl, p.twh, p.tw_ls, p.tw_lg, jt, TA);
+    MfmaDft16 mf;
+    if constexpr (PREC == PREC_BF16X3)
+        mf.init();
+    auto addr = [&](int bb, int o) { return bb + (bb >> 5) + o + (o >> 5); };
+
+    // ------------------------------------------------------------------ load + radix-R step
+    cf v[PPT];
+#pragma unroll
+    for (int q = 0; q < PPT; ++q)
+        v[q] = cf{0.f, 0.f};
+    {
+        const float*   tb  = tbase(p.src, p.src_offs, p.src_dist, b);
+        const float    cs  = p.conj_in ? -1.f : 1.f;
+        const uint32_t lim = p.src_size < (int64_t)R * M ? (uint32_t)p.src_size : (uint32_t)(R * M);
+#pragma unroll
+        for (int n1 = 0; n1 < R; ++n1) {
+            // W_R^(n1*k1) = W_M^(n1*k1*M/R), a uniform read of the W_M table
+            const float2   w2 = p.twA[((n1 * k1) & (R - 1)) << (LM - LR)];
+            const cf       wr = cf{w2.x, w2.y};
+            const uint32_t s0 = (uint32_t)n1 << LM;
+            // segment n1 starts at poff(n1*M) (poff(n2 + n1*M) = poff(n2) + poff(n1*M) for every
+            // power-of-two pack); elements past src_size read as zero through the range check
+            const uint32_t nseg = lim > s0 ? (lim - s0 < (uint32_t)M ? lim - s0 : (uint32_t)M) : 0u;
+            const BufR     seg(tb + poff(s0, p.src_lp), p.src_lp, poff(nseg, p.src_lp));
+#pragma unroll
+            for (int m = 0; m < PPT / R0; ++m)
+#pragma unroll
+                for (int r = 0; r < R0; ++r) {
+                    cf x = seg.ld(poff((uint32_t)in_elem<LM>(jt, m, r), p.src_lp), 0);
+                    x.y *= cs;
+                    if (n1 == 0)
+                        v[m * R0 + r] = x;
+                    else
+                        v[m * R0 + r] = cadd(v[m * R0 + r], cmul(x, wr));
+                }
+        }
+    }
+    __syncthreads(); // twiddle tables are in LDS
+    if (k1 != 0) {
+#pragma unroll
+        for (int m = 0; m < PPT / R0; ++m)
+#pragma unroll
+            for (int r = 0; r < R0; ++r) {
+                uint32_t e    = (uint32_t)in_elem<LM>(jt, m, r) * (uint32_t)k1;
+                v[m * R0 + r] = cmul(v[m * R0 + r], PassTw<TWN>::get(lptw, p.twl, p.twh, p.tw_ls, p.tw_lg, e));
+            }
+    }
+    // ------------------------------------------------------------------ M-point transform
+    stockham<LM, false, decltype(addr), PREC>(v, lre, lim, jt, addr, ltw, &mf);
+    const float sc = p.scale;
+    const float so = p.conj_out ? -sc : sc;
+#pragma unroll
+    for (int q = 0; q < PPT; ++q) {
+        v[q].x *= sc;
+        v[q].y *= so;
+    }
+    // ------------------------------------------------------------------ store
+    const BufW dst(tbase(p.dst, p.dst_offs, p.dst_dist, b), p.dst_lp);
+    if constexpr (ORD == ORD_NORMAL) {
+#pragma unroll
+        for (int m = 0; m < PPT / RL; ++m)
+#pragma unroll
+            for (int r = 0; r < RL; ++r) {
+                uint32_t i = ((uint32_t)out_elem<LM>(jt, m, r) << LR) + (uint32_t)k1;
+                dst.st(poff(i, p.dst_lp), v[m * RL + r], p.nt_store);
+            }
+    } else {
+        __syncthreads(); // the last stage's LDS reads are done
+#pragma unroll
+        for (int m = 0; m < PPT / RL; ++m)
+#pragma unroll
+            for (int r = 0; r < RL; ++r) {
+                int o  = addr(jt + m * TA, r * (M / RL));
+                lre[o] = v[m * RL + r].x;
+                lim[o] = v[m * RL + r].y;
+            }
+        __syncthreads();
+        const uint32_t base = brev_bits((uint32_t)k1, LR) << LM;
+#pragma unroll
+        for (int u = 0; u < PPT; ++u) {
+            int s = jt + u * TA;
+            int o = addr(perm_base<LM, ORD>(jt), brevc(u, ilog2c(PPT)));
+            dst.st(poff(base + s, p.dst_lp), cf{lre[o], lim[o]}, p.nt_store);
+        }
     }
 }
 

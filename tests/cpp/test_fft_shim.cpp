@@ -112,9 +112,12 @@ static void test_fftu_float(size_t size) {
         unit(out, vec);
         auto got = out.to_host();
         CHECK(rel_l2(got, want) < tol, "bit_reversed N=%zu err=%g", size, rel_l2(got, want));
-        unit.fft_raw<PackSize>(vec.data()); // fft_raw vs operator(): identical
-        auto raw = vec.to_host();
-        CHECK(raw == got, "bit_reversed fft_raw != operator() N=%zu", size);
+        // in-place fft_raw vs out-of-place operator() (test_fft.cpp:491-505 compares them with
+        // equal_eps); the two forms may run different kernels, so compare within the tolerance
+        unit.fft_raw<PackSize>(vec.data());
+        auto             raw = vec.to_host();
+        std::vector<cxd> gotd(got.begin(), got.end());
+        CHECK(rel_l2(raw, gotd) < tol, "bit_reversed fft_raw vs operator() N=%zu err=%g", size, rel_l2(raw, gotd));
         unit.template ifft<true>(vec);
         std::vector<cxd> xd(x.begin(), x.end());
         CHECK(rel_l2(vec.to_host(), xd) < 2 * tol, "bit_reversed round trip N=%zu", size);

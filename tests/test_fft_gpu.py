@@ -165,3 +165,38 @@ def test_reference_leaky_tone(orc, pa):
         pa.fft_unit(n)(v)
         torch.cuda.synchronize()
         assert orc.rel_l2(v.to_complex(), want) < fft_tol(n) + 1e-6
+
+
+@pytest.mark.parametrize("lg", [15, 16])
+@pytest.mark.parametrize("order", ORDERS)
+def test_one_pass_dif(orc, pa, lg, order):
+    """N = 2^15 runs the one-pass decimation-in-frequency kernel (2^16 the four-step): batches
+    that leave part of the last group of eight transforms empty, zero-extension and repacking,
+    forward in every order, inverse in natural order."""
+    n = 1 << lg
+    for batch in (1, 9):
+        assert _fwd_case(orc, pa, n, order, 8, 8, batch=batch) < fft_tol(n)
+    assert _fwd_case(orc, pa, n, order, 1, n, batch=2, src_size=n - 777) < fft_tol(n)
+    if order == "normal":
+        assert _inv_case(orc, pa, n, order, n, n, batch=3) < fft_tol(n)
+        assert _inv_case(orc, pa, n, order, 8, 2, normalized=False) < fft_tol(n)
+
+
+@pytest.mark.parametrize("order", ORDERS)
+def test_one_pass_dif_agrees_with_four_step(orc, pa, monkeypatch, order):
+    """The same batch through the one-pass kernel and through the two-pass four-step split."""
+    import torch
+
+    n, batch = 1 << 15, 4
+    xs = [orc.gen_normal(0x5EED0040 + b, n) for b in range(batch)]
+    src = to_device(batch_buffer(xs, n, 2 * n))
+    outs = []
+    for flag in ("1", "0"):
+        monkeypatch.setenv("PCX_DIF", flag)
+        unit = pa.fft_unit(n, order=order)
+        d = torch.zeros_like(src)
+        unit.batch(d.data_ptr(), src.data_ptr(), batch, n, 2 * n, n, 2 * n)
+        torch.cuda.synchronize()
+        outs.append(d.cpu().numpy().astype(np.float64))
+    diff = np.linalg.norm(outs[0] - outs[1]) / np.linalg.norm(outs[1])
+    assert diff < fft_tol(n), diff
