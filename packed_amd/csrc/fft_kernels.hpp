@@ -71,6 +71,9 @@ __device__ __forceinline__ uint32_t poff(uint32_t i, int lp) {
     int l = lp < 30 ? lp : 30;
     return i + ((i >> l) << l);
 }
+// poff(jt + c) = poff(jt) + poff(c) when c is a multiple of the thread count TA of a transform
+// and jt < TA (both powers of two, any power-of-two pack): the contiguous loads and stores of a
+// thread share one per-lane offset poff(jt) and add the uniform poff(c) as a scalar offset.
 __device__ __forceinline__ float ldf(const float* p, int nt) { return nt ? __builtin_nontemporal_load(p) : *p; }
 __device__ __forceinline__ void  stf(float* p, float v, int nt) {
     if (nt)
@@ -85,6 +88,7 @@ struct CxPtr {
     const float* im;
     __device__ CxPtr(const float* base, int lp) : re(base), im(base + ((int64_t)1 << lp)) {}
     __device__ cf ld(uint32_t o, int nt) const { return {ldf(re + o, nt), ldf(im + o, nt)}; }
+    __device__ cf ld(uint32_t vo, uint32_t so, int nt) const { return ld(vo + so, nt); }
 };
 struct CxPtrW {
     float* re;
@@ -94,6 +98,7 @@ struct CxPtrW {
         stf(re + o, v.x, nt);
         stf(im + o, v.y, nt);
     }
+    __device__ void st(uint32_t vo, uint32_t so, cf v, int nt) const { st(vo + so, v, nt); }
 };
 
 // Buffer-resource views (wave-uniform base in SGPRs, 32-bit byte offsets): one VGPR per address.
@@ -111,13 +116,26 @@ struct BufR {
         return {__uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(re, o * 4u, 0, 0)),
                 __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(im, o * 4u, 0, 0))};
     }
+    // element at vo + so, so wave-uniform (scalar offset); the range is not checked against so
+    __device__ cf ld(uint32_t vo, uint32_t so, int) const {
+        return {__uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(re, vo * 4u, so * 4u, 0)),
+                __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(im, vo * 4u, so * 4u, 0))};
+    }
 };
 struct BufW {
     __amdgpu_buffer_rsrc_t re, im;
     __device__ BufW(float* base, int lp) : re(rsrc(base)), im(rsrc(base + ((int64_t)1 << lp))) {}
+    // Stores at float offsets >= nfl are dropped by the range check (nfl = poff(lim): exactly the
+    // elements i >= lim)
+    __device__ BufW(float* base, int lp, uint32_t nfl)
+        : re(rsrc(base, nfl * 4u)), im(rsrc(base + ((int64_t)1 << lp), nfl * 4u)) {}
     __device__ void st(uint32_t o, cf v, int) const {
         __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v.x), re, o * 4u, 0, 0);
         __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v.y), im, o * 4u, 0, 0);
+    }
+    __device__ void st(uint32_t vo, uint32_t so, cf v, int) const {
+        __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v.x), re, vo * 4u, so * 4u, 0);
+        __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v.y), im, vo * 4u, so * 4u, 0);
     }
 };
 template<bool UNI>
@@ -173,7 +191,9 @@ struct RowShape {
     static constexpr int NT = G * TA;
     // odd row stride: rows of a strided access land in distinct banks
     static constexpr int STRIDE = (1 << LA) + ((1 << LA) >> 5) + 1;
-    static constexpr int DATA   = 2 * G * STRIDE;
+    // re and im arrays, or one array shared by both halves of each exchange (Plan::HALF)
+    static constexpr int DATA   = (Plan<LA>::HALF ? 1 : 2) * G * STRIDE;
+    __device__ static float* im(float* lre) { return Plan<LA>::HALF ? lre : lre + DATA / 2; }
 };
 
 // Waves per SIMD that the LDS footprint allows (argument of __launch_bounds__).
@@ -241,13 +261,25 @@ __device__ __forceinline__ void row_group(const RowArgs& p, int64_t wg, float* l
     if constexpr (LD == LD_CONTIG) {
         const RdView<UNI> src(tbase(p.src, p.src_offs, p.src_dist, btu), p.src_lp);
         if (row_ok && (int64_t)(k2 + 1) << LA <= p.src_size) { // whole row inside the source
+            if constexpr (P::HALF) {
+                // one per-lane offset plus scalar offsets: the 2^14-point plans need the registers
+                const uint32_t vo = poff((uint32_t)jt, p.src_lp);
 #pragma unroll
-            for (int m = 0; m < PPT / R0; ++m)
+                for (int m = 0; m < PPT / R0; ++m)
 #pragma unroll
-                for (int r = 0; r < R0; ++r) {
-                    uint32_t i    = (uint32_t)in_elem<LA>(jt, m, r) + (k2 << LA);
-                    v[m * R0 + r] = src.ld(poff(i, p.src_lp), p.nt_load);
-                }
+                    for (int r = 0; r < R0; ++r) {
+                        uint32_t c    = (uint32_t)(in_elem<LA>(jt, m, r) - jt) + (k2 << LA);
+                        v[m * R0 + r] = src.ld(vo, poff(c, p.src_lp), p.nt_load);
+                    }
+            } else {
+#pragma unroll
+                for (int m = 0; m < PPT / R0; ++m)
+#pragma unroll
+                    for (int r = 0; r < R0; ++r) {
+                        uint32_t i    = (uint32_t)in_elem<LA>(jt, m, r) + (k2 << LA);
+                        v[m * R0 + r] = src.ld(poff(i, p.src_lp), p.nt_load);
+                    }
+            }
         } else {
 #pragma unroll
             for (int m = 0; m < PPT / R0; ++m)
@@ -259,6 +291,53 @@ __device__ __forceinline__ void row_group(const RowArgs& p, int64_t wg, float* l
                     v[m * R0 + r] = ok ? x : cf{0.f, 0.f};
                 }
         }
+    } else if constexpr (P::HALF) {
+        // one LDS array: gather into registers in slot order, then real parts through LDS into
+        // the first stage's layout, then imaginary parts
+        auto pos = [&](int u) {
+            if constexpr (LD == LD_PERM) {
+                if constexpr (TA >= 32)
+                    return addr(perm_base<LA, ORD>(jt), brevc(u, ilog2c(PPT)));
+                else
+                    return addr(perm_elem<LA, ORD>(jt, u), 0);
+            } else {
+                int idx = t + u * NT;
+                int k1  = idx / G;
+                return (idx % G) * STR + k1 + (k1 >> 5);
+            }
+        };
+        if constexpr (LD == LD_PERM) {
+            const RdView<UNI> src(tbase(p.src, p.src_offs, p.src_dist, btu), p.src_lp);
+            uint32_t          base = brev_bits(k2, p.rows_lg) << LA;
+#pragma unroll
+            for (int u = 0; u < PPT; ++u) {
+                cf x = src.ld(row_ok ? poff(base + jt + u * TA, p.src_lp) : 0u, p.nt_load);
+                v[u] = row_ok ? x : cf{0.f, 0.f};
+            }
+        } else {
+            const RdView<UNI> src(tbase(p.src, p.src_offs, p.src_dist, row0 >> p.rows_lg), p.src_lp);
+#pragma unroll
+            for (int u = 0; u < PPT; ++u) {
+                int     idx = t + u * NT;
+                int64_t rw  = row0 + idx % G;
+                bool    ok  = rw < p.nrows;
+                cf      x   = src.ld(ok ? poff((uint32_t)(rw & rmask) + ((uint32_t)(idx / G) << p.rows_lg), p.src_lp)
+                                        : 0u,
+                                     p.nt_load);
+                v[u]        = ok ? x : cf{0.f, 0.f};
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < PPT; ++u)
+            lre[pos(u)] = v[u].x;
+        __syncthreads();
+        stage_read<LA, false, 0, 0>(v, lre, lim, jt, addr);
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < PPT; ++u)
+            lre[pos(u)] = v[u].y;
+        __syncthreads();
+        stage_read<LA, false, 0, 1>(v, lre, lim, jt, addr);
     } else {
         if constexpr (LD == LD_PERM) {
             // this row's spectrum occupies slots [A*brev(k2), A*brev(k2) + A); local slot s holds
@@ -340,13 +419,76 @@ __device__ __forceinline__ void row_group(const RowArgs& p, int64_t wg, float* l
     if constexpr (ST == ST_CONTIG) {
         if (row_ok) {
             const WrView<UNI> dst(tbase(p.dst, p.dst_offs, p.dst_dist, btu), p.dst_lp);
+            if constexpr (P::HALF) {
+                const uint32_t vo = poff((uint32_t)jt, p.dst_lp);
+#pragma unroll
+                for (int m = 0; m < PPT / RL; ++m)
+#pragma unroll
+                    for (int r = 0; r < RL; ++r) {
+                        uint32_t c = (uint32_t)(out_elem<LA>(jt, m, r) - jt) + (k2 << LA);
+                        dst.st(vo, poff(c, p.dst_lp), v[m * RL + r], p.nt_store);
+                    }
+            } else {
+#pragma unroll
+                for (int m = 0; m < PPT / RL; ++m)
+#pragma unroll
+                    for (int r = 0; r < RL; ++r) {
+                        uint32_t i = (uint32_t)out_elem<LA>(jt, m, r) + (k2 << LA);
+                        dst.st(poff(i, p.dst_lp), v[m * RL + r], p.nt_store);
+                    }
+            }
+        }
+    } else if constexpr (P::HALF) {
+        // one LDS array: real parts into slot order, then imaginary parts, then the stores
+        auto pos = [&](int u) {
+            if constexpr (ST == ST_PERM) {
+                if constexpr (TA >= 32)
+                    return addr(perm_base<LA, ORD>(jt), brevc(u, ilog2c(PPT)));
+                else
+                    return addr(perm_elem<LA, ORD>(jt, u), 0);
+            } else {
+                int idx = t + u * NT;
+                int k1  = idx / G;
+                return (idx % G) * STR + k1 + (k1 >> 5);
+            }
+        };
+        auto put = [&](auto comp) {
 #pragma unroll
             for (int m = 0; m < PPT / RL; ++m)
 #pragma unroll
-                for (int r = 0; r < RL; ++r) {
-                    uint32_t i = (uint32_t)out_elem<LA>(jt, m, r) + (k2 << LA);
-                    dst.st(poff(i, p.dst_lp), v[m * RL + r], p.nt_store);
-                }
+                for (int r = 0; r < RL; ++r)
+                    lre[addr(jt + m * TA, r * (P::A / RL))] = decltype(comp)::value ? v[m * RL + r].y : v[m * RL + r].x;
+        };
+        __syncthreads(); // the last stage's LDS reads are done
+        put(std::false_type{});
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < PPT; ++u)
+            v[u].x = lre[pos(u)];
+        __syncthreads();
+        put(std::true_type{}); // imaginary parts are still in the stage layout
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < PPT; ++u)
+            v[u].y = lre[pos(u)];
+        if constexpr (ST == ST_PERM) {
+            if (row_ok) {
+                const WrView<UNI> dst(tbase(p.dst, p.dst_offs, p.dst_dist, btu), p.dst_lp);
+                uint32_t          base = brev_bits(k2, p.rows_lg) << LA;
+#pragma unroll
+                for (int u = 0; u < PPT; ++u)
+                    dst.st(poff(base + jt + u * TA, p.dst_lp), v[u], p.nt_store);
+            }
+        } else {
+            const WrView<UNI> dst(tbase(p.dst, p.dst_offs, p.dst_dist, row0 >> p.rows_lg), p.dst_lp);
+#pragma unroll
+            for (int u = 0; u < PPT; ++u) {
+                int     idx = t + u * NT;
+                int64_t rw  = row0 + idx % G;
+                if (rw < p.nrows)
+                    dst.st(poff((uint32_t)(rw & rmask) + ((uint32_t)(idx / G) << p.rows_lg), p.dst_lp), v[u],
+                           p.nt_store);
+            }
         }
     } else {
         __syncthreads(); // the last stage's LDS reads are done
@@ -409,7 +551,7 @@ void k_row(RowArgs p) {
     constexpr int    NT = G * Plan<LA>::TA;
     __shared__ float lds[row_lds_floats<LA, G, TW>()];
     float*           lre  = lds;
-    float*           lim  = lds + RowShape<LA, G>::DATA / 2;
+    float*           lim  = RowShape<LA, G>::im(lds);
     float*           ltw  = lds + RowShape<LA, G>::DATA;
     float*           lptw = ltw + TwTable<LA>::FLOATS;
     TwTable<LA>::fill(ltw, p.twA, threadIdx.x, NT);
@@ -459,9 +601,10 @@ void k_dif(RowArgs p) {
     constexpr int R   = 1 << LR;
     constexpr int TWN = pass_tw_entries(LM + LR);
     static_assert(LM >= 5 && TA >= 32, "k_dif needs the block-permuted store layout (TA >= 32)");
+    static_assert(!P::HALF || ORD == ORD_NORMAL, "permuted k_dif stores stage through separate re / im arrays");
     __shared__ float lds[dif_lds_floats<LM, LR>()];
     float*           lre  = lds;
-    float*           lim  = lds + RowShape<LM, 1>::DATA / 2;
+    float*           lim  = RowShape<LM, 1>::im(lds);
     float*           ltw  = lds + RowShape<LM, 1>::DATA;
     float*           lptw = ltw + TwTable<LM>::FLOATS;
 
@@ -497,11 +640,18 @@ void k_dif(RowArgs p) {
             // power-of-two pack); elements past src_size read as zero through the range check
             const uint32_t nseg = lim > s0 ? (lim - s0 < (uint32_t)M ? lim - s0 : (uint32_t)M) : 0u;
             const BufR     seg(tb + poff(s0, p.src_lp), p.src_lp, poff(nseg, p.src_lp));
+            // the per-lane offset of segment n1 > 0 consumes the sums so far (an opaque zero is
+            // added), so one segment's loads are in registers at a time beside v
+            int jv = jt;
+            if (n1 > 0)
+                asm volatile("" : "+v"(jv) : "v"(v[0].x), "v"(v[PPT - 1].y));
+            const uint32_t vo = poff((uint32_t)jv, p.src_lp);
 #pragma unroll
             for (int m = 0; m < PPT / R0; ++m)
 #pragma unroll
                 for (int r = 0; r < R0; ++r) {
-                    cf x = seg.ld(poff((uint32_t)in_elem<LM>(jt, m, r), p.src_lp), 0);
+                    uint32_t c = (uint32_t)(in_elem<LM>(jt, m, r) - jt);
+                    cf       x = seg.ld(vo + poff(c, p.src_lp), 0);
                     x.y *= cs;
                     if (n1 == 0)
                         v[m * R0 + r] = x;
@@ -532,13 +682,19 @@ void k_dif(RowArgs p) {
     // ------------------------------------------------------------------ store
     const BufW dst(tbase(p.dst, p.dst_offs, p.dst_dist, b), p.dst_lp);
     if constexpr (ORD == ORD_NORMAL) {
+        // element (jt + c) * R + k1 with c a multiple of TA: poff of it is
+        // poff(jt * R + k1) + poff(c * R) (the same splitting at the multiple R * TA)
+        const int      jt = opaque_after_memory(threadIdx.x);
+        const uint32_t vo = poff(((uint32_t)jt << LR) + (uint32_t)k1, p.dst_lp);
 #pragma unroll
-        for (int m = 0; m < PPT / RL; ++m)
+        for (int m = 0; m < PPT / RL; ++m) {
+            const int lp = opaque_s(p.dst_lp);
 #pragma unroll
             for (int r = 0; r < RL; ++r) {
-                uint32_t i = ((uint32_t)out_elem<LM>(jt, m, r) << LR) + (uint32_t)k1;
-                dst.st(poff(i, p.dst_lp), v[m * RL + r], p.nt_store);
+                uint32_t c = (uint32_t)(out_elem<LM>(jt, m, r) - jt);
+                dst.st(vo, poff(c << LR, lp), v[m * RL + r], p.nt_store);
             }
+        }
     } else {
         __syncthreads(); // the last stage's LDS reads are done
 #pragma unroll
@@ -644,6 +800,7 @@ template<int LA, int C, int TW, int PREC = PREC_FP32>
 __global__ __launch_bounds__(C * Plan<LA>::TA,
                              mfma_waves(waves_per_simd(C * Plan<LA>::TA, 4 * col_lds_floats<LA, C, TW>()), PREC))
 void k_col(ColArgs p) {
+    static_assert(!Plan<LA>::HALF, "column tiles keep separate re / im arrays");
     constexpr int    NT   = C * Plan<LA>::TA;
     constexpr int    DATA = 2 * (1 << LA) * C;
     __shared__ float lds[col_lds_floats<LA, C, TW>()];

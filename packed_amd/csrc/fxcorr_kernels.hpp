@@ -67,7 +67,7 @@ void k_fxcorr(XcArgs p) {
     static_assert(PI::first_radix == RL, "fused multiply needs matching register maps");
     __shared__ float lds[xc_lds_floats<LA, G>()];
     float*           lre = lds;
-    float*           lim = lds + RowShape<LA, G>::DATA / 2;
+    float*           lim = RowShape<LA, G>::im(lds);
     float*           ltw = lds + RowShape<LA, G>::DATA;
     TwTable<LA>::fill(ltw, p.twA, threadIdx.x, G * TA);
     __syncthreads();
@@ -76,7 +76,8 @@ void k_fxcorr(XcArgs p) {
     if constexpr (PREC == PREC_BF16X3)
         mf.init();
     const int     t    = threadIdx.x;
-    const int     g    = t / TA;
+    // a wavefront lies in one block when TA >= 64: keep the block's offsets in SGPRs
+    const int     g    = TA >= 64 ? __builtin_amdgcn_readfirstlane(t / TA) : t / TA;
     const int     jt   = t % TA;
     const int64_t bg   = (int64_t)blockIdx.x * G + g; // block over all channels
     const bool    ok   = bg < p.nblocks * p.channels;
@@ -87,38 +88,81 @@ void k_fxcorr(XcArgs p) {
     float*        dst  = p.dst + ch * p.dst_dist;
     auto          addr = [&](int b, int o) { return g * STR + b + (b >> 5) + o + (o >> 5); };
 
+    // Fast addressing (B32, block offset a multiple of both packs): element off + a of the
+    // signal sits at poff(off) + poff(a), so a resource based at poff(off) serves the block, and
+    // its range check supplies the zeros past the signal (loads) and drops the outputs past
+    // `keep` (stores): no per-element branches.
+    const bool aligned = B32 && (off & (((int64_t)1 << p.src_lp) - 1)) == 0 &&
+                         (off & (((int64_t)1 << p.dst_lp) - 1)) == 0;
     cf v[PPT];
+    if (aligned && !p.hre) {
+        const int64_t  rem = p.len - off;
+        const uint32_t nv  = ok ? (uint32_t)(rem < Plan<LA>::A ? rem : Plan<LA>::A) : 0u;
+        const BufR     sr(src + poff((uint32_t)off, p.src_lp), p.src_lp, poff(nv, p.src_lp));
+        const uint32_t vo  = poff((uint32_t)jt, p.src_lp);
 #pragma unroll
-    for (int m = 0; m < PPT / R0; ++m)
+        for (int m = 0; m < PPT / R0; ++m)
 #pragma unroll
-        for (int r = 0; r < R0; ++r) {
-            int     a = in_elem<LA>(jt, m, r);
-            int64_t i = off + a;
-            cf      x = {0.f, 0.f};
-            if (ok && i < p.len) {
-                if (p.hre && a >= p.step) {
-                    int64_t h = bg * p.hlen + (a - p.step);
-                    x         = {p.hre[h], p.him[h]};
-                } else if constexpr (B32) {
-                    x = BufR(src, p.src_lp).ld(poff((uint32_t)i, p.src_lp), 0);
-                } else {
-                    x = ld_cx64(src, i, p.src_lp);
-                }
+            for (int r = 0; r < R0; ++r) {
+                uint32_t c    = (uint32_t)(in_elem<LA>(jt, m, r) - jt);
+                v[m * R0 + r] = sr.ld(vo + poff(c, p.src_lp), 0);
             }
-            v[m * R0 + r] = x;
-        }
+    } else {
+#pragma unroll
+        for (int m = 0; m < PPT / R0; ++m)
+#pragma unroll
+            for (int r = 0; r < R0; ++r) {
+                int     a = in_elem<LA>(jt, m, r);
+                int64_t i = off + a;
+                cf      x = {0.f, 0.f};
+                if (ok && i < p.len) {
+                    if (p.hre && a >= p.step) {
+                        int64_t h = bg * p.hlen + (a - p.step);
+                        x         = {p.hre[h], p.him[h]};
+                    } else if constexpr (B32) {
+                        x = BufR(src, p.src_lp).ld(poff((uint32_t)i, p.src_lp), 0);
+                    } else {
+                        x = ld_cx64(src, i, p.src_lp);
+                    }
+                }
+                v[m * R0 + r] = x;
+            }
+    }
     stockham<LA, false, decltype(addr), PREC>(v, lre, lim, jt, addr, ltw, &mf);
+    {
+        // Y = conj(X) * K.  The spectrum is read in two halves: the second half's offset depends
+        // (opaquely) on the first half's products, so at most half of it is in registers beside v.
+        const BufR kr(p.kre, LA); // split spectrum: re[N], im[N]
+        constexpr int NG = PPT / RL;
+        int           jk = jt;
 #pragma unroll
-    for (int m = 0; m < PPT / RL; ++m)
+        for (int m = 0; m < NG; ++m) {
+            if constexpr (NG >= 2 && PPT >= 32) {
+                if (m == NG / 2)
+                    asm volatile("" : "+v"(jk) : "v"(v[0].x), "v"(v[(NG / 2) * RL - 1].y));
+            }
 #pragma unroll
-        for (int r = 0; r < RL; ++r) {
-            int e         = out_elem<LA, false>(jt, m, r);
-            cf  k         = BufR(p.kre, LA).ld((uint32_t)e, 0); // split spectrum: re[N], im[N]
-            v[m * RL + r] = cmul(cconj(v[m * RL + r]), k);
+            for (int r = 0; r < RL; ++r) {
+                uint32_t c    = (uint32_t)(out_elem<LA, false>(jt, m, r) - jt);
+                cf       k    = kr.ld((uint32_t)jk, c, 0);
+                v[m * RL + r] = cmul(cconj(v[m * RL + r]), k);
+            }
         }
+    }
     stockham<LA, true, decltype(addr), PREC>(v, lre, lim, jt, addr, ltw, &mf);
-    if (ok) {
-        int64_t keep = p.len - off < p.step ? p.len - off : p.step;
+    const int64_t keep = p.len - off < p.step ? p.len - off : p.step;
+    if (aligned) {
+        const int      jt = opaque_after_memory(t % TA);
+        const BufW     dw(dst + poff((uint32_t)off, p.dst_lp), p.dst_lp, ok ? poff((uint32_t)keep, p.dst_lp) : 0u);
+        const uint32_t vo = poff((uint32_t)jt, p.dst_lp);
+#pragma unroll
+        for (int m = 0; m < PPT / RLI; ++m)
+#pragma unroll
+            for (int r = 0; r < RLI; ++r) {
+                uint32_t c = (uint32_t)(out_elem<LA, true>(jt, m, r) - jt);
+                dw.st(vo + poff(c, p.dst_lp), cconj(v[m * RLI + r]), 0);
+            }
+    } else if (ok) {
 #pragma unroll
         for (int m = 0; m < PPT / RLI; ++m)
 #pragma unroll

@@ -56,7 +56,7 @@ __device__ __forceinline__ void queue_item(const QueueArgs& q, const QItem& it, 
     float* ltw             = lds + RowShape<LA, G>::DATA;
     TwTable<LA>::fill(ltw, p.twA, threadIdx.x, kQueueThreads);
     __syncthreads();
-    row_group<LA, G, LD_CONTIG, ST_CONTIG, 0, (Plan<LA>::TA >= 64), 0>(p, 0, lds, lds + RowShape<LA, G>::DATA / 2, ltw,
+    row_group<LA, G, LD_CONTIG, ST_CONTIG, 0, (Plan<LA>::TA >= 64), 0>(p, 0, lds, RowShape<LA, G>::im(lds), ltw,
                                                                          nullptr);
 }
 

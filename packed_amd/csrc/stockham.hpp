@@ -30,9 +30,14 @@ struct Plan {
     static constexpr int NFULL   = LA / 4;           // radix-16 stages
     static constexpr int REM     = LA % 4;           // bits left for the small-radix stage
     static constexpr int NSTAGES = NFULL + (REM ? 1 : 0);
-    // points per thread: 32 for the largest sizes keeps a transform within 512 threads
-    static constexpr int PPT     = LA >= 13 ? 32 : (LA >= 4 ? 16 : A);
+    // points per thread: more for the largest sizes, so that a 2^13 transform takes 256 threads
+    // and a 2^14 transform 256 threads as well (two workgroups share a CU: one loads or stores
+    // while the other computes)
+    static constexpr int PPT     = LA >= 14 ? 64 : LA >= 13 ? 32 : (LA >= 4 ? 16 : A);
     static constexpr int TA      = A / PPT;          // threads per transform
+    // HALF: the stage exchange moves real parts, then imaginary parts, through one A-float LDS
+    // array (2^14 points: 66 KiB instead of 132 KiB, so two workgroups fit a CU)
+    static constexpr bool HALF   = LA >= 14;
     static constexpr int radix(int s) {
         if (REM == 0)
             return 16;
@@ -79,12 +84,72 @@ struct TwTable {
     }
 };
 
+// A copy of x that the compiler cannot see through: per-stage LDS addresses derived from it are
+// computed where they are used instead of being hoisted (and spilled) for all stages at once.
+__device__ __forceinline__ int opaque(int x) {
+    asm volatile("" : "+v"(x));
+    return x;
+}
+// Scalar form: uniform values derived from it (offsets from a run-time pack size) are
+// computed where used instead of all being held in SGPRs at once.
+__device__ __forceinline__ int opaque_s(int x) {
+    asm volatile("" : "+s"(x));
+    return x;
+}
+// The same, also ordered after every memory access before it: addresses and predicates of a
+// store phase are then computed after the last LDS exchange, not kept live across it.
+__device__ __forceinline__ int opaque_after_memory(int x) {
+    asm volatile("" : "+v"(x) : : "memory");
+    return x;
+}
+
+// Reads of stage s (component C: 0 real, 1 imaginary, 2 both) from LDS into v.
+template<int LA, bool REV, int s, int C, class Addr>
+__device__ __forceinline__ void stage_read(cf* v, const float* lre, const float* lim, int jt, const Addr& addr) {
+    using P          = Plan<LA, REV>;
+    constexpr int R  = P::radix(s);
+    constexpr int NB = P::PPT / R;
+#pragma unroll
+    for (int m = 0; m < NB; ++m) {
+        int j = jt + m * P::TA;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            int o = addr(j, r * (P::A / R));
+            if constexpr (C != 1)
+                v[m * R + r].x = lre[o];
+            if constexpr (C != 0)
+                v[m * R + r].y = lim[o];
+        }
+    }
+}
+// Writes of stage s (component C) from v to LDS.
+template<int LA, bool REV, int s, int C, class Addr>
+__device__ __forceinline__ void stage_write(const cf* v, float* lre, float* lim, int jt, const Addr& addr) {
+    using P          = Plan<LA, REV>;
+    constexpr int R  = P::radix(s);
+    constexpr int Ns = P::ns(s);
+    constexpr int NB = P::PPT / R;
+#pragma unroll
+    for (int m = 0; m < NB; ++m) {
+        int j  = jt + m * P::TA;
+        int hi = (j / Ns) * Ns * R + (j & (Ns - 1));
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            int o = addr(hi, r * Ns);
+            if constexpr (C != 1)
+                lre[o] = v[m * R + r].x;
+            if constexpr (C != 0)
+                lim[o] = v[m * R + r].y;
+        }
+    }
+}
+
 // PREC_BF16X3 runs the radix-16 butterflies on the matrix cores (mf holds the operands).
+// With Plan::HALF, lre and lim are the same array and each exchange runs in two halves.
 template<int LA, bool REV, class Addr, int PREC = PREC_FP32>
-__device__ __forceinline__ void stockham(cf* v, float* lre, float* lim, int jt, const Addr& addr, const float* ltw,
+__device__ __forceinline__ void stockham(cf* v, float* lre, float* lim, int jt0, const Addr& addr, const float* ltw,
                                          const MfmaDft16* mf = nullptr) {
     using P           = Plan<LA, REV>;
-    constexpr int A   = P::A;
     constexpr int TA  = P::TA;
     constexpr int PPT = P::PPT;
     sfor<0, P::NSTAGES>([&](auto ss) {
@@ -92,17 +157,12 @@ __device__ __forceinline__ void stockham(cf* v, float* lre, float* lim, int jt, 
         constexpr int R  = P::radix(s);
         constexpr int Ns = P::ns(s);
         constexpr int NB = PPT / R; // butterflies per thread
-        if constexpr (s > 0) {
+        // (the 2^14-point plans need the registers; smaller plans run faster with the addresses
+        // precomputed)
+        const int     jt = P::HALF ? opaque(jt0) : jt0;
+        if constexpr (s > 0 && !P::HALF) {
             __syncthreads(); // previous stage's LDS writes are visible
-#pragma unroll
-            for (int m = 0; m < NB; ++m) {
-                int j = jt + m * TA;
-#pragma unroll
-                for (int r = 0; r < R; ++r) {
-                    int o        = addr(j, r * (A / R));
-                    v[m * R + r] = cf{lre[o], lim[o]};
-                }
-            }
+            stage_read<LA, REV, s, 2>(v, lre, lim, jt, addr);
         }
         // twiddle and butterfly one radix-R group at a time, so only one group's twiddles are live
 #pragma unroll
@@ -123,16 +183,17 @@ __device__ __forceinline__ void stockham(cf* v, float* lre, float* lim, int jt, 
         }
         if constexpr (s + 1 < P::NSTAGES) {
             __syncthreads(); // every read of this stage is done before LDS is overwritten
-#pragma unroll
-            for (int m = 0; m < NB; ++m) {
-                int j  = jt + m * TA;
-                int hi = (j / Ns) * Ns * R + (j & (Ns - 1));
-#pragma unroll
-                for (int r = 0; r < R; ++r) {
-                    int o  = addr(hi, r * Ns);
-                    lre[o] = v[m * R + r].x;
-                    lim[o] = v[m * R + r].y;
-                }
+            if constexpr (!P::HALF) {
+                stage_write<LA, REV, s, 2>(v, lre, lim, jt, addr);
+            } else {
+                // real parts out and back in the next stage's layout, then imaginary parts
+                stage_write<LA, REV, s, 0>(v, lre, lim, jt, addr);
+                __syncthreads();
+                stage_read<LA, REV, s + 1, 0>(v, lre, lim, jt, addr);
+                __syncthreads();
+                stage_write<LA, REV, s, 1>(v, lre, lim, jt, addr);
+                __syncthreads();
+                stage_read<LA, REV, s + 1, 1>(v, lre, lim, jt, addr);
             }
         }
     });
