@@ -4,15 +4,16 @@
 // of the FP32 path) are computed as two real 32x32x16 products per operand pair:
 //   [Yr; Yi] = [Fr; Fi] * Xr + [-Fi; Fr] * Xi,  F[k][r] = exp(-2*pi*i*k*r/16),
 // with the 32 outputs of 32 butterflies in one 32x32 FP32 accumulator
-// (v_mfma_f32_32x32x16_bf16).  Operands are split into three BF16 terms by truncation, which is
-// exact (8 + 8 + 8 bits cover the FP32 significand): x = x0 + x1 + x2, F = F0 + F1 + F2, and the
-// six products with combined weight >= 2^-16 are accumulated (smallest first) in FP32:
+// (v_mfma_f32_32x32x16_bf16).  Operands are split into three BF16 terms: F by truncation, which is
+// exact (8 + 8 + 8 bits cover the FP32 significand), the data by round-to-nearest conversion
+// (v_cvt_pk_bf16_f32, residual below 2^-24): x = x0 + x1 + x2, F = F0 + F1 + F2, and the six
+// products with combined weight >= 2^-16 are accumulated (smallest first) in FP32:
 //   F1x1 + F0x2 + F2x0 + F0x1 + F1x0 + F0x0;  the dropped terms are below 2^-24 relative.
 //
 // Lane layout of v_mfma_f32_32x32x16_{bf16,f16}: A (32x16) lane l holds A[l%32][8*(l/32) + t],
 // B (16x32) lane l holds B[8*(l/32) + t][l%32], t < 8; the FP32 accumulator lane l holds
-// D[8*(v/4) + 4*(l/32) + v%4][l%32], v < 16.  Two half-wave exchanges (lane l <-> l^32) move the
-// butterflies between the one-butterfly-per-lane layout and this one.
+// D[8*(v/4) + 4*(l/32) + v%4][l%32], v < 16.  Two half-wave exchanges (v_permlane32_swap) move
+// the butterflies between the one-butterfly-per-lane layout and this one.
 #pragma once
 #include "dev_common.hpp"
 
@@ -33,6 +34,22 @@ __device__ __forceinline__ void split3(float x, __bf16& t0, __bf16& t1, __bf16& 
     t0          = __builtin_bit_cast(__bf16, (uint16_t)(u0 >> 16));
     t1          = __builtin_bit_cast(__bf16, (uint16_t)(u1 >> 16));
     t2          = __builtin_bit_cast(__bf16, (uint16_t)(u2 >> 16));
+}
+
+typedef __bf16   bf16x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// Round-to-nearest three-term split of the pair (a, b) into packed BF16 pairs t0, t1, t2 (a in
+// the low half) with v_cvt_pk_bf16_f32: a = a0 + a1 + a2 + e, |e| <= 2^-24 |a| (each remainder
+// is exact in FP32 and each rounding keeps 8 significant bits).
+__device__ __forceinline__ void split3_pk(float a, float b, uint32_t& t0, uint32_t& t1, uint32_t& t2) {
+    t0       = __builtin_bit_cast(uint32_t, bf16x2{(__bf16)a, (__bf16)b});
+    float ra = a - __uint_as_float(t0 << 16);
+    float rb = b - __uint_as_float(t0 & 0xFFFF0000u);
+    t1       = __builtin_bit_cast(uint32_t, bf16x2{(__bf16)ra, (__bf16)rb});
+    ra -= __uint_as_float(t1 << 16);
+    rb -= __uint_as_float(t1 & 0xFFFF0000u);
+    t2 = __builtin_bit_cast(uint32_t, bf16x2{(__bf16)ra, (__bf16)rb});
 }
 
 // Per-lane A operands: [Fr; Fi] and [-Fi; Fr], three split terms each.
@@ -62,14 +79,20 @@ struct MfmaDft16 {
 
     // 32 butterflies: xr/xi hold this lane's 8 inputs of its butterfly (rows 8h .. 8h+7).
     __device__ floatx16 run(const float* xr, const float* xi) const {
+        u32x4 wr[3], wi[3];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            uint32_t a0, a1, a2, b0, b1, b2;
+            split3_pk(xr[2 * t], xr[2 * t + 1], a0, a1, a2);
+            split3_pk(xi[2 * t], xi[2 * t + 1], b0, b1, b2);
+            wr[0][t] = a0, wr[1][t] = a1, wr[2][t] = a2;
+            wi[0][t] = b0, wi[1][t] = b1, wi[2][t] = b2;
+        }
         bf16x8 br[3], bi[3];
 #pragma unroll
-        for (int t = 0; t < 8; ++t) {
-            __bf16 p0, p1, p2;
-            split3(xr[t], p0, p1, p2);
-            br[0][t] = p0, br[1][t] = p1, br[2][t] = p2;
-            split3(xi[t], p0, p1, p2);
-            bi[0][t] = p0, bi[1][t] = p1, bi[2][t] = p2;
+        for (int k = 0; k < 3; ++k) {
+            br[k] = __builtin_bit_cast(bf16x8, wr[k]);
+            bi[k] = __builtin_bit_cast(bf16x8, wi[k]);
         }
         floatx16 d = {};
         // smallest terms first
@@ -89,45 +112,33 @@ struct MfmaDft16 {
     }
 
     // In-place DFT16 of v[0..15] for every lane (all 64 lanes of the wavefront take part).
+    // v_permlane32_swap(a, b) returns ([a_lo | b_lo], [a_hi | b_hi]) (lanes 0-31 | 32-63), so one
+    // swap per value pair moves the butterflies into the MFMA layout and back, with no selects.
     __device__ void dft16(cf* v) const {
-        const int h = (threadIdx.x >> 5) & 1;
-        // group 0 = butterflies of lanes 0..31, group 1 = lanes 32..63; each lane supplies rows
-        // 8h .. 8h+7 of one butterfly per group
+        // group 0 = butterflies of lanes 0..31, group 1 = lanes 32..63; low lanes supply rows
+        // 0..7 of a group's butterfly, high lanes rows 8..15
         float g0r[8], g0i[8], g1r[8], g1i[8];
 #pragma unroll
         for (int t = 0; t < 8; ++t) {
-            cf    send = h ? v[t] : v[8 + t];
-            float rr   = __shfl_xor(send.x, 32);
-            float ri   = __shfl_xor(send.y, 32);
-            if (h == 0) {
-                g0r[t] = v[t].x, g0i[t] = v[t].y; // own butterfly, rows 0..7
-                g1r[t] = rr, g1i[t] = ri;         // partner's butterfly, rows 0..7
-            } else {
-                g0r[t] = rr, g0i[t] = ri;                 // partner's butterfly, rows 8..15
-                g1r[t] = v[8 + t].x, g1i[t] = v[8 + t].y; // own butterfly, rows 8..15
-            }
+            auto sr = __builtin_amdgcn_permlane32_swap(__float_as_uint(v[t].x), __float_as_uint(v[8 + t].x), false,
+                                                       false);
+            auto si = __builtin_amdgcn_permlane32_swap(__float_as_uint(v[t].y), __float_as_uint(v[8 + t].y), false,
+                                                       false);
+            g0r[t] = __uint_as_float(sr[0]), g1r[t] = __uint_as_float(sr[1]);
+            g0i[t] = __uint_as_float(si[0]), g1i[t] = __uint_as_float(si[1]);
         }
         floatx16 d0 = run(g0r, g0i);
         floatx16 d1 = run(g1r, g1i);
-        // accumulator v: row 8*(v/4) + 4h + v%4 -> Yr[k] (v < 8) or Yi[k] (v >= 8) with
-        // k = 8*((v/4)&1) + 4h + v%4; lane keeps its own butterfly's rows, swaps the rest
+        // accumulator element q < 8 of lane l holds rows k + 4h (re; q + 8: im), k = 8*(q/4) + q%4,
+        // of column l%32: after the swap every lane has rows k and k + 4 of its own butterfly
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
-            int   k     = 8 * (q >> 2) + (q & 3); // + 4h
-            float own_r = h ? d1[q] : d0[q];
-            float own_i = h ? d1[q + 8] : d0[q + 8];
-            float snd_r = h ? d0[q] : d1[q];
-            float snd_i = h ? d0[q + 8] : d1[q + 8];
-            float rr    = __shfl_xor(snd_r, 32);
-            float ri    = __shfl_xor(snd_i, 32);
-            // own rows: k + 4h; received rows: k + 4(1-h)
-            if (h == 0) {
-                v[k]     = cf{own_r, own_i};
-                v[k + 4] = cf{rr, ri};
-            } else {
-                v[k + 4] = cf{own_r, own_i};
-                v[k]     = cf{rr, ri};
-            }
+            int  k  = 8 * (q >> 2) + (q & 3);
+            auto sr = __builtin_amdgcn_permlane32_swap(__float_as_uint(d0[q]), __float_as_uint(d1[q]), false, false);
+            auto si =
+                __builtin_amdgcn_permlane32_swap(__float_as_uint(d0[q + 8]), __float_as_uint(d1[q + 8]), false, false);
+            v[k]     = cf{__uint_as_float(sr[0]), __uint_as_float(si[0])};
+            v[k + 4] = cf{__uint_as_float(sr[1]), __uint_as_float(si[1])};
         }
     }
 };

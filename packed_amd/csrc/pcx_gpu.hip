@@ -264,8 +264,8 @@ struct pcx_plan_s {
     int64_t chunk        = 0; // transforms per four-step chunk
     const FourStep* shape = nullptr; // four-step kernels and tile sizes
     int     precision    = PCX_PREC_FP32;
-    int64_t grid_col     = 1024; // grid caps of the two passes (persistent loop; 0: one workgroup per tile)
-    int64_t grid_row     = 1024;
+    int64_t grid_col     = 2048; // grid caps of the two passes (persistent loop; 0: one workgroup per tile)
+    int64_t grid_row     = 2048;
     int     nt_flags     = 0; // bit 0/1: streaming load/store of src/dst, bit 2/3: of the scratch
     float2* tw_dif       = nullptr; // W_M for the one-pass k_dif kernel
     bool    use_dif      = false;   // out-of-place transforms of 2^15 run as one k_dif pass
