@@ -38,6 +38,8 @@ def parse():
     ap.add_argument("--quick", action="store_true", help="time the forward step only (for profiler runs)")
     ap.add_argument("--log2n", type=int, default=N_LOG2)
     ap.add_argument("--batch", type=int, default=BATCH)
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16x3"],
+                    help="arithmetic of the default workload (bf16x3: split-BF16 matrix-core butterflies)")
     ap.add_argument("--copy", action="store_true",
                     help="time a plain device copy of the same bytes instead (calibrates the HBM counters)")
     ap.add_argument("--config", type=int, default=2, choices=[1, 2, 3, 4, 5],
@@ -46,16 +48,21 @@ def parse():
     return ap.parse_args()
 
 
-def measured_traffic(log2n: int, batch: int):
-    """HBM bytes per step from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this
-    workload (counters scaled by a plain copy of the same bytes); None for other shapes."""
-    path = os.path.join(ROOT, "profiles", f"fft_fourstep_n{1 << log2n}_b{batch}_hbm_traffic.md")
+def profile_figure(name: str, key: str):
+    """A figure (`key = value` line) from a committed rocprofv3 summary under profiles/."""
+    path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
     for line in open(path):
-        if line.startswith("traffic_bytes_per_step"):
+        if line.startswith(key):
             return int(line.split("=")[1])
     return None
+
+
+def measured_traffic(log2n: int, batch: int):
+    """HBM bytes per step from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this
+    workload (counters scaled by a plain copy of the same bytes); None for other shapes."""
+    return profile_figure(f"fft_fourstep_n{1 << log2n}_b{batch}_hbm_traffic.md", "traffic_bytes_per_step")
 
 
 def time_steps(fn, steps, warmup):
@@ -144,7 +151,7 @@ def main():
     check(lib().pcx_generate(x.data_ptr(), n, pts, SEED, 1, 1.0, stream))
     torch.cuda.synchronize()
 
-    fwd = pa.fft_unit(n, order="normal")
+    fwd = pa.fft_unit(n, order="normal", precision=args.precision)
     fwd_br = pa.fft_unit(n, order="bit_reversed")
 
     def step():
@@ -161,7 +168,7 @@ def main():
         print(json.dumps({"ms_per_step": round(ms, 4)}))
         return
     ms_inv = time_steps(step_inv, max(3, args.steps // 2), 1)
-    ms_br = time_steps(step_br, max(3, args.steps // 2), 1)
+    ms_br = time_steps(step_br, max(3, args.steps // 2), 1)  # FP32 (the matrix-core form is natural order only)
 
     flops = 5.0 * n * math.log2(n) * batch
     gflops = flops / (ms * 1e-3) / 1e9
@@ -183,7 +190,7 @@ def main():
     want_br = np.fft.fft(xin[:n].astype(np.float64) + 1j * xin[n:])[O.slot_freq("bit_reversed", n)]
     err_br = O.rel_l2(zin[:n] + 1j * zin[n:], want_br)
 
-    traffic = measured_traffic(args.log2n, batch)
+    traffic = measured_traffic(args.log2n, batch) if args.precision == "fp32" else None
     out = {
         "metric": "C2C FFT GFLOP/s (5N*log2N), N=2^16 batch 4096, FP32 split re/im",
         "value": round(gflops, 1),
@@ -195,11 +202,11 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "f32" if args.precision == "fp32" else "f32 (bf16x3 split operands, f32 accumulation)",
         "data": "synthetic (standard-normal re/im from splitmix64 seed 0x5EED0002, generated in HBM)",
         "config": {"workload": f"batched C2C FFT forward, N=2^{args.log2n}, batch {batch}, split re/im "
                                f"(pack=N), natural order, out of place",
-                   "fft_size": n, "batch": batch, "order": "normal", "precision": "fp32"},
+                   "fft_size": n, "batch": batch, "order": "normal", "precision": args.precision},
         "roofline": {
             "bound": "hbm",
             "achieved": round(achieved, 1),
@@ -234,6 +241,8 @@ def run_copy(args):
     return {"workload": f"device copy of {2 * pts * 4} bytes", "ms_per_copy": round(ms, 4),
             "gb_per_s": round(2 * 2 * pts * 4 / (ms * 1e-3) / 1e9, 1), "bytes_read": 2 * pts * 4,
             "bytes_written": 2 * pts * 4}
+
+
 def run_config1(args):
     """N=1024 single FP32 FFT, pack 8, forward then inverse: latency, order exactness, round trip."""
     import numpy as np
@@ -263,7 +272,35 @@ def run_config1(args):
             unit.ifft(v, w)
 
         us = time_steps(pair, 2000, 50) * 1e3
-        res["orders"][order] = {"fwd_inv_us": round(us, 2), "rel_l2_vs_fp64": float(f"{err:.3e}"),
+        # device time per pair without host launch overhead: 100 pairs captured in one graph
+        # (the path makes no allocations and no memsets), replayed; every replay is checked
+        # against the eager result
+        v.data.copy_(torch.from_numpy(O.to_packed(x, 8)).cuda())
+        torch.cuda.synchronize()
+        for _ in range(100):
+            pair()
+        torch.cuda.synchronize()
+        eager = v.data.clone()
+        v.data.copy_(torch.from_numpy(O.to_packed(x, 8)).cuda())
+        g = torch.cuda.CUDAGraph()
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            g.capture_begin()
+            for _ in range(100):
+                pair()
+            g.capture_end()
+        torch.cuda.synchronize()
+        replays_ok = True
+        for _ in range(3):
+            v.data.copy_(torch.from_numpy(O.to_packed(x, 8)).cuda())
+            torch.cuda.synchronize()
+            g.replay()
+            torch.cuda.synchronize()
+            replays_ok &= bool(torch.equal(v.data, eager))
+        g_us = time_steps(g.replay, 20, 2) * 1e3 / 100
+        res["orders"][order] = {"fwd_inv_us": round(us, 2), "fwd_inv_us_graph": round(g_us, 2),
+                                "graph_replays_equal_eager": replays_ok, "rel_l2_vs_fp64": float(f"{err:.3e}"),
                                 "round_trip_rel_l2": float(f"{rt:.3e}")}
     res["value"] = res["orders"]["normal"]["fwd_inv_us"]
     return res
@@ -315,27 +352,33 @@ def run_config3(args):
     same = float(np.max(np.abs(work.to_complex() - res_out)))
     flops = blocks * (2 * 5 * N * math.log2(N) + 6 * N)
     algo_bytes = 16 * L + 8 * N
-    # host-core baseline: the oracle's FP32 port on a 2^20-sample slice of the same signal
-    threads = max(1, min(os.cpu_count() or 1, 16))
-    fs = f[: 1 << 20]
-    O.port_fxcorr(fs, g, N, threads)
-    t = time.perf_counter()
-    _, nb = O.port_fxcorr(fs, g, N, threads)
-    dt = time.perf_counter() - t
-    cpu_flops = nb * (2 * 5 * N * math.log2(N) + 6 * N)
-    return {
+    res = {
         "metric": "fxcorr GFLOP/s (blocks*(2*5N*log2N+6N)), L=2^24, M=4096, N=16384",
         "value": round(flops / (ms_oop * 1e-3) / 1e9, 1), "unit": "GFLOP/s", "higher_is_better": True,
         "ms_out_of_place": round(ms_oop, 4), "ms_in_place": round(ms_in, 4), "blocks": blocks, "step": step,
         "msamples_per_s": round(L / (ms_oop * 1e-3) / 1e6, 1),
         "roofline": {"bound": "hbm", "achieved": round(algo_bytes / (ms_oop * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(algo_bytes / (ms_oop * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+                     "unit": "GB/s", "frac": round(algo_bytes / (ms_oop * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                     "traffic": profile_figure("fxcorr_l16777216_m4096_n16384_trace_traffic.md",
+                                               "traffic_bytes_per_launch"),
+                     "kernel": "k_fxcorr (fused forward FFT, conj-multiply, inverse FFT per block)"},
         "peaks_index_exact": bool(np.array_equal(top, offs)), "rel_l2_vs_fp64_first_2^16": float(f"{err:.3e}"),
         "in_place_equals_out_of_place_maxabs": same,
-        "cpu_baseline": {"value": round(cpu_flops / dt / 1e9, 2), "unit": "GFLOP/s", "cores": threads, "kind": "port",
-                         "sample": f"2^20-sample slice ({nb} blocks), same kernel and block size, {threads} threads"},
         "config": {"workload": "overlap-save cross-correlation, L=2^24, M=4096, N=16384, pack 8"},
     }
+    if not args.no_cpu:
+        # host-core baseline: the oracle's FP32 port on a 2^20-sample slice of the same signal
+        threads = max(1, min(os.cpu_count() or 1, 16))
+        fs = f[: 1 << 20]
+        O.port_fxcorr(fs, g, N, threads)
+        t = time.perf_counter()
+        _, nb = O.port_fxcorr(fs, g, N, threads)
+        dt = time.perf_counter() - t
+        cpu_flops = nb * (2 * 5 * N * math.log2(N) + 6 * N)
+        res["cpu_baseline"] = {"value": round(cpu_flops / dt / 1e9, 2), "unit": "GFLOP/s", "cores": threads,
+                               "kind": "port",
+                               "sample": f"2^20-sample slice ({nb} blocks), same kernel and block size, {threads} threads"}
+    return res
 
 
 def run_config4(args):
