@@ -79,7 +79,7 @@ void k_fxcorr(XcArgs p) {
     // a wavefront lies in one block when TA >= 64: keep the block's offsets in SGPRs
     const int     g    = TA >= 64 ? __builtin_amdgcn_readfirstlane(t / TA) : t / TA;
     const int     jt   = t % TA;
-    const int64_t bg   = (int64_t)blockIdx.x * G + g; // block over all channels
+    const int64_t bg   = xcd_item(0) * G + g; // block over all channels; neighbours share one L2
     const bool    ok   = bg < p.nblocks * p.channels;
     const int64_t ch   = ok ? bg / p.nblocks : 0;
     const int64_t b    = ok ? bg - ch * p.nblocks : 0;

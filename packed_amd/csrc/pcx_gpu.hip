@@ -84,12 +84,15 @@ using RowLaunch = void (*)(const RowArgs&, int64_t nrows, int64_t grid_cap, hipS
 using ColLaunch = void (*)(const ColArgs&, int64_t grid_cap, hipStream_t);
 using XcLaunch  = void (*)(const XcArgs&, hipStream_t);
 
-// grid_cap > 0 bounds the grid; the kernels then loop over their work (grid-stride).
+// grid_cap > 0 bounds the grid; the kernels then loop over their work (grid-stride).  Grids are
+// rounded up to a multiple of 8 for the XCD-aware item order (spare workgroups exit at once).
+int64_t grid8(int64_t n) { return (n + 7) & ~(int64_t)7; }
 template<int LA, int G, int LD, int ST, int TW, bool UNI = (Plan<LA>::TA >= 64), int ORD = 0, int PREC = PREC_FP32>
 void launch_row(const RowArgs& a, int64_t nrows, int64_t grid_cap, hipStream_t st) {
     int64_t nwg = (nrows + G - 1) / G;
     if (grid_cap > 0 && nwg > grid_cap)
         nwg = grid_cap;
+    nwg = grid8(nwg);
     hipLaunchKernelGGL((k_row<LA, G, LD, ST, TW, UNI, ORD, PREC>), dim3((unsigned)nwg), dim3(G * Plan<LA>::TA), 0, st,
                        a);
 }
@@ -98,12 +101,13 @@ void launch_col(const ColArgs& a, int64_t grid_cap, hipStream_t st) {
     int64_t nwg = a.ntiles;
     if (grid_cap > 0 && nwg > grid_cap)
         nwg = grid_cap;
+    nwg = grid8(nwg);
     hipLaunchKernelGGL((k_col<LA, C, TW, PREC>), dim3((unsigned)nwg), dim3(C * Plan<LA>::TA), 0, st, a);
 }
 template<int LA, int PREC = PREC_FP32>
 void launch_xc(const XcArgs& a, hipStream_t st) {
     constexpr int G   = default_rows<LA>();
-    int64_t       nwg = (a.nblocks * a.channels + G - 1) / G;
+    int64_t       nwg = grid8((a.nblocks * a.channels + G - 1) / G);
     // 32-bit buffer offsets cover a channel when its packed footprint stays below 2^29 floats
     int  lp    = a.src_lp > a.dst_lp ? a.src_lp : a.dst_lp;
     bool small = pidx64(a.len + (int64_t)(1 << LA), lp) < ((int64_t)1 << 29) && a.src_lp < 29 && a.dst_lp < 29;
