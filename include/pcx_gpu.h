@@ -134,6 +134,55 @@ int pcx_queue_launch(pcx_queue_t queue, void* stream);
 /* algorithmic work of one pass: sum 5*N*log2(N) flops and 16*N bytes */
 int pcx_queue_work(pcx_queue_t queue, double* flops, double* bytes);
 
+/* ------------------------------------------------------------------ vector expressions
+ * Device-side evaluation of the reference's vector expressions (vector_arithm.hpp:450-1130:
+ * binary + - * / of two vectors of any pack sizes, + - * / with a real or complex scalar on
+ * either side, conj) and of pack conversions (vector.hpp:55-75 copy between pack sizes;
+ * avx2_common.hpp:243-324).  An expression tree is flattened into a program for a machine with
+ * four complex registers per element; one fused kernel reads every operand once and writes the
+ * destination once, in its pack size, for every element i < n.
+ *   dst[i] = r0 after running prog[0 .. nprog) with the sources' element i.
+ * Arithmetic follows the reference's formulas (simd_common.hpp:296-410): complex division is
+ * (a * conj(b)) / (br^2 + bi^2); a real scalar adds to / subtracts from the real part only. */
+typedef enum {
+    PCX_VOP_LOAD = 0,  /* r[d] = src[a] element i */
+    PCX_VOP_ADD  = 1,  /* r[d] = r[a] + r[b] */
+    PCX_VOP_SUB  = 2,  /* r[d] = r[a] - r[b] */
+    PCX_VOP_MUL  = 3,  /* r[d] = r[a] * r[b] */
+    PCX_VOP_DIV  = 4,  /* r[d] = r[a] / r[b] */
+    PCX_VOP_CONJ = 5,  /* r[d] = conj(r[a]) */
+    PCX_VOP_ADDS = 6,  /* r[d] = r[a] + s          (s complex: s_re + i s_im) */
+    PCX_VOP_SUBS = 7,  /* r[d] = r[a] - s */
+    PCX_VOP_RSUBS = 8, /* r[d] = s - r[a] */
+    PCX_VOP_MULS = 9,  /* r[d] = r[a] * s */
+    PCX_VOP_DIVS = 10, /* r[d] = r[a] / s */
+    PCX_VOP_RDIVS = 11,/* r[d] = s / r[a] */
+    PCX_VOP_ADDR = 12, /* the same six with a real scalar s_re (s_im ignored) */
+    PCX_VOP_SUBR = 13,
+    PCX_VOP_RSUBR = 14,
+    PCX_VOP_MULR = 15,
+    PCX_VOP_DIVR = 16,
+    PCX_VOP_RDIVR = 17
+} pcx_vop;
+enum { PCX_VEXPR_MAX_PROG = 64, PCX_VEXPR_MAX_SRC = 16, PCX_VEXPR_REGS = 4 };
+typedef struct {
+    int32_t op;   /* pcx_vop */
+    int32_t d;    /* destination register, < PCX_VEXPR_REGS */
+    int32_t a;    /* register operand, or the source index for PCX_VOP_LOAD */
+    int32_t b;    /* second register operand (binary ops) */
+    float   s_re; /* scalar operand */
+    float   s_im;
+} pcx_vinstr;
+typedef struct {
+    const float* data; /* device pointer, packed layout */
+    int64_t      pack; /* pack size, a power of two */
+} pcx_vsource;
+int pcx_vexpr_eval(const pcx_vsource* srcs, int32_t nsrc, const pcx_vinstr* prog, int32_t nprog, float* dst,
+                   int64_t dst_pack, int64_t n, void* stream);
+/* Pack conversion: dst (pack dst_pack) = src (pack src_pack), n elements; in place only when the
+ * packs are equal (vector.hpp copy between pack sizes). */
+int pcx_repack(float* dst, int64_t dst_pack, const float* src, int64_t src_pack, int64_t n, void* stream);
+
 /* ------------------------------------------------------------------ memory and utilities
  * The reference's Allocator template parameter (allocators.hpp:11-48) decides where buffers
  * live; these back the C++ shim's device allocator. */

@@ -246,3 +246,63 @@ def rel_l2(a: np.ndarray, b: np.ndarray) -> float:
     b = np.asarray(b, np.complex128)
     d = np.linalg.norm(b)
     return float(np.linalg.norm(a - b) / (d if d > 0 else 1.0))
+
+
+# ---------------------------------------------------------------------------- vector expressions
+def cdiv_ref(a, b):
+    """simd_common.hpp:296-309 complex division: (a * conj(b)) / (br^2 + bi^2), in FP64."""
+    a = np.asarray(a, np.complex128)
+    b = np.asarray(b, np.complex128)
+    den = b.real * b.real + b.imag * b.imag
+    return ((a.real * b.real + a.imag * b.imag) + 1j * (a.imag * b.real - a.real * b.imag)) / den
+
+
+def vexpr_run(prog, srcs):
+    """FP64 restatement of the four-register machine that pcx_vexpr_eval runs per element
+    (include/pcx_gpu.h pcx_vop); srcs are complex arrays.  Real-scalar ops follow
+    simd_common.hpp:368-410 (add/sub touch the real part only)."""
+    names = ["LOAD", "ADD", "SUB", "MUL", "DIV", "CONJ", "ADDS", "SUBS", "RSUBS", "MULS", "DIVS", "RDIVS",
+             "ADDR", "SUBR", "RSUBR", "MULR", "DIVR", "RDIVR"]
+    n = len(srcs[0]) if srcs else 0
+    r = [np.zeros(n, np.complex128) for _ in range(4)]
+    for op, d, a, b, sre, sim in prog:
+        name = names[op]
+        x, y, s = r[a] if name != "LOAD" else None, r[b], complex(sre, sim)
+        if name == "LOAD":
+            z = np.asarray(srcs[a], np.complex128)
+        elif name == "ADD":
+            z = x + y
+        elif name == "SUB":
+            z = x - y
+        elif name == "MUL":
+            z = x * y
+        elif name == "DIV":
+            z = cdiv_ref(x, y)
+        elif name == "CONJ":
+            z = np.conj(x)
+        elif name == "ADDS":
+            z = x + s
+        elif name == "SUBS":
+            z = x - s
+        elif name == "RSUBS":
+            z = s - x
+        elif name == "MULS":
+            z = x * s
+        elif name == "DIVS":
+            z = cdiv_ref(x, s)
+        elif name == "RDIVS":
+            z = cdiv_ref(s, x)
+        elif name == "ADDR":
+            z = (x.real + sre) + 1j * x.imag
+        elif name == "SUBR":
+            z = (x.real - sre) + 1j * x.imag
+        elif name == "RSUBR":
+            z = (sre - x.real) - 1j * x.imag
+        elif name == "MULR":
+            z = x * sre
+        elif name == "DIVR":
+            z = x / sre
+        else:  # RDIVR
+            z = cdiv_ref(sre, x)
+        r[d] = z
+    return r[0]

@@ -2,5 +2,7 @@
 split-complex layout, behind a C ABI (include/pcx_gpu.h, libpcx_gpu.so)."""
 from ._lib import PcxError  # noqa: F401
 from .fft import fft_queue, fft_unit, fxcorr_unit, pidx, real_size, vector  # noqa: F401
+from .vexpr import conj, repack, store  # noqa: F401
 
-__all__ = ["fft_unit", "fxcorr_unit", "fft_queue", "vector", "pidx", "real_size", "PcxError"]
+__all__ = ["fft_unit", "fxcorr_unit", "fft_queue", "vector", "pidx", "real_size", "PcxError", "conj", "store",
+           "repack"]

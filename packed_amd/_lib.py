@@ -33,6 +33,25 @@ class Job(C.Structure):
                 ("src_offset", C.c_int64), ("dst_offset", C.c_int64)]
 
 
+class VSource(C.Structure):
+    """pcx_vsource (include/pcx_gpu.h)."""
+
+    _fields_ = [("data", C.c_void_p), ("pack", C.c_int64)]
+
+
+class VInstr(C.Structure):
+    """pcx_vinstr (include/pcx_gpu.h)."""
+
+    _fields_ = [("op", C.c_int32), ("d", C.c_int32), ("a", C.c_int32), ("b", C.c_int32),
+                ("s_re", C.c_float), ("s_im", C.c_float)]
+
+
+# pcx_vop (include/pcx_gpu.h)
+VOP = {name: i for i, name in enumerate(
+    ["LOAD", "ADD", "SUB", "MUL", "DIV", "CONJ", "ADDS", "SUBS", "RSUBS", "MULS", "DIVS", "RDIVS",
+     "ADDR", "SUBR", "RSUBR", "MULR", "DIVR", "RDIVR"])}
+VEXPR_MAX_PROG, VEXPR_MAX_SRC, VEXPR_REGS = 64, 16, 4
+
 # Every symbol include/pcx_gpu.h declares, with (restype, argtypes).
 _vp, _i64, _int, _sz, _u64, _dbl = C.c_void_p, C.c_int64, C.c_int, C.c_size_t, C.c_uint64, C.c_double
 SIGNATURES = {
@@ -68,6 +87,8 @@ SIGNATURES = {
     "pcx_memset": (_int, [_vp, _int, _sz]),
     "pcx_synchronize": (_int, [_vp]),
     "pcx_generate": (_int, [_vp, _i64, _i64, _u64, _int, _dbl, _vp]),
+    "pcx_vexpr_eval": (_int, [C.POINTER(VSource), C.c_int32, C.POINTER(VInstr), C.c_int32, _vp, _i64, _i64, _vp]),
+    "pcx_repack": (_int, [_vp, _i64, _vp, _i64, _i64, _vp]),
 }
 
 _lib = None

@@ -80,6 +80,36 @@ class vector:
         buf[idx + pack] = np.imag(z)
         return cls(len(z), pack, torch.from_numpy(buf).cuda())
 
+    # vector_arithm.hpp:450-495: lazy expressions, evaluated on the GPU by store()
+    def _expr(self):
+        from .vexpr import Leaf
+
+        return Leaf(self)
+
+    def __add__(self, o):
+        return self._expr() + o
+
+    def __radd__(self, o):
+        return o + self._expr()
+
+    def __sub__(self, o):
+        return self._expr() - o
+
+    def __rsub__(self, o):
+        return o - self._expr()
+
+    def __mul__(self, o):
+        return self._expr() * o
+
+    def __rmul__(self, o):
+        return o * self._expr()
+
+    def __truediv__(self, o):
+        return self._expr() / o
+
+    def __rtruediv__(self, o):
+        return o / self._expr()
+
     def to_complex(self, n: int | None = None, pack: int | None = None) -> np.ndarray:
         """Complex values read in `pack` layout (default: this vector's pack)."""
         n = self.n if n is None else n

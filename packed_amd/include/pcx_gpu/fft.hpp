@@ -67,6 +67,12 @@ struct device_allocator {
 };
 
 // pcxo::vector<T, PackSize, Allocator> (vector.hpp:26-382) resident in device memory.
+// Lazy vector expressions (vector_arithm.hpp in this directory) are marked by this trait.
+template<typename E>
+struct is_vexpr : std::false_type {};
+template<typename E, typename V>
+void store(const E& expr, V& dest);
+
 template<typename T, uZ PackSize = 8, typename Allocator = device_allocator<T>>
     requires std::same_as<T, float> && pack_size<PackSize>
 class vector {
@@ -100,6 +106,13 @@ public:
     vector& operator=(vector other) noexcept {
         std::swap(m_size, other.m_size);
         std::swap(m_ptr, other.m_ptr);
+        return *this;
+    }
+    // vector = expression (vector_arithm.hpp:405-447 store): evaluated on the GPU in one pass
+    template<typename E>
+        requires is_vexpr<E>::value
+    vector& operator=(const E& expr) {
+        store(expr, *this);
         return *this;
     }
     ~vector() {
@@ -332,3 +345,5 @@ private:
 };
 
 } // namespace pcx_gpu
+
+#include "vector_arithm.hpp"

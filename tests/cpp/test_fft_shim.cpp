@@ -244,6 +244,57 @@ static void test_fxcorr() {
     CHECK(xc2.step() == (N - (M - 1)) / 8 * 8, "fxcorr step");
 }
 
+// tests/test_op.cpp:89-226 test_bin_ops: the 16 vector-vector operations, then complex- and
+// real-scalar operations on either side, checked with equal_eps (10 FLT_EPSILON relative).
+template<size_t PackSize>
+static void test_bin_ops(size_t length) {
+    using namespace pcx_gpu;
+    const float        eps = std::numeric_limits<float>::epsilon();
+    std::vector<cxf>   lhs(length), rhs(length);
+    for (size_t i = 0; i < length; ++i) {
+        lhs[i] = cxf(i * eps, -1.f - (float)i);
+        rhs[i] = cxf((float)i + 1, i * eps);
+    }
+    vector<float, PackSize> L(lhs), R(rhs), out(length);
+    const cxf               cx(512 * (eps + 1), -512 * eps);
+    const float             re = 1024 * (eps - 1);
+    auto check = [&](const char* what, auto ref) {
+        auto got = out.to_host();
+        for (size_t i = 0; i < length; ++i) {
+            cxd want = ref(i);
+            cxd g    = cxd(got[i]);
+            double big = std::max(std::abs(g), std::abs(want));
+            if (!(g == want || std::abs(g - want) < big * 10 * eps)) {
+                CHECK(false, "%s pack %zu length %zu index %zu", what, PackSize, length, i);
+                return;
+            }
+        }
+    };
+    auto l = [&](size_t i) { return cxd(lhs[i]); };
+    auto r = [&](size_t i) { return cxd(rhs[i]); };
+    out = L + R;                 check("a+b", [&](size_t i) { return l(i) + r(i); });
+    out = L - conj(R);           check("a-conj(b)", [&](size_t i) { return l(i) - std::conj(r(i)); });
+    out = conj(L) * R;           check("conj(a)*b", [&](size_t i) { return std::conj(l(i)) * r(i); });
+    out = conj(L) / conj(R);     check("conj(a)/conj(b)", [&](size_t i) { return std::conj(l(i)) / std::conj(r(i)); });
+    out = L / R;                 check("a/b", [&](size_t i) { return l(i) / r(i); });
+    out = L * cx;                check("a*s", [&](size_t i) { return l(i) * cxd(cx); });
+    out = cx / R;                check("s/b", [&](size_t i) { return cxd(cx) / r(i); });
+    out = cx - conj(R);          check("s-conj(b)", [&](size_t i) { return cxd(cx) - std::conj(r(i)); });
+    out = L + re;                check("a+re", [&](size_t i) { return l(i) + (double)re; });
+    out = re - R;                check("re-b", [&](size_t i) { return (double)re - r(i); });
+    out = re / R;                check("re/b", [&](size_t i) { return (double)re / r(i); });
+    out = L / re;                check("a/re", [&](size_t i) { return l(i) / (double)re; });
+    // a fused pipeline across pack sizes: one kernel
+    vector<float, 1> L1(lhs);
+    out = (L1 + R) * conj(L - R) / (R + 2.f) - cx * L1;
+    check("fused", [&](size_t i) {
+        return (l(i) + r(i)) * std::conj(l(i) - r(i)) / (r(i) + 2.0) - cxd(cx) * l(i);
+    });
+    // repack through assignment of a vector of another pack size
+    store(L1, out);
+    check("repack", [&](size_t i) { return l(i); });
+}
+
 int main() {
     for (size_t n = 64; n <= 16384; n *= 2) { // test_fft.cpp:779-798 runs 2^6 .. 2^14
         test_fft_float<8>(n);
@@ -255,6 +306,11 @@ int main() {
     test_zero_extension();
     test_errors();
     test_fxcorr();
+    for (size_t n = 1; n < 32; ++n) { // test_op.cpp:241-252
+        test_bin_ops<1>(n);
+        test_bin_ops<8>(n);
+        test_bin_ops<32>(n);
+    }
     std::printf("%s: %d failed checks\n", g_fail ? "FAIL" : "OK", g_fail);
     return g_fail;
 }
