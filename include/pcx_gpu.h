@@ -87,6 +87,15 @@ int pcx_fft_batch(pcx_plan_t plan, int inverse, int normalized, float* dst, int6
                   int64_t dst_dist, const float* src, int64_t src_pack, int64_t src_dist,
                   int64_t src_size, int64_t batch, void* stream);
 
+/* FFT across a range of vectors: the transform of fft_unit_par (fft.hpp:2168-2572), whose
+ * output order is wrong in the reference (its test is disabled, test_fft.cpp:789); here it is
+ * natural.  For every k < length: dst_j[k] = sum_i src_i[k] * W^(i*j) (inverse: conjugate
+ * twiddles, scaled by 1/fft_size if normalized), i, j < fft_size <= 1024.  src_vecs / dst_vecs
+ * are device arrays of fft_size pointers to packed vectors of `length` elements; dst may equal
+ * src element for element (in place).  Normal-order plans only. */
+int pcx_fft_across(pcx_plan_t plan, int inverse, int normalized, float* const* dst_vecs, int64_t dst_pack,
+                   const float* const* src_vecs, int64_t src_pack, int64_t length, void* stream);
+
 /* ------------------------------------------------------------------ cross-correlation
  * Replaces fxcorr_unit<float>(g, fft_size) (fxcorr.hpp:62-69): spectrum of g (device, pack
  * g_pack, g_size taps) zero-extended to fft_size, scaled by 1/fft_size. */

@@ -89,6 +89,7 @@ SIGNATURES = {
     "pcx_generate": (_int, [_vp, _i64, _i64, _u64, _int, _dbl, _vp]),
     "pcx_vexpr_eval": (_int, [C.POINTER(VSource), C.c_int32, C.POINTER(VInstr), C.c_int32, _vp, _i64, _i64, _vp]),
     "pcx_repack": (_int, [_vp, _i64, _vp, _i64, _i64, _vp]),
+    "pcx_fft_across": (_int, [_vp, _int, _int, _vp, _i64, _vp, _i64, _i64, _vp]),
 }
 
 _lib = None

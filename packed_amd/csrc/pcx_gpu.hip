@@ -1249,3 +1249,51 @@ extern "C" int pcx_repack(float* dst, int64_t dst_pack, const float* src, int64_
     pcx_vinstr  ld{PCX_VOP_LOAD, 0, 0, 0, 0.f, 0.f};
     return pcx_vexpr_eval(&sv, 1, &ld, 1, dst, dst_pack, n, stream);
 }
+
+// ---------------------------------------------------------------- FFT across vectors
+namespace {
+using AcrossLaunch = void (*)(const AcrossArgs&, int64_t, hipStream_t);
+template<int LA>
+void launch_across(const AcrossArgs& a, int64_t grid, hipStream_t st) {
+    constexpr int C = LA <= 8 ? 64 : (LA == 9 ? 32 : 16);
+    int64_t       n = (a.len + C - 1) / C;
+    if (n > grid)
+        n = grid;
+    hipLaunchKernelGGL((k_across<LA, C>), dim3((unsigned)grid8(n)), dim3(C * Plan<LA>::TA), 0, st, a);
+}
+template<int... L>
+constexpr auto across_table(std::integer_sequence<int, L...>) {
+    return std::array<AcrossLaunch, sizeof...(L)>{&launch_across<L + 1>...};
+}
+constexpr int kMaxAcross = 10;
+const auto    kAcross    = across_table(std::make_integer_sequence<int, kMaxAcross>{});
+} // namespace
+
+extern "C" int pcx_fft_across(pcx_plan_t p, int inverse, int normalized, float* const* dst_vecs, int64_t dst_pack,
+                              const float* const* src_vecs, int64_t src_pack, int64_t length, void* stream) {
+    if (!p)
+        return fail(PCX_ERR_INVALID_ARGUMENT, "plan is null");
+    if (!dst_vecs || !src_vecs)
+        return fail(PCX_ERR_INVALID_ARGUMENT, "vector table is null");
+    if (length < 0)
+        return fail(PCX_ERR_INVALID_ARGUMENT, "length (which is " + s(length) + ") is negative");
+    if (int e = check_pack(dst_pack, "destination"))
+        return e;
+    if (int e = check_pack(src_pack, "source"))
+        return e;
+    if (p->order != PCX_ORDER_NORMAL)
+        return fail(PCX_ERR_UNSUPPORTED, "transforms across vectors are built for the normal order");
+    if (p->lg > kMaxAcross)
+        return fail(PCX_ERR_UNSUPPORTED, "transforms across vectors cover at most 1024 vectors (fft_size " + s(p->n) + ")");
+    if (length == 0)
+        return PCX_SUCCESS;
+    AcrossArgs a{};
+    a.src = src_vecs, a.dst = dst_vecs, a.len = length;
+    a.src_lp = ilog2(src_pack), a.dst_lp = ilog2(dst_pack);
+    a.conj_in = inverse, a.conj_out = inverse;
+    a.scale   = (inverse && normalized) ? (float)(1.0 / (double)p->n) : 1.0f;
+    a.twA     = p->tw_row;
+    kAcross[p->lg - 1](a, 2048, (hipStream_t)stream);
+    HIPCHK(hipGetLastError());
+    return PCX_SUCCESS;
+}

@@ -177,6 +177,25 @@ class fft_unit:
             raise PcxError(_lib.ERR_SIZE_MISMATCH, f"{what} size (which is {v.size()}) is not equal to fft size "
                                                    f"(which is {self.size()})")
 
+    def across(self, dst_vecs, src_vecs, inverse: bool = False, normalized: bool = True, stream=None) -> None:
+        """FFT across a range of vectors (fft_unit_par, fft.hpp:2168-2572, in natural order):
+        dst_vecs[j][k] = sum_i src_vecs[i][k] W^(i j) for every element k."""
+        torch = _torch()
+        n = self.size()
+        if len(dst_vecs) != n or len(src_vecs) != n:
+            raise PcxError(_lib.ERR_SIZE_MISMATCH,
+                           f"vector count (which is {len(src_vecs)}) is not equal to fft size (which is {n})")
+        length = src_vecs[0].size()
+        if any(v.size() != length for v in list(src_vecs) + list(dst_vecs)):
+            raise PcxError(_lib.ERR_SIZE_MISMATCH, "vectors differ in size")
+        if len({v.pack for v in src_vecs}) != 1 or len({v.pack for v in dst_vecs}) != 1:
+            raise PcxError(_lib.ERR_INVALID_PACK, "vectors of one range share a pack size")
+        sp = torch.tensor([v.data_ptr() for v in src_vecs], dtype=torch.int64, device="cuda")
+        dp = torch.tensor([v.data_ptr() for v in dst_vecs], dtype=torch.int64, device="cuda")
+        check(lib().pcx_fft_across(self._h, int(inverse), int(normalized), dp.data_ptr(), dst_vecs[0].pack,
+                                   sp.data_ptr(), src_vecs[0].pack, length, _stream_ptr(stream)))
+        self._across_tables = (sp, dp)  # kept alive until the next call (asynchronous launch)
+
     def __call__(self, dst: vector, src: vector | None = None, stream=None) -> None:
         """operator()(v) in place, or operator()(dst, src) with src zero-extended (fft.hpp:728-820)."""
         if src is None:

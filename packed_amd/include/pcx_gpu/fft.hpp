@@ -280,6 +280,72 @@ private:
 
 // fxcorr_unit<float, Allocator, FFT_, TmpFactory_> (fxcorr.hpp:44-147) on the GPU.  FFT_ and
 // TmpFactory_ keep the reference's policy slots: the unit takes its FFT size from FFT_::size(), and
+// fft_unit_par<float, Order> (fft.hpp:2168-2572): FFT across a range of vectors, for every
+// element index k: dest[j][k] = sum_i source[i][k] W^(i j).  The reference's output order is wrong
+// (its test is disabled, test_fft.cpp:789); this one is natural.  The ranges hold device vectors
+// (pcx_gpu::vector or anything with data(), size() and pack_size); size() of a range is the
+// number of vectors and must equal fft size, as the reference requires.
+template<typename T, fft_order Order = fft_order::normal, typename Allocator = device_allocator<T>>
+    requires std::same_as<T, float>
+class fft_unit_par {
+public:
+    using real_type      = T;
+    using allocator_type = Allocator;
+
+    explicit fft_unit_par(uZ size, allocator_type = allocator_type{})
+    : m_size(size) {
+        pcx_plan_t p = nullptr;
+        throw_on(pcx_plan_create(static_cast<int64_t>(size), 2048, static_cast<int>(Order), &p));
+        m_plan = std::shared_ptr<pcx_plan_s>(p, [](pcx_plan_s* q) { (void)pcx_plan_destroy(q); });
+    }
+    [[nodiscard]] uZ size() const { return m_size; }
+    void             set_stream(void* stream) { m_stream = stream; }
+
+    template<typename DestR_, typename SrcR_>
+    void operator()(DestR_& dest, const SrcR_& source) {
+        if (dest.size() != m_size)
+            throw std::invalid_argument("destination size (which is " + std::to_string(dest.size()) +
+                                        " is not equal to fft size (which is " + std::to_string(m_size) + ")");
+        if (source.size() != m_size)
+            throw std::invalid_argument("source size (which is " + std::to_string(source.size()) +
+                                        " is not equal to fft size (which is " + std::to_string(m_size) + ")");
+        using DV = std::remove_cvref_t<decltype(*std::begin(dest))>;
+        using SV = std::remove_cvref_t<decltype(*std::begin(source))>;
+        const uZ len = std::begin(source)->size();
+        std::vector<const float*> sp;
+        std::vector<float*>       dp;
+        for (const auto& v : source) {
+            if (v.size() != len)
+                throw std::invalid_argument("vectors of the source range differ in size");
+            sp.push_back(v.data());
+        }
+        for (auto& v : dest) {
+            if (v.size() != len)
+                throw std::invalid_argument("vectors of the destination range differ in size");
+            dp.push_back(v.data());
+        }
+        const uZ bytes = m_size * sizeof(void*);
+        if (!m_tables) {
+            void* t = nullptr;
+            throw_on(pcx_malloc(&t, 2 * bytes));
+            m_tables = std::shared_ptr<void>(t, [](void* q) { (void)pcx_free(q); });
+        }
+        auto* tab = static_cast<char*>(m_tables.get());
+        throw_on(pcx_synchronize(m_stream)); // the tables of a previous call may still be in use
+        throw_on(pcx_memcpy_htod(tab, sp.data(), bytes));
+        throw_on(pcx_memcpy_htod(tab + bytes, dp.data(), bytes));
+        throw_on(pcx_fft_across(m_plan.get(), 0, 0, reinterpret_cast<float* const*>(tab + bytes), (int64_t)DV::pack_size,
+                                reinterpret_cast<const float* const*>(tab), (int64_t)SV::pack_size,
+                                static_cast<int64_t>(len), m_stream));
+    }
+
+private:
+    uZ                          m_size;
+    std::shared_ptr<pcx_plan_s> m_plan;
+    std::shared_ptr<void>       m_tables;
+    void*                       m_stream = nullptr;
+};
+
 // needs no caller scratch (the fused kernel keeps each block on chip), so TmpFactory_ is unused.
 struct no_tmp_factory {};
 

@@ -295,6 +295,45 @@ static void test_bin_ops(size_t length) {
     check("repack", [&](size_t i) { return l(i); });
 }
 
+// fft_unit_par: FFT across 16 vectors of 100 elements (pack 8), natural order, against the FP64
+// DFT along the vector index; and the reference's size check (fft.hpp:2198-2211).
+static void test_fft_par() {
+    using namespace pcx_gpu;
+    const size_t n = 16, len = 100;
+    std::vector<std::vector<cxf>>          host(n, std::vector<cxf>(len));
+    std::vector<vector<float, 8>>          src, dst;
+    for (size_t i = 0; i < n; ++i) {
+        for (size_t k = 0; k < len; ++k)
+            host[i][k] = cxf(std::sin(0.37f * (float)(i * len + k)), std::cos(0.11f * (float)(i + 3 * k)));
+        src.emplace_back(host[i]);
+        dst.emplace_back(len);
+    }
+    fft_unit_par<float> par(n);
+    par(dst, src);
+    double num = 0, den = 0;
+    for (size_t j = 0; j < n; ++j) {
+        auto got = dst[j].to_host();
+        for (size_t k = 0; k < len; ++k) {
+            cxd want = 0;
+            for (size_t i = 0; i < n; ++i)
+                want += cxd(host[i][k]) * std::polar(1.0, -2 * M_PI * (double)((i * j) % n) / (double)n);
+            num += std::norm(cxd(got[k]) - want);
+            den += std::norm(want);
+        }
+    }
+    double err = std::sqrt(num / den);
+    CHECK(err < 2 * 4 * std::numeric_limits<float>::epsilon(), "fft_unit_par rel L2 %g", err);
+    std::vector<vector<float, 8>> short_range;
+    short_range.emplace_back(len);
+    bool thrown = false;
+    try {
+        par(short_range, src);
+    } catch (const std::invalid_argument& e) {
+        thrown = std::string(e.what()).find("destination size (which is 1 is not equal to fft size") == 0;
+    }
+    CHECK(thrown, "fft_unit_par size check");
+}
+
 int main() {
     for (size_t n = 64; n <= 16384; n *= 2) { // test_fft.cpp:779-798 runs 2^6 .. 2^14
         test_fft_float<8>(n);
@@ -306,6 +345,7 @@ int main() {
     test_zero_extension();
     test_errors();
     test_fxcorr();
+    test_fft_par();
     for (size_t n = 1; n < 32; ++n) { // test_op.cpp:241-252
         test_bin_ops<1>(n);
         test_bin_ops<8>(n);
