@@ -87,6 +87,26 @@ int pcx_fft_batch(pcx_plan_t plan, int inverse, int normalized, float* dst, int6
                   int64_t dst_dist, const float* src, int64_t src_pack, int64_t src_dist,
                   int64_t src_size, int64_t batch, void* stream);
 
+/* ------------------------------------------------------------------ FFT unit, double
+ * Replaces fft_unit<double, Order, Allocator>::fft_unit(fft_size, sub_size, allocator)
+ * (fft.hpp:600-605 with T = double; default pack 4, types.hpp:70-72).  Same checks and messages
+ * as pcx_plan_create.  Orders normal and bit_reversed; fft_size <= 2^23.  The reference declares
+ * the double unit but has no double kernel for the unordered order (avx2_fft.hpp:114-217 is
+ * float-only), so order 2 is rejected with PCX_ERR_UNSUPPORTED. */
+int pcx_plan_create_f64(int64_t fft_size, int64_t sub_size, int order, pcx_plan_t* plan);
+/* Bytes per real element of a plan: 4 (pcx_plan_create) or 8 (pcx_plan_create_f64). */
+int pcx_plan_element_size(pcx_plan_t plan);
+/* pcx_fft / pcx_ifft / pcx_fft_batch for double data (fft.hpp:624-915 with T = double), with
+ * the same arguments in double elements.  A float plan passed here, or a double plan passed to
+ * the float functions, fails with PCX_ERR_INVALID_ARGUMENT. */
+int pcx_fft_f64(pcx_plan_t plan, double* dst, int64_t dst_pack, const double* src, int64_t src_pack,
+                int64_t src_size, void* stream);
+int pcx_ifft_f64(pcx_plan_t plan, int normalized, double* dst, int64_t dst_pack, const double* src,
+                 int64_t src_pack, void* stream);
+int pcx_fft_batch_f64(pcx_plan_t plan, int inverse, int normalized, double* dst, int64_t dst_pack,
+                      int64_t dst_dist, const double* src, int64_t src_pack, int64_t src_dist,
+                      int64_t src_size, int64_t batch, void* stream);
+
 /* FFT across a range of vectors: the transform of fft_unit_par (fft.hpp:2168-2572), whose
  * output order is wrong in the reference (its test is disabled, test_fft.cpp:789); here it is
  * natural.  For every k < length: dst_j[k] = sum_i src_i[k] * W^(i*j) (inverse: conjugate
@@ -205,6 +225,8 @@ int pcx_synchronize(void* stream);
 /* Seeded synthetic input written straight into HBM: splitmix64 outputs 2i and 2i+1 give
  * uniform [-1,1) re/im (dist 0) or Box-Muller normal(0, sigma^2) re/im (dist 1). */
 int pcx_generate(float* dst, int64_t pack, int64_t n, uint64_t seed, int dist, double sigma, void* stream);
+/* The same values for double data (not rounded to float). */
+int pcx_generate_f64(double* dst, int64_t pack, int64_t n, uint64_t seed, int dist, double sigma, void* stream);
 
 #ifdef __cplusplus
 }

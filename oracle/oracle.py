@@ -44,6 +44,7 @@ def lib():
         L.orc_unordered_q_network.restype = i64
         L.orc_slot_freq.argtypes = [C.c_int, i64, i64]
         L.orc_slot_freq.restype = i64
+        L.orc_slot_map.argtypes = [C.c_int, i64, C.POINTER(i64)]
         L.orc_pidx.argtypes = [i64, i64]
         L.orc_pidx.restype = i64
         L.orc_real_size.argtypes = [i64, i64]
@@ -140,7 +141,10 @@ def brev(i: int, bits: int) -> int:
 
 
 def slot_freq(order: str, n: int) -> np.ndarray:
-    return np.array([lib().orc_slot_freq(ORDERS[order], i, n) for i in range(n)], dtype=np.int64)
+    """Frequency held by each output slot (orc_slot_map: orc_slot_freq for every slot)."""
+    out = np.empty(n, np.int64)
+    lib().orc_slot_map(ORDERS[order], n, out.ctypes.data_as(C.POINTER(C.c_int64)))
+    return out
 
 
 def unordered_q(i: int) -> int:

@@ -161,6 +161,12 @@ int64_t orc_slot_freq(int order, int64_t i, int64_t n) {
     return i;
 }
 
+/* out[i] = orc_slot_freq(order, i, n) for every i < n */
+void orc_slot_map(int order, int64_t n, int64_t* out) {
+    for (int64_t i = 0; i < n; ++i)
+        out[i] = orc_slot_freq(order, i, n);
+}
+
 int64_t orc_pidx(int64_t i, int64_t pack) { return i + (i / pack) * pack; }
 
 int64_t orc_real_size(int64_t n, int64_t pack) { return ((n + pack - 1) / pack) * pack * 2; }

@@ -48,6 +48,7 @@ int64_t orc_unordered_q(int64_t i);
 int64_t orc_unordered_q_network(int64_t i);
 /* frequency index held by output slot i for order 0 normal / 1 bit_reversed / 2 unordered */
 int64_t orc_slot_freq(int order, int64_t i, int64_t n);
+void    orc_slot_map(int order, int64_t n, int64_t* out);
 /* element_access.hpp:8-11 pidx: float offset of the real part of complex i in pack P */
 int64_t orc_pidx(int64_t i, int64_t pack);
 /* floats occupied by n complex values in pack P (vector.hpp:373-378) */

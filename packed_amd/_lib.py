@@ -66,6 +66,11 @@ SIGNATURES = {
     "pcx_fft": (_int, [_vp, _vp, _i64, _vp, _i64, _i64, _vp]),
     "pcx_ifft": (_int, [_vp, _int, _vp, _i64, _vp, _i64, _vp]),
     "pcx_fft_batch": (_int, [_vp, _int, _int, _vp, _i64, _i64, _vp, _i64, _i64, _i64, _i64, _vp]),
+    "pcx_plan_create_f64": (_int, [_i64, _i64, _int, C.POINTER(_vp)]),
+    "pcx_plan_element_size": (_int, [_vp]),
+    "pcx_fft_f64": (_int, [_vp, _vp, _i64, _vp, _i64, _i64, _vp]),
+    "pcx_ifft_f64": (_int, [_vp, _int, _vp, _i64, _vp, _i64, _vp]),
+    "pcx_fft_batch_f64": (_int, [_vp, _int, _int, _vp, _i64, _i64, _vp, _i64, _i64, _i64, _i64, _vp]),
     "pcx_fxcorr_create": (_int, [_vp, _i64, _i64, _i64, _vp, C.POINTER(_vp)]),
     "pcx_fxcorr_destroy": (_int, [_vp]),
     "pcx_fxcorr_exec": (_int, [_vp, _vp, _i64, _vp, _i64, _i64, _vp]),
@@ -87,6 +92,7 @@ SIGNATURES = {
     "pcx_memset": (_int, [_vp, _int, _sz]),
     "pcx_synchronize": (_int, [_vp]),
     "pcx_generate": (_int, [_vp, _i64, _i64, _u64, _int, _dbl, _vp]),
+    "pcx_generate_f64": (_int, [_vp, _i64, _i64, _u64, _int, _dbl, _vp]),
     "pcx_vexpr_eval": (_int, [C.POINTER(VSource), C.c_int32, C.POINTER(VInstr), C.c_int32, _vp, _i64, _i64, _vp]),
     "pcx_repack": (_int, [_vp, _i64, _vp, _i64, _i64, _vp]),
     "pcx_fft_across": (_int, [_vp, _int, _int, _vp, _i64, _vp, _i64, _i64, _vp]),

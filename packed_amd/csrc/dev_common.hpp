@@ -8,21 +8,47 @@
 
 namespace pcxk {
 
-struct cf {
-    float x, y;
+// A complex value of element type T (float or double).
+template<class T>
+struct cx {
+    T x, y;
 };
+using cf = cx<float>;
+using cd = cx<double>;
 
-__device__ __forceinline__ cf cadd(cf a, cf b) { return {a.x + b.x, a.y + b.y}; }
-__device__ __forceinline__ cf csub(cf a, cf b) { return {a.x - b.x, a.y - b.y}; }
+__device__ __forceinline__ float  fmat(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fmat(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+template<class T>
+__device__ __forceinline__ cx<T> cadd(cx<T> a, cx<T> b) { return {a.x + b.x, a.y + b.y}; }
+template<class T>
+__device__ __forceinline__ cx<T> csub(cx<T> a, cx<T> b) { return {a.x - b.x, a.y - b.y}; }
 // a * w
-__device__ __forceinline__ cf cmul(cf a, cf w) {
-    return {__builtin_fmaf(a.x, w.x, -a.y * w.y), __builtin_fmaf(a.x, w.y, a.y * w.x)};
+template<class T>
+__device__ __forceinline__ cx<T> cmul(cx<T> a, cx<T> w) {
+    return {fmat(a.x, w.x, -a.y * w.y), fmat(a.x, w.y, a.y * w.x)};
 }
 // a * conj(w)
-__device__ __forceinline__ cf cmulc(cf a, cf w) {
-    return {__builtin_fmaf(a.x, w.x, a.y * w.y), __builtin_fmaf(a.y, w.x, -a.x * w.y)};
+template<class T>
+__device__ __forceinline__ cx<T> cmulc(cx<T> a, cx<T> w) {
+    return {fmat(a.x, w.x, a.y * w.y), fmat(a.y, w.x, -a.x * w.y)};
 }
-__device__ __forceinline__ cf cconj(cf a) { return {a.x, -a.y}; }
+template<class T>
+__device__ __forceinline__ cx<T> cconj(cx<T> a) { return {a.x, -a.y}; }
+
+// Two-component vector of T as stored in twiddle tables (float2 / double2).
+template<class T>
+struct vec2_of;
+template<>
+struct vec2_of<float> {
+    using type = float2;
+};
+template<>
+struct vec2_of<double> {
+    using type = double2;
+};
+template<class T>
+using vec2 = typename vec2_of<T>::type;
 
 // Compile-time loop.
 template<int I, int N, class F>
@@ -65,8 +91,8 @@ constexpr double cos32(int k) {
 constexpr double sin32(int k) { return cos32(k - 8); } // sin(a) = cos(a - pi/2)
 
 // z * exp(-2*pi*i*K/N) for N | 32, with the trivial factors done without multiplies.
-template<int N, int K>
-__device__ __forceinline__ cf twc(cf z) {
+template<int N, int K, class T>
+__device__ __forceinline__ cx<T> twc(cx<T> z) {
     constexpr int k = ((K % N) + N) % N;
     if constexpr (k == 0) {
         return z;
@@ -77,23 +103,23 @@ __device__ __forceinline__ cf twc(cf z) {
     } else if constexpr (4 * k == 3 * N) {
         return {-z.y, z.x};
     } else if constexpr (8 * k == N) {
-        constexpr float r = 0.70710678118654752440f;
+        constexpr T r = (T)0.70710678118654752440;
         return {(z.x + z.y) * r, (z.y - z.x) * r};
     } else if constexpr (8 * k == 3 * N) {
-        constexpr float r = 0.70710678118654752440f;
+        constexpr T r = (T)0.70710678118654752440;
         return {(z.y - z.x) * r, -(z.x + z.y) * r};
     } else {
-        constexpr int   k32 = k * (32 / N);
-        constexpr float c   = (float)cos32(k32);
-        constexpr float s   = (float)(-sin32(k32));
-        return cmul(z, cf{c, s});
+        constexpr int k32 = k * (32 / N);
+        constexpr T   c   = (T)cos32(k32);
+        constexpr T   s   = (T)(-sin32(k32));
+        return cmul(z, cx<T>{c, s});
     }
 }
 
 // In-register forward DFT of size R (power of two, R <= 32): natural order in, natural order out.
 // Radix-2 decimation in frequency with compile-time twiddles, then a register renaming.
-template<int R>
-__device__ __forceinline__ void dft(cf* v) {
+template<int R, class T>
+__device__ __forceinline__ void dft(cx<T>* v) {
     constexpr int L = ilog2c(R);
     sfor<0, L>([&](auto ss) {
         constexpr int s = decltype(ss)::value;
@@ -102,12 +128,12 @@ __device__ __forceinline__ void dft(cf* v) {
             constexpr int b = decltype(bb)::value;
             constexpr int g = (b / h) * 2 * h;
             constexpr int j = b % h;
-            cf a = v[g + j], c = v[g + j + h];
+            cx<T> a = v[g + j], c = v[g + j + h];
             v[g + j]     = cadd(a, c);
             v[g + j + h] = twc<2 * h, j>(csub(a, c));
         });
     });
-    cf t[R];
+    cx<T> t[R];
     sfor<0, R>([&](auto kk) {
         constexpr int k = decltype(kk)::value;
         t[k]            = v[brevc(k, L)];
@@ -137,7 +163,7 @@ __device__ __forceinline__ uint32_t slot_freq(int order, uint32_t i, int bits) {
 }
 
 // Packed layout (element_access.hpp:8-11): real part of complex i at i + (i / P) * P, imaginary
-// part P floats later.  lp = log2(P).
+// part P elements later.  lp = log2(P).
 __device__ __forceinline__ int64_t pidx(int64_t i, int lp) { return i + ((i >> lp) << lp); }
 
 } // namespace pcxk

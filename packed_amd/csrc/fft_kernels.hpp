@@ -19,9 +19,10 @@ namespace pcxk {
 enum : int { LD_CONTIG = 0, LD_PERM = 1, LD_STRIDED = 2 };
 enum : int { ST_CONTIG = 0, ST_PERM = 1, ST_STRIDED = 2 };
 
-struct RowArgs {
-    const float*  src;
-    float*        dst;
+template<class T>
+struct RowArgsT {
+    const T*      src;
+    T*            dst;
     int64_t       src_dist, dst_dist; // floats between consecutive transforms
     int64_t       src_size;           // elements >= src_size of a transform read as zero
     int64_t       nrows;              // rows in the launch (transforms << rows_lg)
@@ -29,34 +30,37 @@ struct RowArgs {
     int           rows_lg;            // log2(rows per transform): 0 single pass, log2(N2) four-step
     int           order;
     int           conj_in, conj_out;
-    float         scale;              // applied at the store
-    const float2* twA;                // W_A, A = row length
-    const float2* twl;                // post-FFT twiddle W_N^{a*row} = twl[e & lmask] * twh[e >> ls]
-    const float2* twh;
+    T             scale;              // applied at the store
+    const vec2<T>* twA;               // W_A, A = row length
+    const vec2<T>* twl;               // post-FFT twiddle W_N^{a*row} = twl[e & lmask] * twh[e >> ls]
+    const vec2<T>* twh;
     int           tw_ls, tw_lg;       // ls and log2(N)
     int           nt_load, nt_store;  // non-temporal (streaming) global loads / stores
-    const int64_t* src_offs;          // optional per-transform float offsets (instead of b * dist)
+    const int64_t* src_offs;          // optional per-transform element offsets (instead of b * dist)
     const int64_t* dst_offs;
 };
+using RowArgs = RowArgsT<float>;
 
-struct ColArgs {
-    const float*  src;
-    float*        dst;
+template<class T>
+struct ColArgsT {
+    const T*      src;
+    T*            dst;
     int64_t       src_dist, dst_dist;
     int64_t       src_size;
     int64_t       ntiles; // transforms * (N1 / C)
     int           src_lp, dst_lp;
     int           n1_lg;  // log2 of the row length N1 (the column stride)
     int           conj_in, conj_out;
-    float         scale;
-    const float2* twA;    // W_A, A = column length N2
-    const float2* twl;    // post-FFT twiddle W_N^{n1*a}
-    const float2* twh;
+    T             scale;
+    const vec2<T>* twA;   // W_A, A = column length N2
+    const vec2<T>* twl;   // post-FFT twiddle W_N^{n1*a}
+    const vec2<T>* twh;
     int           tw_ls, tw_lg;
     int           nt_load, nt_store;
     const int64_t* src_offs;
     const int64_t* dst_offs;
 };
+using ColArgs = ColArgsT<float>;
 
 // XCD-aware order of work items.  Workgroup w runs on XCD w mod 8 (round-robin placement); items
 // are dealt in rounds of gridDim.x, and within a round XCD x takes the contiguous range
@@ -71,11 +75,13 @@ __device__ __forceinline__ int64_t xcd_item(int64_t round_base) {
     return round_base + (int64_t)(blockIdx.x & 7u) * (g >> 3) + (blockIdx.x >> 3);
 }
 
-// Base of transform b: b * dist floats, or offs[b] when a per-transform offset table is given.
-__device__ __forceinline__ const float* tbase(const float* base, const int64_t* offs, int64_t dist, int64_t b) {
+// Base of transform b: b * dist elements, or offs[b] when a per-transform offset table is given.
+template<class T>
+__device__ __forceinline__ const T* tbase(const T* base, const int64_t* offs, int64_t dist, int64_t b) {
     return offs ? base + offs[b] : base + b * dist;
 }
-__device__ __forceinline__ float* tbase(float* base, const int64_t* offs, int64_t dist, int64_t b) {
+template<class T>
+__device__ __forceinline__ T* tbase(T* base, const int64_t* offs, int64_t dist, int64_t b) {
     return offs ? base + offs[b] : base + b * dist;
 }
 
@@ -87,8 +93,10 @@ __device__ __forceinline__ uint32_t poff(uint32_t i, int lp) {
 // poff(jt + c) = poff(jt) + poff(c) when c is a multiple of the thread count TA of a transform
 // and jt < TA (both powers of two, any power-of-two pack): the contiguous loads and stores of a
 // thread share one per-lane offset poff(jt) and add the uniform poff(c) as a scalar offset.
-__device__ __forceinline__ float ldf(const float* p, int nt) { return nt ? __builtin_nontemporal_load(p) : *p; }
-__device__ __forceinline__ void  stf(float* p, float v, int nt) {
+template<class T>
+__device__ __forceinline__ T ldf(const T* p, int nt) { return nt ? __builtin_nontemporal_load(p) : *p; }
+template<class T>
+__device__ __forceinline__ void stf(T* p, T v, int nt) {
     if (nt)
         __builtin_nontemporal_store(v, p);
     else
@@ -96,117 +104,136 @@ __device__ __forceinline__ void  stf(float* p, float v, int nt) {
 }
 
 // Re/im views of one packed vector.
+template<class T = float>
 struct CxPtr {
-    const float* re;
-    const float* im;
-    __device__ CxPtr(const float* base, int lp) : re(base), im(base + ((int64_t)1 << lp)) {}
-    __device__ cf ld(uint32_t o, int nt) const { return {ldf(re + o, nt), ldf(im + o, nt)}; }
-    __device__ cf ld(uint32_t vo, uint32_t so, int nt) const { return ld(vo + so, nt); }
+    const T* re;
+    const T* im;
+    __device__ CxPtr(const T* base, int lp) : re(base), im(base + ((int64_t)1 << lp)) {}
+    __device__ cx<T> ld(uint32_t o, int nt) const { return {ldf(re + o, nt), ldf(im + o, nt)}; }
+    __device__ cx<T> ld(uint32_t vo, uint32_t so, int nt) const { return ld(vo + so, nt); }
 };
+template<class T = float>
 struct CxPtrW {
-    float* re;
-    float* im;
-    __device__ CxPtrW(float* base, int lp) : re(base), im(base + ((int64_t)1 << lp)) {}
-    __device__ void st(uint32_t o, cf v, int nt) const {
+    T* re;
+    T* im;
+    __device__ CxPtrW(T* base, int lp) : re(base), im(base + ((int64_t)1 << lp)) {}
+    __device__ void st(uint32_t o, cx<T> v, int nt) const {
         stf(re + o, v.x, nt);
         stf(im + o, v.y, nt);
     }
-    __device__ void st(uint32_t vo, uint32_t so, cf v, int nt) const { st(vo + so, v, nt); }
+    __device__ void st(uint32_t vo, uint32_t so, cx<T> v, int nt) const { st(vo + so, v, nt); }
 };
 
 // Buffer-resource views (wave-uniform base in SGPRs, 32-bit byte offsets): one VGPR per address.
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(const float* base, uint32_t bytes = 0x7FFFFFFFu) {
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(base), 0, (int)bytes, 0x00020000);
+template<class T>
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(const T* base, uint32_t bytes = 0x7FFFFFFFu) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(base), 0, (int)bytes, 0x00020000);
 }
+// One element load / store through a buffer resource (byte offsets vb + sb).
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float buf_ld(float*, __amdgpu_buffer_rsrc_t r, uint32_t vb, uint32_t sb) {
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, vb, sb, 0));
+}
+__device__ __forceinline__ double buf_ld(double*, __amdgpu_buffer_rsrc_t r, uint32_t vb, uint32_t sb) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, vb, sb, 0));
+}
+__device__ __forceinline__ void buf_st(float v, __amdgpu_buffer_rsrc_t r, uint32_t vb, uint32_t sb) {
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, vb, sb, 0);
+}
+__device__ __forceinline__ void buf_st(double v, __amdgpu_buffer_rsrc_t r, uint32_t vb, uint32_t sb) {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, vb, sb, 0);
+}
+template<class T = float>
 struct BufR {
+    static constexpr uint32_t E = sizeof(T);
     __amdgpu_buffer_rsrc_t re, im;
-    __device__ BufR(const float* base, int lp) : re(rsrc(base)), im(rsrc(base + ((int64_t)1 << lp))) {}
-    // Loads at float offsets >= nfl return 0 (the buffer range check).  With nfl = poff(lim),
+    __device__ BufR(const T* base, int lp) : re(rsrc(base)), im(rsrc(base + ((int64_t)1 << lp))) {}
+    // Loads at element offsets >= nfl return 0 (the buffer range check).  With nfl = poff(lim),
     // exactly the elements i >= lim read as zero, in both views (poff is increasing).
-    __device__ BufR(const float* base, int lp, uint32_t nfl)
-        : re(rsrc(base, nfl * 4u)), im(rsrc(base + ((int64_t)1 << lp), nfl * 4u)) {}
-    __device__ cf ld(uint32_t o, int) const {
-        return {__uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(re, o * 4u, 0, 0)),
-                __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(im, o * 4u, 0, 0))};
+    __device__ BufR(const T* base, int lp, uint32_t nfl)
+        : re(rsrc(base, nfl * E)), im(rsrc(base + ((int64_t)1 << lp), nfl * E)) {}
+    __device__ cx<T> ld(uint32_t o, int) const {
+        return {buf_ld((T*)nullptr, re, o * E, 0), buf_ld((T*)nullptr, im, o * E, 0)};
     }
     // element at vo + so, so wave-uniform (scalar offset); the range is not checked against so
-    __device__ cf ld(uint32_t vo, uint32_t so, int) const {
-        return {__uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(re, vo * 4u, so * 4u, 0)),
-                __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(im, vo * 4u, so * 4u, 0))};
+    __device__ cx<T> ld(uint32_t vo, uint32_t so, int) const {
+        return {buf_ld((T*)nullptr, re, vo * E, so * E), buf_ld((T*)nullptr, im, vo * E, so * E)};
     }
 };
+template<class T = float>
 struct BufW {
+    static constexpr uint32_t E = sizeof(T);
     __amdgpu_buffer_rsrc_t re, im;
-    __device__ BufW(float* base, int lp) : re(rsrc(base)), im(rsrc(base + ((int64_t)1 << lp))) {}
-    // Stores at float offsets >= nfl are dropped by the range check (nfl = poff(lim): exactly the
-    // elements i >= lim)
-    __device__ BufW(float* base, int lp, uint32_t nfl)
-        : re(rsrc(base, nfl * 4u)), im(rsrc(base + ((int64_t)1 << lp), nfl * 4u)) {}
-    __device__ void st(uint32_t o, cf v, int) const {
-        __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v.x), re, o * 4u, 0, 0);
-        __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v.y), im, o * 4u, 0, 0);
+    __device__ BufW(T* base, int lp) : re(rsrc(base)), im(rsrc(base + ((int64_t)1 << lp))) {}
+    // Stores at element offsets >= nfl are dropped by the range check (nfl = poff(lim): exactly
+    // the elements i >= lim)
+    __device__ BufW(T* base, int lp, uint32_t nfl)
+        : re(rsrc(base, nfl * E)), im(rsrc(base + ((int64_t)1 << lp), nfl * E)) {}
+    __device__ void st(uint32_t o, cx<T> v, int) const {
+        buf_st(v.x, re, o * E, 0);
+        buf_st(v.y, im, o * E, 0);
     }
-    __device__ void st(uint32_t vo, uint32_t so, cf v, int) const {
-        __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v.x), re, vo * 4u, so * 4u, 0);
-        __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v.y), im, vo * 4u, so * 4u, 0);
+    __device__ void st(uint32_t vo, uint32_t so, cx<T> v, int) const {
+        buf_st(v.x, re, vo * E, so * E);
+        buf_st(v.y, im, vo * E, so * E);
     }
 };
-template<bool UNI>
-using RdView = typename std::conditional<UNI, BufR, CxPtr>::type;
-template<bool UNI>
-using WrView = typename std::conditional<UNI, BufW, CxPtrW>::type;
+template<bool UNI, class T = float>
+using RdView = typename std::conditional<UNI, BufR<T>, CxPtr<T>>::type;
+template<bool UNI, class T = float>
+using WrView = typename std::conditional<UNI, BufW<T>, CxPtrW<T>>::type;
 
 // LDS copy of the two-level W_N table for four-step pass twiddles, TWN complex entries
 // (lo + hi, TWN >= 4); TWN == 2 reads both levels from global memory instead.
-template<int TWN>
+template<int TWN, class T = float>
 struct PassTw {
     static constexpr bool in_lds = TWN > 2;
-    __device__ static void fill(float* t, const float2* __restrict__ twl, const float2* __restrict__ twh, int ls,
+    __device__ static void fill(T* t, const vec2<T>* __restrict__ twl, const vec2<T>* __restrict__ twh, int ls,
                                 int lg, int tid, int nthreads) {
         if constexpr (in_lds) {
             int nl = 1 << ls, nh = 1 << (lg - ls);
             for (int e = tid; e < nl; e += nthreads) {
-                float2 w  = twl[e];
+                vec2<T> w = twl[e];
                 t[e]      = w.x;
                 t[nl + e] = w.y;
             }
             for (int h = tid; h < nh; h += nthreads) {
-                float2 w           = twh[h];
+                vec2<T> w          = twh[h];
                 t[2 * nl + h]      = w.x;
                 t[2 * nl + nh + h] = w.y;
             }
         }
     }
-    __device__ static cf get(const float* t, const float2* __restrict__ twl, const float2* __restrict__ twh, int ls,
-                             int lg, uint32_t e) {
+    __device__ static cx<T> get(const T* t, const vec2<T>* __restrict__ twl, const vec2<T>* __restrict__ twh, int ls,
+                                int lg, uint32_t e) {
         uint32_t lo = e & ((1u << ls) - 1), hi = e >> ls;
         if constexpr (in_lds) {
             int nl = 1 << ls, nh = 1 << (lg - ls);
-            return cmul(cf{t[lo], t[nl + lo]}, cf{t[2 * nl + hi], t[2 * nl + nh + hi]});
+            return cmul(cx<T>{t[lo], t[nl + lo]}, cx<T>{t[2 * nl + hi], t[2 * nl + nh + hi]});
         } else {
-            float2 a = twl[lo], b = twh[hi];
-            return cmul(cf{a.x, a.y}, cf{b.x, b.y});
+            vec2<T> a = twl[lo], b = twh[hi];
+            return cmul(cx<T>{a.x, a.y}, cx<T>{b.x, b.y});
         }
     }
 };
 // pass-twiddle table capacity for a transform of 2^lg points (0: none needed)
 constexpr int pass_tw_entries(int lg) { return (1 << ((lg + 1) / 2)) + (1 << (lg / 2)); }
 
-template<int LA>
+template<int LA, class T = float>
 constexpr int default_rows() {
-    return Plan<LA>::TA >= 256 ? 1 : 256 / Plan<LA>::TA;
+    return Plan<LA, false, T>::TA >= 256 ? 1 : 256 / Plan<LA, false, T>::TA;
 }
 
-template<int LA, int G_>
+template<int LA, int G_, class T = float>
 struct RowShape {
-    static constexpr int TA = Plan<LA>::TA;
+    static constexpr int TA = Plan<LA, false, T>::TA;
     static constexpr int G  = G_; // rows per workgroup
     static constexpr int NT = G * TA;
     // odd row stride: rows of a strided access land in distinct banks
     static constexpr int STRIDE = (1 << LA) + ((1 << LA) >> 5) + 1;
     // re and im arrays, or one array shared by both halves of each exchange (Plan::HALF)
-    static constexpr int DATA   = (Plan<LA>::HALF ? 1 : 2) * G * STRIDE;
-    __device__ static float* im(float* lre) { return Plan<LA>::HALF ? lre : lre + DATA / 2; }
+    static constexpr int DATA   = (Plan<LA, false, T>::HALF ? 1 : 2) * G * STRIDE;
+    __device__ static T* im(T* lre) { return Plan<LA, false, T>::HALF ? lre : lre + DATA / 2; }
 };
 
 // Waves per SIMD that the LDS footprint allows (argument of __launch_bounds__).
@@ -226,28 +253,28 @@ constexpr int waves_per_simd(int threads, int lds_bytes) {
 // Element held by local slot s = jt + u*TA of a permuted (bit_reversed / unordered) row.  When a
 // thread's slots differ only in their high bits (TA >= 32: q touches the low five bits, which
 // belong to jt) this is base(jt) + brev(u), a compile-time offset from one address.
-template<int LA, int ORD>
+template<int LA, int ORD, class T = float>
 __device__ __forceinline__ int perm_base(int jt) {
-    using P           = Plan<LA>;
+    using P           = Plan<LA, false, T>;
     constexpr int LT  = ilog2c(P::TA);
     constexpr int LPP = ilog2c(P::PPT);
     uint32_t      j   = ORD == ORD_UNORD ? uq((uint32_t)jt) : (uint32_t)jt;
     return (int)(brev_bits(j, LT) << LPP);
 }
-template<int LA, int ORD>
+template<int LA, int ORD, class T = float>
 __device__ __forceinline__ int perm_elem(int jt, int u) {
-    using P = Plan<LA>;
+    using P = Plan<LA, false, T>;
     if constexpr (P::TA >= 32)
-        return perm_base<LA, ORD>(jt) + brevc(u, ilog2c(P::PPT));
+        return perm_base<LA, ORD, T>(jt) + brevc(u, ilog2c(P::PPT));
     else
         return (int)slot_freq(ORD, (uint32_t)(jt + u * P::TA), LA);
 }
 
-template<int LA, int G, int LD, int ST, int TW, bool UNI, int ORD, int PREC = PREC_FP32>
-__device__ __forceinline__ void row_group(const RowArgs& p, int64_t wg, float* lre, float* lim, const float* ltw,
-                                          const float* lptw, const MfmaDft16* mf = nullptr) {
-    using P           = Plan<LA>;
-    using S           = RowShape<LA, G>;
+template<int LA, int G, int LD, int ST, int TW, bool UNI, int ORD, int PREC = PREC_FP32, class T = float>
+__device__ __forceinline__ void row_group(const RowArgsT<T>& p, int64_t wg, T* lre, T* lim, const T* ltw,
+                                          const T* lptw, const MfmaDft16* mf = nullptr) {
+    using P           = Plan<LA, false, T>;
+    using S           = RowShape<LA, G, T>;
     constexpr int TA  = P::TA;
     constexpr int PPT = P::PPT;
     constexpr int NT  = S::NT;
@@ -269,10 +296,10 @@ __device__ __forceinline__ void row_group(const RowArgs& p, int64_t wg, float* l
     const int64_t  btu    = (UNI && TA < 64) ? (row0 >> p.rows_lg) : bt;
     auto           addr   = [&](int b, int o) { return g * STR + b + (b >> 5) + o + (o >> 5); };
 
-    cf v[PPT];
+    cx<T> v[PPT];
     // ------------------------------------------------------------------ load
     if constexpr (LD == LD_CONTIG) {
-        const RdView<UNI> src(tbase(p.src, p.src_offs, p.src_dist, btu), p.src_lp);
+        const RdView<UNI, T> src(tbase(p.src, p.src_offs, p.src_dist, btu), p.src_lp);
         if (row_ok && (int64_t)(k2 + 1) << LA <= p.src_size) { // whole row inside the source
             if constexpr (P::HALF) {
                 // one per-lane offset plus scalar offsets: the 2^14-point plans need the registers
@@ -281,7 +308,7 @@ __device__ __forceinline__ void row_group(const RowArgs& p, int64_t wg, float* l
                 for (int m = 0; m < PPT / R0; ++m)
 #pragma unroll
                     for (int r = 0; r < R0; ++r) {
-                        uint32_t c    = (uint32_t)(in_elem<LA>(jt, m, r) - jt) + (k2 << LA);
+                        uint32_t c    = (uint32_t)(in_elem<LA, false, T>(jt, m, r) - jt) + (k2 << LA);
                         v[m * R0 + r] = src.ld(vo, poff(c, p.src_lp), p.nt_load);
                     }
             } else {
@@ -289,7 +316,7 @@ __device__ __forceinline__ void row_group(const RowArgs& p, int64_t wg, float* l
                 for (int m = 0; m < PPT / R0; ++m)
 #pragma unroll
                     for (int r = 0; r < R0; ++r) {
-                        uint32_t i    = (uint32_t)in_elem<LA>(jt, m, r) + (k2 << LA);
+                        uint32_t i    = (uint32_t)in_elem<LA, false, T>(jt, m, r) + (k2 << LA);
                         v[m * R0 + r] = src.ld(poff(i, p.src_lp), p.nt_load);
                     }
             }
@@ -298,10 +325,10 @@ __device__ __forceinline__ void row_group(const RowArgs& p, int64_t wg, float* l
             for (int m = 0; m < PPT / R0; ++m)
 #pragma unroll
                 for (int r = 0; r < R0; ++r) {
-                    uint32_t i    = (uint32_t)in_elem<LA>(jt, m, r) + (k2 << LA);
+                    uint32_t i    = (uint32_t)in_elem<LA, false, T>(jt, m, r) + (k2 << LA);
                     bool     ok   = row_ok && i < p.src_size;
-                    cf       x    = src.ld(ok ? poff(i, p.src_lp) : 0u, p.nt_load);
-                    v[m * R0 + r] = ok ? x : cf{0.f, 0.f};
+                    cx<T>    x    = src.ld(ok ? poff(i, p.src_lp) : 0u, p.nt_load);
+                    v[m * R0 + r] = ok ? x : cx<T>{T(0), T(0)};
                 }
         }
     } else if constexpr (P::HALF) {
@@ -310,9 +337,9 @@ __device__ __forceinline__ void row_group(const RowArgs& p, int64_t wg, float* l
         auto pos = [&](int u) {
             if constexpr (LD == LD_PERM) {
                 if constexpr (TA >= 32)
-                    return addr(perm_base<LA, ORD>(jt), brevc(u, ilog2c(PPT)));
+                    return addr(perm_base<LA, ORD, T>(jt), brevc(u, ilog2c(PPT)));
                 else
-                    return addr(perm_elem<LA, ORD>(jt, u), 0);
+                    return addr(perm_elem<LA, ORD, T>(jt, u), 0);
             } else {
                 int idx = t + u * NT;
                 int k1  = idx / G;
@@ -320,24 +347,24 @@ __device__ __forceinline__ void row_group(const RowArgs& p, int64_t wg, float* l
             }
         };
         if constexpr (LD == LD_PERM) {
-            const RdView<UNI> src(tbase(p.src, p.src_offs, p.src_dist, btu), p.src_lp);
+            const RdView<UNI, T> src(tbase(p.src, p.src_offs, p.src_dist, btu), p.src_lp);
             uint32_t          base = brev_bits(k2, p.rows_lg) << LA;
 #pragma unroll
             for (int u = 0; u < PPT; ++u) {
-                cf x = src.ld(row_ok ? poff(base + jt + u * TA, p.src_lp) : 0u, p.nt_load);
-                v[u] = row_ok ? x : cf{0.f, 0.f};
+                cx<T> x = src.ld(row_ok ? poff(base + jt + u * TA, p.src_lp) : 0u, p.nt_load);
+                v[u] = row_ok ? x : cx<T>{T(0), T(0)};
             }
         } else {
-            const RdView<UNI> src(tbase(p.src, p.src_offs, p.src_dist, row0 >> p.rows_lg), p.src_lp);
+            const RdView<UNI, T> src(tbase(p.src, p.src_offs, p.src_dist, row0 >> p.rows_lg), p.src_lp);
 #pragma unroll
             for (int u = 0; u < PPT; ++u) {
                 int     idx = t + u * NT;
                 int64_t rw  = row0 + idx % G;
                 bool    ok  = rw < p.nrows;
-                cf      x   = src.ld(ok ? poff((uint32_t)(rw & rmask) + ((uint32_t)(idx / G) << p.rows_lg), p.src_lp)
+                cx<T>   x   = src.ld(ok ? poff((uint32_t)(rw & rmask) + ((uint32_t)(idx / G) << p.rows_lg), p.src_lp)
                                         : 0u,
                                      p.nt_load);
-                v[u]        = ok ? x : cf{0.f, 0.f};
+                v[u]        = ok ? x : cx<T>{T(0), T(0)};
             }
         }
 #pragma unroll
@@ -355,24 +382,24 @@ __device__ __forceinline__ void row_group(const RowArgs& p, int64_t wg, float* l
         if constexpr (LD == LD_PERM) {
             // this row's spectrum occupies slots [A*brev(k2), A*brev(k2) + A); local slot s holds
             // element slot_freq(order, s)
-            const RdView<UNI> src(tbase(p.src, p.src_offs, p.src_dist, btu), p.src_lp);
+            const RdView<UNI, T> src(tbase(p.src, p.src_offs, p.src_dist, btu), p.src_lp);
             uint32_t          base = brev_bits(k2, p.rows_lg) << LA;
 #pragma unroll
             for (int u = 0; u < PPT; ++u) {
-                int s  = jt + u * TA;
-                cf  x  = src.ld(row_ok ? poff(base + s, p.src_lp) : 0u, p.nt_load);
-                x      = row_ok ? x : cf{0.f, 0.f};
+                int   s  = jt + u * TA;
+                cx<T> x  = src.ld(row_ok ? poff(base + s, p.src_lp) : 0u, p.nt_load);
+                x      = row_ok ? x : cx<T>{T(0), T(0)};
                 int o;
                 if constexpr (TA >= 32)
-                    o = addr(perm_base<LA, ORD>(jt), brevc(u, ilog2c(PPT)));
+                    o = addr(perm_base<LA, ORD, T>(jt), brevc(u, ilog2c(PPT)));
                 else
-                    o = addr(perm_elem<LA, ORD>(jt, u), 0);
+                    o = addr(perm_elem<LA, ORD, T>(jt, u), 0);
                 lre[o] = x.x;
                 lim[o] = x.y;
             }
         } else { // LD_STRIDED: element k1 of row k2 sits at slot k2 + (k1 << rows_lg)
             // (the rows of a workgroup lie in one transform: G divides the rows per transform)
-            const RdView<UNI> src(tbase(p.src, p.src_offs, p.src_dist, row0 >> p.rows_lg), p.src_lp);
+            const RdView<UNI, T> src(tbase(p.src, p.src_offs, p.src_dist, row0 >> p.rows_lg), p.src_lp);
 #pragma unroll
             for (int u = 0; u < PPT; ++u) {
                 int     idx = t + u * NT;
@@ -380,9 +407,9 @@ __device__ __forceinline__ void row_group(const RowArgs& p, int64_t wg, float* l
                 int     k1  = idx / G;
                 int64_t rw  = row0 + gg;
                 bool    ok  = rw < p.nrows;
-                cf      x   = src.ld(ok ? poff((uint32_t)(rw & rmask) + ((uint32_t)k1 << p.rows_lg), p.src_lp) : 0u,
+                cx<T>   x   = src.ld(ok ? poff((uint32_t)(rw & rmask) + ((uint32_t)k1 << p.rows_lg), p.src_lp) : 0u,
                                      p.nt_load);
-                x           = ok ? x : cf{0.f, 0.f};
+                x           = ok ? x : cx<T>{T(0), T(0)};
                 int o  = gg * STR + k1 + (k1 >> 5);
                 lre[o] = x.x;
                 lim[o] = x.y;
@@ -394,7 +421,7 @@ __device__ __forceinline__ void row_group(const RowArgs& p, int64_t wg, float* l
 #pragma unroll
             for (int r = 0; r < R0; ++r) {
                 int o         = addr(jt + m * TA, r * (P::A / R0));
-                v[m * R0 + r] = cf{lre[o], lim[o]};
+                v[m * R0 + r] = cx<T>{lre[o], lim[o]};
             }
     }
     if (p.conj_in) {
@@ -404,19 +431,19 @@ __device__ __forceinline__ void row_group(const RowArgs& p, int64_t wg, float* l
     }
     // ------------------------------------------------------------------ transform
     // (the barrier before the first stage's LDS writes also orders them after the reads above)
-    stockham<LA, false, decltype(addr), PREC>(v, lre, lim, jt, addr, ltw, mf);
+    stockham<LA, false, decltype(addr), PREC, T>(v, lre, lim, jt, addr, ltw, mf);
 
     if constexpr (TW > 0) {
 #pragma unroll
         for (int m = 0; m < PPT / RL; ++m)
 #pragma unroll
             for (int r = 0; r < RL; ++r) {
-                uint32_t e    = (uint32_t)out_elem<LA>(jt, m, r) * k2;
-                v[m * RL + r] = cmul(v[m * RL + r], PassTw<TW>::get(lptw, p.twl, p.twh, p.tw_ls, p.tw_lg, e));
+                uint32_t e    = (uint32_t)out_elem<LA, false, T>(jt, m, r) * k2;
+                v[m * RL + r] = cmul(v[m * RL + r], PassTw<TW, T>::get(lptw, p.twl, p.twh, p.tw_ls, p.tw_lg, e));
                 __builtin_amdgcn_sched_barrier(0); // one element's table reads in flight at a time
             }
     }
-    if (p.scale != 1.0f) {
+    if (p.scale != T(1)) {
 #pragma unroll
         for (int q = 0; q < PPT; ++q) {
             v[q].x *= p.scale;
@@ -431,14 +458,14 @@ __device__ __forceinline__ void row_group(const RowArgs& p, int64_t wg, float* l
     // ------------------------------------------------------------------ store
     if constexpr (ST == ST_CONTIG) {
         if (row_ok) {
-            const WrView<UNI> dst(tbase(p.dst, p.dst_offs, p.dst_dist, btu), p.dst_lp);
+            const WrView<UNI, T> dst(tbase(p.dst, p.dst_offs, p.dst_dist, btu), p.dst_lp);
             if constexpr (P::HALF) {
                 const uint32_t vo = poff((uint32_t)jt, p.dst_lp);
 #pragma unroll
                 for (int m = 0; m < PPT / RL; ++m)
 #pragma unroll
                     for (int r = 0; r < RL; ++r) {
-                        uint32_t c = (uint32_t)(out_elem<LA>(jt, m, r) - jt) + (k2 << LA);
+                        uint32_t c = (uint32_t)(out_elem<LA, false, T>(jt, m, r) - jt) + (k2 << LA);
                         dst.st(vo, poff(c, p.dst_lp), v[m * RL + r], p.nt_store);
                     }
             } else {
@@ -446,7 +473,7 @@ __device__ __forceinline__ void row_group(const RowArgs& p, int64_t wg, float* l
                 for (int m = 0; m < PPT / RL; ++m)
 #pragma unroll
                     for (int r = 0; r < RL; ++r) {
-                        uint32_t i = (uint32_t)out_elem<LA>(jt, m, r) + (k2 << LA);
+                        uint32_t i = (uint32_t)out_elem<LA, false, T>(jt, m, r) + (k2 << LA);
                         dst.st(poff(i, p.dst_lp), v[m * RL + r], p.nt_store);
                     }
             }
@@ -456,9 +483,9 @@ __device__ __forceinline__ void row_group(const RowArgs& p, int64_t wg, float* l
         auto pos = [&](int u) {
             if constexpr (ST == ST_PERM) {
                 if constexpr (TA >= 32)
-                    return addr(perm_base<LA, ORD>(jt), brevc(u, ilog2c(PPT)));
+                    return addr(perm_base<LA, ORD, T>(jt), brevc(u, ilog2c(PPT)));
                 else
-                    return addr(perm_elem<LA, ORD>(jt, u), 0);
+                    return addr(perm_elem<LA, ORD, T>(jt, u), 0);
             } else {
                 int idx = t + u * NT;
                 int k1  = idx / G;
@@ -486,14 +513,14 @@ __device__ __forceinline__ void row_group(const RowArgs& p, int64_t wg, float* l
             v[u].y = lre[pos(u)];
         if constexpr (ST == ST_PERM) {
             if (row_ok) {
-                const WrView<UNI> dst(tbase(p.dst, p.dst_offs, p.dst_dist, btu), p.dst_lp);
+                const WrView<UNI, T> dst(tbase(p.dst, p.dst_offs, p.dst_dist, btu), p.dst_lp);
                 uint32_t          base = brev_bits(k2, p.rows_lg) << LA;
 #pragma unroll
                 for (int u = 0; u < PPT; ++u)
                     dst.st(poff(base + jt + u * TA, p.dst_lp), v[u], p.nt_store);
             }
         } else {
-            const WrView<UNI> dst(tbase(p.dst, p.dst_offs, p.dst_dist, row0 >> p.rows_lg), p.dst_lp);
+            const WrView<UNI, T> dst(tbase(p.dst, p.dst_offs, p.dst_dist, row0 >> p.rows_lg), p.dst_lp);
 #pragma unroll
             for (int u = 0; u < PPT; ++u) {
                 int     idx = t + u * NT;
@@ -516,21 +543,21 @@ __device__ __forceinline__ void row_group(const RowArgs& p, int64_t wg, float* l
         __syncthreads();
         if constexpr (ST == ST_PERM) {
             if (row_ok) {
-                const WrView<UNI> dst(tbase(p.dst, p.dst_offs, p.dst_dist, btu), p.dst_lp);
+                const WrView<UNI, T> dst(tbase(p.dst, p.dst_offs, p.dst_dist, btu), p.dst_lp);
                 uint32_t          base = brev_bits(k2, p.rows_lg) << LA;
 #pragma unroll
                 for (int u = 0; u < PPT; ++u) {
                     int s = jt + u * TA;
                     int o;
                     if constexpr (TA >= 32)
-                        o = addr(perm_base<LA, ORD>(jt), brevc(u, ilog2c(PPT)));
+                        o = addr(perm_base<LA, ORD, T>(jt), brevc(u, ilog2c(PPT)));
                     else
-                        o = addr(perm_elem<LA, ORD>(jt, u), 0);
-                    dst.st(poff(base + s, p.dst_lp), cf{lre[o], lim[o]}, p.nt_store);
+                        o = addr(perm_elem<LA, ORD, T>(jt, u), 0);
+                    dst.st(poff(base + s, p.dst_lp), cx<T>{lre[o], lim[o]}, p.nt_store);
                 }
             }
         } else { // ST_STRIDED: element k1 of row k2 goes to slot k2 + (k1 << rows_lg)
-            const WrView<UNI> dst(tbase(p.dst, p.dst_offs, p.dst_dist, row0 >> p.rows_lg), p.dst_lp);
+            const WrView<UNI, T> dst(tbase(p.dst, p.dst_offs, p.dst_dist, row0 >> p.rows_lg), p.dst_lp);
 #pragma unroll
             for (int u = 0; u < PPT; ++u) {
                 int     idx = t + u * NT;
@@ -539,7 +566,7 @@ __device__ __forceinline__ void row_group(const RowArgs& p, int64_t wg, float* l
                 int64_t rw  = row0 + gg;
                 if (rw < p.nrows) {
                     int o = gg * STR + k1 + (k1 >> 5);
-                    dst.st(poff((uint32_t)(rw & rmask) + ((uint32_t)k1 << p.rows_lg), p.dst_lp), cf{lre[o], lim[o]},
+                    dst.st(poff((uint32_t)(rw & rmask) + ((uint32_t)k1 << p.rows_lg), p.dst_lp), cx<T>{lre[o], lim[o]},
                            p.nt_store);
                 }
             }
@@ -547,9 +574,10 @@ __device__ __forceinline__ void row_group(const RowArgs& p, int64_t wg, float* l
     }
 }
 
-template<int LA, int G, int TW>
+// LDS elements (of T) of a row workgroup
+template<int LA, int G, int TW, class T = float>
 constexpr int row_lds_floats() {
-    return RowShape<LA, G>::DATA + TwTable<LA>::FLOATS + 2 * TW;
+    return RowShape<LA, G, T>::DATA + TwTable<LA, T>::FLOATS + 2 * TW;
 }
 
 // Grid-stride over row groups: gridDim.x may be smaller than the number of groups.
@@ -557,19 +585,21 @@ constexpr int row_lds_floats() {
 // but at least two, so the register budget (256) never pushes accumulators into AGPRs.
 constexpr int mfma_waves(int w, int prec) { return prec == PREC_FP32 ? w : (w > 4 ? w / 2 : 2); }
 
-template<int LA, int G, int LD, int ST, int TW, bool UNI, int ORD, int PREC = PREC_FP32>
-__global__ __launch_bounds__(G * Plan<LA>::TA,
-                             mfma_waves(waves_per_simd(G * Plan<LA>::TA, 4 * row_lds_floats<LA, G, TW>()), PREC))
-void k_row(RowArgs p) {
-    constexpr int    NT = G * Plan<LA>::TA;
-    __shared__ float lds[row_lds_floats<LA, G, TW>()];
-    float*           lre  = lds;
-    float*           lim  = RowShape<LA, G>::im(lds);
-    float*           ltw  = lds + RowShape<LA, G>::DATA;
-    float*           lptw = ltw + TwTable<LA>::FLOATS;
-    TwTable<LA>::fill(ltw, p.twA, threadIdx.x, NT);
+template<int LA, int G, int LD, int ST, int TW, bool UNI, int ORD, int PREC = PREC_FP32, class T = float>
+__global__ __launch_bounds__((G * Plan<LA, false, T>::TA),
+                             mfma_waves(waves_per_simd(G * Plan<LA, false, T>::TA,
+                                                       (int)sizeof(T) * row_lds_floats<LA, G, TW, T>()),
+                                        PREC))
+void k_row(RowArgsT<T> p) {
+    constexpr int NT = G * Plan<LA, false, T>::TA;
+    __shared__ T  lds[row_lds_floats<LA, G, TW, T>()];
+    T*            lre  = lds;
+    T*            lim  = RowShape<LA, G, T>::im(lds);
+    T*            ltw  = lds + RowShape<LA, G, T>::DATA;
+    T*            lptw = ltw + TwTable<LA, T>::FLOATS;
+    TwTable<LA, T>::fill(ltw, p.twA, threadIdx.x, NT);
     if constexpr (TW > 0)
-        PassTw<TW>::fill(lptw, p.twl, p.twh, p.tw_ls, p.tw_lg, threadIdx.x, NT);
+        PassTw<TW, T>::fill(lptw, p.twl, p.twh, p.tw_ls, p.tw_lg, threadIdx.x, NT);
     __syncthreads();
     MfmaDft16 mf;
     if constexpr (PREC == PREC_BF16X3)
@@ -579,7 +609,7 @@ void k_row(RowArgs p) {
         const int64_t wg = xcd_item(base);
         if (wg >= nwg)
             break; // uniform over the workgroup; later rounds have no items for it either
-        row_group<LA, G, LD, ST, TW, UNI, ORD, PREC>(p, wg, lre, lim, ltw, lptw, &mf);
+        row_group<LA, G, LD, ST, TW, UNI, ORD, PREC, T>(p, wg, lre, lim, ltw, lptw, &mf);
         __syncthreads(); // LDS is reused by the next group
     }
 }
@@ -655,7 +685,7 @@ void k_dif(RowArgs p) {
             // segment n1 starts at poff(n1*M) (poff(n2 + n1*M) = poff(n2) + poff(n1*M) for every
             // power-of-two pack); elements past src_size read as zero through the range check
             const uint32_t nseg = lim > s0 ? (lim - s0 < (uint32_t)M ? lim - s0 : (uint32_t)M) : 0u;
-            const BufR     seg(tb + poff(s0, p.src_lp), p.src_lp, poff(nseg, p.src_lp));
+            const BufR<>   seg(tb + poff(s0, p.src_lp), p.src_lp, poff(nseg, p.src_lp));
             // the per-lane offset of segment n1 > 0 consumes the sums so far (an opaque zero is
             // added), so one segment's loads are in registers at a time beside v
             int jv = jt;
@@ -696,7 +726,7 @@ void k_dif(RowArgs p) {
         v[q].y *= so;
     }
     // ------------------------------------------------------------------ store
-    const BufW dst(tbase(p.dst, p.dst_offs, p.dst_dist, b), p.dst_lp);
+    const BufW<> dst(tbase(p.dst, p.dst_offs, p.dst_dist, b), p.dst_lp);
     if constexpr (ORD == ORD_NORMAL) {
         // element (jt + c) * R + k1 with c a multiple of TA: poff of it is
         // poff(jt * R + k1) + poff(c * R) (the same splitting at the multiple R * TA)
@@ -732,10 +762,10 @@ void k_dif(RowArgs p) {
     }
 }
 
-template<int LA, int C, int TW, int PREC = PREC_FP32>
-__device__ __forceinline__ void col_tile(const ColArgs& p, int64_t tile, float* lre, float* lim, const float* ltw,
-                                         const float* lptw, const MfmaDft16* mf = nullptr) {
-    using P           = Plan<LA>;
+template<int LA, int C, int TW, int PREC = PREC_FP32, class T = float>
+__device__ __forceinline__ void col_tile(const ColArgsT<T>& p, int64_t tile, T* lre, T* lim, const T* ltw,
+                                         const T* lptw, const MfmaDft16* mf = nullptr) {
+    using P           = Plan<LA, false, T>;
     constexpr int PPT = P::PPT;
     constexpr int R0  = P::first_radix;
     constexpr int RL  = P::last_radix;
@@ -747,16 +777,16 @@ __device__ __forceinline__ void col_tile(const ColArgs& p, int64_t tile, float* 
     const int64_t  bt   = tile >> ctl;
     const uint32_t n1   = (uint32_t)((tile & (((int64_t)1 << ctl) - 1)) * C) + c;
     auto           addr = [&](int b, int o) { return (b + o) * C + c; };
-    const BufR     src(tbase(p.src, p.src_offs, p.src_dist, bt), p.src_lp);
-    const BufW     dst(tbase(p.dst, p.dst_offs, p.dst_dist, bt), p.dst_lp);
+    const BufR<T>  src(tbase(p.src, p.src_offs, p.src_dist, bt), p.src_lp);
+    const BufW<T>  dst(tbase(p.dst, p.dst_offs, p.dst_dist, bt), p.dst_lp);
 
-    cf v[PPT];
+    cx<T> v[PPT];
     if (((int64_t)1 << (p.n1_lg + LA)) <= p.src_size) { // the whole transform is inside the source
 #pragma unroll
         for (int m = 0; m < PPT / R0; ++m)
 #pragma unroll
             for (int r = 0; r < R0; ++r) {
-                uint32_t i    = n1 + ((uint32_t)in_elem<LA>(jt, m, r) << p.n1_lg);
+                uint32_t i    = n1 + ((uint32_t)in_elem<LA, false, T>(jt, m, r) << p.n1_lg);
                 v[m * R0 + r] = src.ld(poff(i, p.src_lp), p.nt_load);
             }
     } else {
@@ -764,10 +794,10 @@ __device__ __forceinline__ void col_tile(const ColArgs& p, int64_t tile, float* 
         for (int m = 0; m < PPT / R0; ++m)
 #pragma unroll
             for (int r = 0; r < R0; ++r) {
-                uint32_t i    = n1 + ((uint32_t)in_elem<LA>(jt, m, r) << p.n1_lg);
+                uint32_t i    = n1 + ((uint32_t)in_elem<LA, false, T>(jt, m, r) << p.n1_lg);
                 bool     ok   = i < p.src_size;
-                cf       x    = src.ld(ok ? poff(i, p.src_lp) : 0u, p.nt_load);
-                v[m * R0 + r] = ok ? x : cf{0.f, 0.f};
+                cx<T>    x    = src.ld(ok ? poff(i, p.src_lp) : 0u, p.nt_load);
+                v[m * R0 + r] = ok ? x : cx<T>{T(0), T(0)};
             }
     }
     if (p.conj_in) {
@@ -775,18 +805,18 @@ __device__ __forceinline__ void col_tile(const ColArgs& p, int64_t tile, float* 
         for (int q = 0; q < PPT; ++q)
             v[q].y = -v[q].y;
     }
-    stockham<LA, false, decltype(addr), PREC>(v, lre, lim, jt, addr, ltw, mf);
+    stockham<LA, false, decltype(addr), PREC, T>(v, lre, lim, jt, addr, ltw, mf);
     if constexpr (TW > 0) {
 #pragma unroll
         for (int m = 0; m < PPT / RL; ++m)
 #pragma unroll
             for (int r = 0; r < RL; ++r) {
-                uint32_t e    = n1 * (uint32_t)out_elem<LA>(jt, m, r);
-                v[m * RL + r] = cmul(v[m * RL + r], PassTw<TW>::get(lptw, p.twl, p.twh, p.tw_ls, p.tw_lg, e));
+                uint32_t e    = n1 * (uint32_t)out_elem<LA, false, T>(jt, m, r);
+                v[m * RL + r] = cmul(v[m * RL + r], PassTw<TW, T>::get(lptw, p.twl, p.twh, p.tw_ls, p.tw_lg, e));
                 __builtin_amdgcn_sched_barrier(0); // one element's table reads in flight at a time
             }
     }
-    if (p.scale != 1.0f) {
+    if (p.scale != T(1)) {
 #pragma unroll
         for (int q = 0; q < PPT; ++q) {
             v[q].x *= p.scale;
@@ -802,31 +832,33 @@ __device__ __forceinline__ void col_tile(const ColArgs& p, int64_t tile, float* 
     for (int m = 0; m < PPT / RL; ++m)
 #pragma unroll
         for (int r = 0; r < RL; ++r) {
-            uint32_t i = n1 + ((uint32_t)out_elem<LA>(jt, m, r) << p.n1_lg);
+            uint32_t i = n1 + ((uint32_t)out_elem<LA, false, T>(jt, m, r) << p.n1_lg);
             dst.st(poff(i, p.dst_lp), v[m * RL + r], p.nt_store);
         }
 }
 
-template<int LA, int C, int TW>
+template<int LA, int C, int TW, class T = float>
 constexpr int col_lds_floats() {
-    return 2 * (1 << LA) * C + TwTable<LA>::FLOATS + 2 * TW;
+    return 2 * (1 << LA) * C + TwTable<LA, T>::FLOATS + 2 * TW;
 }
 
-template<int LA, int C, int TW, int PREC = PREC_FP32>
-__global__ __launch_bounds__(C * Plan<LA>::TA,
-                             mfma_waves(waves_per_simd(C * Plan<LA>::TA, 4 * col_lds_floats<LA, C, TW>()), PREC))
-void k_col(ColArgs p) {
-    static_assert(!Plan<LA>::HALF, "column tiles keep separate re / im arrays");
-    constexpr int    NT   = C * Plan<LA>::TA;
-    constexpr int    DATA = 2 * (1 << LA) * C;
-    __shared__ float lds[col_lds_floats<LA, C, TW>()];
-    float*           lre  = lds;
-    float*           lim  = lds + DATA / 2;
-    float*           ltw  = lds + DATA;
-    float*           lptw = ltw + TwTable<LA>::FLOATS;
-    TwTable<LA>::fill(ltw, p.twA, threadIdx.x, NT);
+template<int LA, int C, int TW, int PREC = PREC_FP32, class T = float>
+__global__ __launch_bounds__((C * Plan<LA, false, T>::TA),
+                             mfma_waves(waves_per_simd(C * Plan<LA, false, T>::TA,
+                                                       (int)sizeof(T) * col_lds_floats<LA, C, TW, T>()),
+                                        PREC))
+void k_col(ColArgsT<T> p) {
+    static_assert(!Plan<LA, false, T>::HALF, "column tiles keep separate re / im arrays");
+    constexpr int NT   = C * Plan<LA, false, T>::TA;
+    constexpr int DATA = 2 * (1 << LA) * C;
+    __shared__ T  lds[col_lds_floats<LA, C, TW, T>()];
+    T*            lre  = lds;
+    T*            lim  = lds + DATA / 2;
+    T*            ltw  = lds + DATA;
+    T*            lptw = ltw + TwTable<LA, T>::FLOATS;
+    TwTable<LA, T>::fill(ltw, p.twA, threadIdx.x, NT);
     if constexpr (TW > 0)
-        PassTw<TW>::fill(lptw, p.twl, p.twh, p.tw_ls, p.tw_lg, threadIdx.x, NT);
+        PassTw<TW, T>::fill(lptw, p.twl, p.twh, p.tw_ls, p.tw_lg, threadIdx.x, NT);
     __syncthreads();
     MfmaDft16 mf;
     if constexpr (PREC == PREC_BF16X3)
@@ -835,7 +867,7 @@ void k_col(ColArgs p) {
         const int64_t tile = xcd_item(base);
         if (tile >= p.ntiles)
             break; // uniform over the workgroup; later rounds have no items for it either
-        col_tile<LA, C, TW, PREC>(p, tile, lre, lim, ltw, lptw, &mf);
+        col_tile<LA, C, TW, PREC, T>(p, tile, lre, lim, ltw, lptw, &mf);
         __syncthreads(); // LDS is reused by the next tile
     }
 }
@@ -915,14 +947,15 @@ __global__ __launch_bounds__(C * Plan<LA>::TA) void k_across(AcrossArgs p) {
     }
 }
 
-__global__ void k_twiddles(float2* t, int64_t M, int64_t stride, int64_t count) {
+template<class T>
+__global__ void k_twiddles(vec2<T>* t, int64_t M, int64_t stride, int64_t count) {
     int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= count)
         return;
     int64_t k = (e * stride) % M;
     double  s, c;
     sincospi(-2.0 * (double)k / (double)M, &s, &c);
-    t[e] = make_float2((float)c, (float)s);
+    t[e] = vec2<T>{(T)c, (T)s};
 }
 
 // Seeded inputs: output k of splitmix64(seed) is mix(seed + (k+1)*gamma), so element i uses
@@ -932,23 +965,24 @@ __device__ __forceinline__ uint64_t smix(uint64_t z) {
     z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
     return z ^ (z >> 31);
 }
-__global__ void k_generate(float* dst, int lp, int64_t n, uint64_t seed, int dist, double sigma) {
+template<class T>
+__global__ void k_generate(T* dst, int lp, int64_t n, uint64_t seed, int dist, double sigma) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n)
         return;
     const uint64_t gam = 0x9E3779B97F4A7C15ULL;
     double         u1  = (double)(smix(seed + (uint64_t)(2 * i + 1) * gam) >> 11) * (1.0 / 9007199254740992.0);
     double         u2  = (double)(smix(seed + (uint64_t)(2 * i + 2) * gam) >> 11) * (1.0 / 9007199254740992.0);
-    float          re, im;
+    T              re, im;
     if (dist == 0) {
-        re = (float)(2.0 * u1 - 1.0);
-        im = (float)(2.0 * u2 - 1.0);
+        re = (T)(2.0 * u1 - 1.0);
+        im = (T)(2.0 * u2 - 1.0);
     } else {
         double r = sqrt(-2.0 * log(1.0 - u1));
         double s, c;
         sincospi(2.0 * u2, &s, &c);
-        re = (float)(sigma * r * c);
-        im = (float)(sigma * r * s);
+        re = (T)(sigma * r * c);
+        im = (T)(sigma * r * s);
     }
     int64_t o                   = i + ((i >> lp) << lp);
     dst[o]                      = re;

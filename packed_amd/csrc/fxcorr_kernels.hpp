@@ -98,7 +98,7 @@ void k_fxcorr(XcArgs p) {
     if (aligned && !p.hre) {
         const int64_t  rem = p.len - off;
         const uint32_t nv  = ok ? (uint32_t)(rem < Plan<LA>::A ? rem : Plan<LA>::A) : 0u;
-        const BufR     sr(src + poff((uint32_t)off, p.src_lp), p.src_lp, poff(nv, p.src_lp));
+        const BufR<>   sr(src + poff((uint32_t)off, p.src_lp), p.src_lp, poff(nv, p.src_lp));
         const uint32_t vo  = poff((uint32_t)jt, p.src_lp);
 #pragma unroll
         for (int m = 0; m < PPT / R0; ++m)
@@ -120,7 +120,7 @@ void k_fxcorr(XcArgs p) {
                         int64_t h = bg * p.hlen + (a - p.step);
                         x         = {p.hre[h], p.him[h]};
                     } else if constexpr (B32) {
-                        x = BufR(src, p.src_lp).ld(poff((uint32_t)i, p.src_lp), 0);
+                        x = BufR<>(src, p.src_lp).ld(poff((uint32_t)i, p.src_lp), 0);
                     } else {
                         x = ld_cx64(src, i, p.src_lp);
                     }
@@ -132,7 +132,7 @@ void k_fxcorr(XcArgs p) {
     {
         // Y = conj(X) * K.  The spectrum is read in two halves: the second half's offset depends
         // (opaquely) on the first half's products, so at most half of it is in registers beside v.
-        const BufR kr(p.kre, LA); // split spectrum: re[N], im[N]
+        const BufR<> kr(p.kre, LA); // split spectrum: re[N], im[N]
         constexpr int NG = PPT / RL;
         int           jk = jt;
 #pragma unroll
@@ -153,7 +153,7 @@ void k_fxcorr(XcArgs p) {
     const int64_t keep = p.len - off < p.step ? p.len - off : p.step;
     if (aligned) {
         const int      jt = opaque_after_memory(t % TA);
-        const BufW     dw(dst + poff((uint32_t)off, p.dst_lp), p.dst_lp, ok ? poff((uint32_t)keep, p.dst_lp) : 0u);
+        const BufW<>   dw(dst + poff((uint32_t)off, p.dst_lp), p.dst_lp, ok ? poff((uint32_t)keep, p.dst_lp) : 0u);
         const uint32_t vo = poff((uint32_t)jt, p.dst_lp);
 #pragma unroll
         for (int m = 0; m < PPT / RLI; ++m)
@@ -170,7 +170,7 @@ void k_fxcorr(XcArgs p) {
                 int a = out_elem<LA, true>(jt, m, r);
                 if (a < keep) {
                     if constexpr (B32)
-                        BufW(dst, p.dst_lp).st(poff((uint32_t)(off + a), p.dst_lp), cconj(v[m * RLI + r]), 0);
+                        BufW<>(dst, p.dst_lp).st(poff((uint32_t)(off + a), p.dst_lp), cconj(v[m * RLI + r]), 0);
                     else
                         st_cx64(dst, off + a, p.dst_lp, cconj(v[m * RLI + r]));
                 }

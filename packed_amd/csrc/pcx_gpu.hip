@@ -41,6 +41,8 @@ int fail(int code, std::string msg) {
 
 constexpr int kMaxSinglePass = 14; // N <= 16384 runs in one LDS-resident pass
 constexpr int kMaxLog2       = 24; // four-step limit: N2 <= 1024 columns, N1 <= 16384
+constexpr int kMaxSinglePass64 = 13; // FP64: N <= 8192 in one pass
+constexpr int kMaxLog2_64      = 23; // FP64 four-step limit
 
 bool    is_pow2(int64_t v) { return v > 0 && (v & (v - 1)) == 0; }
 int64_t pidx64(int64_t i, int lp) { return i + ((i >> lp) << lp); }
@@ -74,36 +76,43 @@ int check_pack(int64_t p, const char* what) {
     return fail(PCX_ERR_INVALID_PACK, std::string(what) + " pack size (which is " + s(p) + ") is not a power of two");
 }
 
-hipError_t gen_twiddles(float2* t, int64_t M, int64_t stride, int64_t count) {
+template<class V>
+hipError_t gen_twiddles(V* t, int64_t M, int64_t stride, int64_t count) {
+    using T    = decltype(V{}.x);
     int64_t nb = (count + 255) / 256;
-    hipLaunchKernelGGL(k_twiddles, dim3((unsigned)nb), dim3(256), 0, 0, t, M, stride, count);
+    hipLaunchKernelGGL(k_twiddles<T>, dim3((unsigned)nb), dim3(256), 0, 0, t, M, stride, count);
     return hipGetLastError();
 }
 
 // ---------------------------------------------------------------- kernel dispatch tables
-using RowLaunch = void (*)(const RowArgs&, int64_t nrows, int64_t grid_cap, hipStream_t);
-using ColLaunch = void (*)(const ColArgs&, int64_t grid_cap, hipStream_t);
+template<class T>
+using RowLaunchT = void (*)(const RowArgsT<T>&, int64_t nrows, int64_t grid_cap, hipStream_t);
+template<class T>
+using ColLaunchT = void (*)(const ColArgsT<T>&, int64_t grid_cap, hipStream_t);
+using RowLaunch  = RowLaunchT<float>;
+using ColLaunch  = ColLaunchT<float>;
 using XcLaunch  = void (*)(const XcArgs&, hipStream_t);
 
 // grid_cap > 0 bounds the grid; the kernels then loop over their work (grid-stride).  Grids are
 // rounded up to a multiple of 8 for the XCD-aware item order (spare workgroups exit at once).
 int64_t grid8(int64_t n) { return (n + 7) & ~(int64_t)7; }
-template<int LA, int G, int LD, int ST, int TW, bool UNI = (Plan<LA>::TA >= 64), int ORD = 0, int PREC = PREC_FP32>
-void launch_row(const RowArgs& a, int64_t nrows, int64_t grid_cap, hipStream_t st) {
+template<int LA, int G, int LD, int ST, int TW, bool UNI = (Plan<LA>::TA >= 64), int ORD = 0, int PREC = PREC_FP32,
+         class T = float>
+void launch_row(const RowArgsT<T>& a, int64_t nrows, int64_t grid_cap, hipStream_t st) {
     int64_t nwg = (nrows + G - 1) / G;
     if (grid_cap > 0 && nwg > grid_cap)
         nwg = grid_cap;
     nwg = grid8(nwg);
-    hipLaunchKernelGGL((k_row<LA, G, LD, ST, TW, UNI, ORD, PREC>), dim3((unsigned)nwg), dim3(G * Plan<LA>::TA), 0, st,
-                       a);
+    hipLaunchKernelGGL((k_row<LA, G, LD, ST, TW, UNI, ORD, PREC, T>), dim3((unsigned)nwg),
+                       dim3(G * Plan<LA, false, T>::TA), 0, st, a);
 }
-template<int LA, int C, int TW, int PREC = PREC_FP32>
-void launch_col(const ColArgs& a, int64_t grid_cap, hipStream_t st) {
+template<int LA, int C, int TW, int PREC = PREC_FP32, class T = float>
+void launch_col(const ColArgsT<T>& a, int64_t grid_cap, hipStream_t st) {
     int64_t nwg = a.ntiles;
     if (grid_cap > 0 && nwg > grid_cap)
         nwg = grid_cap;
     nwg = grid8(nwg);
-    hipLaunchKernelGGL((k_col<LA, C, TW, PREC>), dim3((unsigned)nwg), dim3(C * Plan<LA>::TA), 0, st, a);
+    hipLaunchKernelGGL((k_col<LA, C, TW, PREC, T>), dim3((unsigned)nwg), dim3(C * Plan<LA, false, T>::TA), 0, st, a);
 }
 template<int LA, int PREC = PREC_FP32>
 void launch_xc(const XcArgs& a, hipStream_t st) {
@@ -160,6 +169,23 @@ template<int LG>
 constexpr int tw_cap() {
     return pass_tw_entries(LG) <= 2048 ? pass_tw_entries(LG) : 2;
 }
+// ---- FP64 (fft_unit<double, Order>): natural and bit_reversed orders (the reference has no FP64
+// unordered kernel: avx2_fft.hpp:114-217 is written for float only)
+template<int LA>
+constexpr std::array<RowLaunchT<double>, 5> single_set64() {
+    using D             = double;
+    constexpr int  G    = default_rows<LA, D>();
+    constexpr bool UNI  = Plan<LA, false, D>::TA >= 64;
+    return {&launch_row<LA, G, LD_CONTIG, ST_CONTIG, 0, UNI, 0, PREC_FP32, D>,
+            &launch_row<LA, G, LD_CONTIG, ST_PERM, 0, UNI, ORD_BITREV, PREC_FP32, D>, nullptr,
+            &launch_row<LA, G, LD_PERM, ST_CONTIG, 0, UNI, ORD_BITREV, PREC_FP32, D>, nullptr};
+}
+template<int... L>
+constexpr auto single_table64(std::integer_sequence<int, L...>) {
+    return std::array<std::array<RowLaunchT<double>, 5>, sizeof...(L)>{single_set64<L + 1>()...};
+}
+const auto kSingle64 = single_table64(std::make_integer_sequence<int, kMaxSinglePass64>{}); // index LA-1
+
 // Four-step row-pass variants: forward with strided store (natural order) or permuted store
 // (bit_reversed, unordered); inverse with strided or permuted load and the pass twiddle.
 // The G rows of a workgroup lie in one transform (G <= N2), so buffer addressing applies.
@@ -181,11 +207,13 @@ constexpr std::array<ColLaunch, 2> col_set() {
 // The four-step shapes this build instantiates: N = 2^lg = N1 * N2 with N1 = 2^n1 (row length,
 // G rows per workgroup, G * N1 / PPT <= 1024 threads) and N2 = 2^n2 (column length, C columns
 // per tile, C * N2 <= 16384 for LDS).
-struct FourStep {
-    int     lg, n1, n2, g, c;
-    PassSet rows;
-    std::array<ColLaunch, 2> cols;
+template<class T>
+struct FourStepT {
+    int                              lg, n1, n2, g, c;
+    std::array<RowLaunchT<T>, 6>     rows;
+    std::array<ColLaunchT<T>, 2>     cols;
 };
+using FourStep = FourStepT<float>;
 template<int LG, int N1, int N2, int G, int C>
 FourStep shape() {
     static_assert(N1 + N2 == LG, "shape");
@@ -201,6 +229,30 @@ const std::array<FourStep, 20> kShapes = {
     shape<16, 10, 6, 16, 64>(), shape<16, 7, 9, 64, 32>(), shape<16, 6, 10, 64, 16>(),
     shape<16, 9, 7, 32, 32>(), shape<16, 10, 6, 16, 32>(),
 };
+// FP64 shapes, 2^14 .. 2^23: LDS of about 64-80 KiB per workgroup (two workgroups per CU), 256
+// threads; C and G are 16 doubles (128-byte segments) up to 2^16, fewer above.  Pass-twiddle
+// tables above 512 entries are read from global memory.
+template<int LG>
+constexpr int tw_cap64() {
+    return pass_tw_entries(LG) <= 512 ? pass_tw_entries(LG) : 2;
+}
+template<int LG, int N1, int N2, int G, int C>
+FourStepT<double> shape64() {
+    static_assert(N1 + N2 == LG && G <= (1 << N2), "shape");
+    using D          = double;
+    constexpr int TW = tw_cap64<LG>();
+    return {LG, N1, N2, G, C,
+            {&launch_row<N1, G, LD_CONTIG, ST_STRIDED, 0, true, 0, PREC_FP32, D>,
+             &launch_row<N1, G, LD_CONTIG, ST_PERM, 0, true, ORD_BITREV, PREC_FP32, D>, nullptr,
+             &launch_row<N1, G, LD_STRIDED, ST_CONTIG, TW, true, 0, PREC_FP32, D>,
+             &launch_row<N1, G, LD_PERM, ST_CONTIG, TW, true, ORD_BITREV, PREC_FP32, D>, nullptr},
+            {&launch_col<N2, C, 0, PREC_FP32, D>, &launch_col<N2, C, TW, PREC_FP32, D>}};
+}
+const std::array<FourStepT<double>, 10> kShapes64 = {
+    shape64<14, 7, 7, 32, 32>(),  shape64<15, 8, 7, 16, 32>(), shape64<16, 8, 8, 16, 16>(), shape64<17, 9, 8, 8, 16>(),
+    shape64<18, 10, 8, 4, 16>(),  shape64<19, 10, 9, 4, 8>(),  shape64<20, 11, 9, 2, 8>(),  shape64<21, 12, 9, 2, 8>(),
+    shape64<22, 13, 9, 1, 8>(),   shape64<23, 13, 10, 1, 4>()};
+
 // split-BF16 matrix-core variants of the default shapes for 2^15 .. 2^20 (natural order):
 // rows {forward, inverse}, cols {inverse, forward}
 struct FourStepMfma {
@@ -274,6 +326,13 @@ struct pcx_plan_s {
     int     nt_flags     = 0; // bit 0/1: streaming load/store of src/dst, bit 2/3: of the scratch
     float2* tw_dif       = nullptr; // W_M for the one-pass k_dif kernel
     bool    use_dif      = false;   // out-of-place transforms of 2^15 run as one k_dif pass
+    // FP64 plans (pcx_plan_create_f64): the same roles in double precision
+    bool     f64      = false;
+    double2* tw64_row = nullptr;
+    double2* tw64_col = nullptr;
+    double2* twl64    = nullptr;
+    double2* twh64    = nullptr;
+    const FourStepT<double>* shape64 = nullptr;
     std::mutex mu;
 };
 
@@ -388,15 +447,92 @@ extern "C" int pcx_plan_destroy(pcx_plan_t p) {
     (void)hipFree(p->twh);
     (void)hipFree(p->scratch);
     (void)hipFree(p->tw_dif);
+    (void)hipFree(p->tw64_row);
+    (void)hipFree(p->tw64_col);
+    (void)hipFree(p->twl64);
+    (void)hipFree(p->twh64);
     delete p;
     return PCX_SUCCESS;
 }
+
+extern "C" int pcx_plan_create_f64(int64_t fft_size, int64_t sub_size, int order, pcx_plan_t* out) {
+    if (!out)
+        return fail(PCX_ERR_INVALID_ARGUMENT, "plan output pointer is null");
+    *out = nullptr;
+    if (int e = check_size(fft_size))
+        return e;
+    if (int e = check_sub_size(sub_size))
+        return e;
+    if (order < 0 || order > 2)
+        return fail(PCX_ERR_INVALID_ARGUMENT, "order (which is " + s(order) + ") is not a fft_order");
+    if (order == PCX_ORDER_UNORDERED)
+        return fail(PCX_ERR_UNSUPPORTED, "the unordered order is defined for float only (the reference's "
+                                         "unsorted kernels are float-only)");
+    int lg = ilog2(fft_size);
+    if (lg > kMaxLog2_64)
+        return fail(PCX_ERR_UNSUPPORTED,
+                    "double fft_size (which is " + s(fft_size) + ") exceeds the supported maximum 2^23");
+    auto* p  = new pcx_plan_s;
+    p->n     = fft_size;
+    p->lg    = lg;
+    p->order = order;
+    p->f64   = true;
+    hipError_t he = hipSuccess;
+    if (lg <= kMaxSinglePass64) {
+        he = hipMalloc(&p->tw64_row, sizeof(double2) * fft_size);
+        if (he == hipSuccess)
+            he = gen_twiddles(p->tw64_row, fft_size, 1, fft_size);
+    } else {
+        p->two_pass = true;
+        for (const auto& f : kShapes64)
+            if (f.lg == lg && !p->shape64)
+                p->shape64 = &f;
+        p->n1_lg = p->shape64->n1;
+        p->n2_lg = p->shape64->n2;
+        p->tw_ls = (lg + 1) / 2;
+        int64_t n1 = (int64_t)1 << p->n1_lg, n2 = (int64_t)1 << p->n2_lg;
+        int64_t nl = (int64_t)1 << p->tw_ls, nh = fft_size >> p->tw_ls;
+        he = hipMalloc(&p->tw64_row, sizeof(double2) * n1);
+        if (he == hipSuccess)
+            he = hipMalloc(&p->tw64_col, sizeof(double2) * n2);
+        if (he == hipSuccess)
+            he = hipMalloc(&p->twl64, sizeof(double2) * nl);
+        if (he == hipSuccess)
+            he = hipMalloc(&p->twh64, sizeof(double2) * nh);
+        if (he == hipSuccess)
+            he = gen_twiddles(p->tw64_row, n1, 1, n1);
+        if (he == hipSuccess)
+            he = gen_twiddles(p->tw64_col, n2, 1, n2);
+        if (he == hipSuccess)
+            he = gen_twiddles(p->twl64, fft_size, 1, nl);
+        if (he == hipSuccess)
+            he = gen_twiddles(p->twh64, fft_size, nl, nh);
+        int64_t chunk_bytes = (int64_t)2048 << 20;
+        if (const char* env = std::getenv("PCX_CHUNK_MB"))
+            chunk_bytes = (int64_t)std::atoll(env) << 20;
+        p->chunk = chunk_bytes / (fft_size * 16);
+        if (p->chunk < 1)
+            p->chunk = 1;
+    }
+    if (he == hipSuccess)
+        he = hipDeviceSynchronize();
+    if (he != hipSuccess) {
+        pcx_plan_destroy(p);
+        return fail(PCX_ERR_HIP, std::string("plan setup: ") + hipGetErrorString(he));
+    }
+    *out = p;
+    return PCX_SUCCESS;
+}
+
+extern "C" int pcx_plan_element_size(pcx_plan_t p) { return p ? (p->f64 ? 8 : 4) : 0; }
 
 extern "C" int64_t pcx_plan_size(pcx_plan_t p) { return p ? p->n : 0; }
 
 extern "C" int pcx_plan_set_precision(pcx_plan_t p, int precision) {
     if (!p)
         return fail(PCX_ERR_INVALID_ARGUMENT, "plan is null");
+    if (p->f64)
+        return fail(PCX_ERR_UNSUPPORTED, "a double plan has no precision modes");
     if (precision == PCX_PREC_FP32) {
         p->precision = precision;
         return PCX_SUCCESS;
@@ -440,12 +576,18 @@ int plan_apply_shape(pcx_plan_t p, const FourStep* sh) {
 
 namespace {
 
-// src_offs / dst_offs (optional, device arrays of `batch` float offsets) replace b * dist.
-int run_batch(pcx_plan_t p, int inverse, int normalized, float* dst, int64_t dst_pack, int64_t dst_dist,
-              const float* src, int64_t src_pack, int64_t src_dist, int64_t src_size, int64_t batch,
+// src_offs / dst_offs (optional, device arrays of `batch` element offsets) replace b * dist.
+// T is the element type: float, or double for a plan made by pcx_plan_create_f64.
+template<class T>
+int run_batch(pcx_plan_t p, int inverse, int normalized, T* dst, int64_t dst_pack, int64_t dst_dist,
+              const T* src, int64_t src_pack, int64_t src_dist, int64_t src_size, int64_t batch,
               hipStream_t st, const int64_t* src_offs = nullptr, const int64_t* dst_offs = nullptr) {
+    constexpr bool F64 = std::is_same_v<T, double>;
     if (!p)
         return fail(PCX_ERR_INVALID_ARGUMENT, "plan is null");
+    if (p->f64 != F64)
+        return fail(PCX_ERR_INVALID_ARGUMENT, p->f64 ? "the plan is for double data (use the _f64 functions)"
+                                                     : "the plan is for float data (made by pcx_plan_create)");
     if (!dst || !src)
         return fail(PCX_ERR_INVALID_ARGUMENT, "data pointer is null");
     if (batch < 0)
@@ -467,25 +609,31 @@ int run_batch(pcx_plan_t p, int inverse, int normalized, float* dst, int64_t dst
         if (!inverse && dst_pack < 8)
             dst_pack = 8;
     }
-    if ((const float*)dst == src && dst_pack != src_pack && (dst_pack > 8 || src_pack > 8))
+    if ((const T*)dst == src && dst_pack != src_pack && (dst_pack > 8 || src_pack > 8))
         return fail(PCX_ERR_INPLACE_REPACK, "cannot perform in-place repack from source pack size (which is " +
                                                 s(src_pack) + ") to destination pack size (which is " + s(dst_pack) + ")");
     if (batch == 0)
         return PCX_SUCCESS;
     const int   src_lp = ilog2(src_pack), dst_lp = ilog2(dst_pack);
-    const float scale  = (inverse && normalized) ? (float)(1.0 / (double)n) : 1.0f;
-    const bool  permuted = p->order != PCX_ORDER_NORMAL;
+    const T     scale  = (inverse && normalized) ? (T)(1.0 / (double)n) : T(1);
     if (inverse)
         src_size = n;
+    const vec2<T> *tw_row, *tw_col, *twl, *twh;
+    if constexpr (F64)
+        tw_row = p->tw64_row, tw_col = p->tw64_col, twl = p->twl64, twh = p->twh64;
+    else
+        tw_row = p->tw_row, tw_col = p->tw_col, twl = p->twl, twh = p->twh;
 
     if (!p->two_pass) {
-        RowArgs a{};
+        RowArgsT<T> a{};
         a.src = src, a.dst = dst, a.src_dist = src_dist, a.dst_dist = dst_dist;
         a.src_size = src_size, a.nrows = batch, a.src_lp = src_lp, a.dst_lp = dst_lp, a.rows_lg = 0;
         a.order = p->order, a.conj_in = inverse, a.conj_out = inverse, a.scale = scale;
-        a.twA = p->tw_row, a.twl = nullptr, a.twh = nullptr, a.tw_ls = 0;
+        a.twA = tw_row, a.twl = nullptr, a.twh = nullptr, a.tw_ls = 0;
         a.src_offs = src_offs, a.dst_offs = dst_offs;
-        if (p->precision == PCX_PREC_BF16X3)
+        if constexpr (F64)
+            kSingle64[p->lg - 1][single_variant(inverse, p->order)](a, batch, 0, st);
+        else if (p->precision == PCX_PREC_BF16X3)
             kSingleMfma[p->lg - 4](a, batch, 0, st);
         else
             kSingle[p->lg - 1][single_variant(inverse, p->order)](a, batch, 0, st);
@@ -494,6 +642,7 @@ int run_batch(pcx_plan_t p, int inverse, int normalized, float* dst, int64_t dst
     }
 
     // ---- one pass, radix-R decimation in frequency + M-point Stockham (no intermediate)
+    if constexpr (!F64)
     if (p->use_dif && p->precision == PCX_PREC_FP32 && (const float*)dst != src &&
         (!inverse || p->order == PCX_ORDER_NORMAL)) {
         RowArgs a{};
@@ -517,14 +666,21 @@ int run_batch(pcx_plan_t p, int inverse, int normalized, float* dst, int64_t dst
             HIPCHK(hipFree(p->scratch));
             p->scratch = nullptr;
         }
-        hipError_t he = hipMalloc(&p->scratch, sizeof(float) * 2 * n * chunk);
+        hipError_t he = hipMalloc(&p->scratch, sizeof(T) * 2 * n * chunk);
         if (he != hipSuccess)
-            return fail(PCX_ERR_OUT_OF_MEMORY, "scratch allocation of " + s(8 * n * chunk) + " bytes failed");
+            return fail(PCX_ERR_OUT_OF_MEMORY, "scratch allocation of " + s(2 * sizeof(T) * n * chunk) + " bytes failed");
         p->scratch_cap = chunk;
     }
-    const int C    = p->shape->c;
-    ColLaunch colf = p->shape->cols[inverse ? 0 : 1];
-    RowLaunch rowf = p->shape->rows[(inverse ? 3 : 0) + p->order];
+    T* const scratch = reinterpret_cast<T*>(p->scratch);
+    const FourStepT<T>* shape;
+    if constexpr (F64)
+        shape = p->shape64;
+    else
+        shape = p->shape;
+    const int     C    = shape->c;
+    ColLaunchT<T> colf = shape->cols[inverse ? 0 : 1];
+    RowLaunchT<T> rowf = shape->rows[(inverse ? 3 : 0) + p->order];
+    if constexpr (!F64)
     if (p->precision == PCX_PREC_BF16X3) {
         const FourStepMfma* fm = find_mfma(p->lg, p->shape);
         if (!fm)
@@ -535,40 +691,40 @@ int run_batch(pcx_plan_t p, int inverse, int normalized, float* dst, int64_t dst
     const int64_t n1 = (int64_t)1 << p->n1_lg, n2 = (int64_t)1 << p->n2_lg;
     for (int64_t b0 = 0; b0 < batch; b0 += chunk) {
         int64_t      cb = batch - b0 < chunk ? batch - b0 : chunk;
-        const float*   cs  = src_offs ? src : src + b0 * src_dist;
-        float*         cd  = dst_offs ? dst : dst + b0 * dst_dist;
+        const T*       cs  = src_offs ? src : src + b0 * src_dist;
+        T*             cd  = dst_offs ? dst : dst + b0 * dst_dist;
         const int64_t* cso = src_offs ? src_offs + b0 : nullptr;
         const int64_t* cdo = dst_offs ? dst_offs + b0 : nullptr;
-        ColArgs      ca{};
-        RowArgs      ra{};
-        ca.twA = p->tw_col, ca.n1_lg = p->n1_lg, ca.ntiles = cb * (n1 / C);
-        ra.twA = p->tw_row, ra.rows_lg = p->n2_lg, ra.nrows = cb * n2, ra.order = p->order;
+        ColArgsT<T>  ca{};
+        RowArgsT<T>  ra{};
+        ca.twA = tw_col, ca.n1_lg = p->n1_lg, ca.ntiles = cb * (n1 / C);
+        ra.twA = tw_row, ra.rows_lg = p->n2_lg, ra.nrows = cb * n2, ra.order = p->order;
         ca.tw_lg = p->lg, ra.tw_lg = p->lg;
         if (!inverse) {
             // pass A: columns of src -> scratch, times W_N^(n1*k2)
             ca.src = cs, ca.src_dist = src_dist, ca.src_lp = src_lp, ca.src_size = src_size, ca.src_offs = cso;
-            ca.dst = p->scratch, ca.dst_dist = 2 * n, ca.dst_lp = p->lg;
-            ca.conj_in = 0, ca.conj_out = 0, ca.scale = 1.0f;
-            ca.twl = p->twl, ca.twh = p->twh, ca.tw_ls = p->tw_ls;
+            ca.dst = scratch, ca.dst_dist = 2 * n, ca.dst_lp = p->lg;
+            ca.conj_in = 0, ca.conj_out = 0, ca.scale = T(1);
+            ca.twl = twl, ca.twh = twh, ca.tw_ls = p->tw_ls;
             ca.nt_load = p->nt_flags & 1, ca.nt_store = (p->nt_flags >> 2) & 1;
             colf(ca, p->grid_col, st);
             // pass B: rows of scratch -> dst in the plan's order
-            ra.src = p->scratch, ra.src_dist = 2 * n, ra.src_lp = p->lg, ra.src_size = n;
+            ra.src = scratch, ra.src_dist = 2 * n, ra.src_lp = p->lg, ra.src_size = n;
             ra.dst = cd, ra.dst_dist = dst_dist, ra.dst_lp = dst_lp, ra.dst_offs = cdo;
-            ra.conj_in = 0, ra.conj_out = 0, ra.scale = 1.0f;
+            ra.conj_in = 0, ra.conj_out = 0, ra.scale = T(1);
             ra.twl = nullptr, ra.twh = nullptr;
             ra.nt_load = (p->nt_flags >> 3) & 1, ra.nt_store = (p->nt_flags >> 1) & 1;
             rowf(ra, ra.nrows, p->grid_row, st);
         } else {
             // pass 1: rows (over k1) of the spectrum -> scratch, times W_N^(n1*k2)
             ra.src = cs, ra.src_dist = src_dist, ra.src_lp = src_lp, ra.src_size = n, ra.src_offs = cso;
-            ra.dst = p->scratch, ra.dst_dist = 2 * n, ra.dst_lp = p->lg;
-            ra.conj_in = 1, ra.conj_out = 0, ra.scale = 1.0f;
-            ra.twl = p->twl, ra.twh = p->twh, ra.tw_ls = p->tw_ls;
+            ra.dst = scratch, ra.dst_dist = 2 * n, ra.dst_lp = p->lg;
+            ra.conj_in = 1, ra.conj_out = 0, ra.scale = T(1);
+            ra.twl = twl, ra.twh = twh, ra.tw_ls = p->tw_ls;
             ra.nt_load = p->nt_flags & 1, ra.nt_store = (p->nt_flags >> 2) & 1;
             rowf(ra, ra.nrows, p->grid_row, st);
             // pass 2: columns (over k2) of scratch -> dst, natural order
-            ca.src = p->scratch, ca.src_dist = 2 * n, ca.src_lp = p->lg, ca.src_size = n;
+            ca.src = scratch, ca.src_dist = 2 * n, ca.src_lp = p->lg, ca.src_size = n;
             ca.dst = cd, ca.dst_dist = dst_dist, ca.dst_lp = dst_lp, ca.dst_offs = cdo;
             ca.conj_in = 0, ca.conj_out = 1, ca.scale = scale;
             ca.twl = nullptr, ca.twh = nullptr;
@@ -598,9 +754,10 @@ extern "C" int pcx_ifft(pcx_plan_t p, int normalized, float* dst, int64_t dst_pa
     return run_batch(p, 1, normalized, dst, dst_pack, 0, src, src_pack, 0, p->n, 1, (hipStream_t)stream);
 }
 
-extern "C" int pcx_fft_batch(pcx_plan_t p, int inverse, int normalized, float* dst, int64_t dst_pack,
-                             int64_t dst_dist, const float* src, int64_t src_pack, int64_t src_dist,
-                             int64_t src_size, int64_t batch, void* stream) {
+namespace {
+template<class T>
+int fft_batch(pcx_plan_t p, int inverse, int normalized, T* dst, int64_t dst_pack, int64_t dst_dist, const T* src,
+              int64_t src_pack, int64_t src_dist, int64_t src_size, int64_t batch, void* stream) {
     if (!p)
         return fail(PCX_ERR_INVALID_ARGUMENT, "plan is null");
     if (batch > 1) {
@@ -614,11 +771,41 @@ extern "C" int pcx_fft_batch(pcx_plan_t p, int inverse, int normalized, float* d
         if (is_pow2(dp) && is_pow2(sp) &&
             (dst_dist < footprint(p->n, dp) || src_dist < footprint(p->n, sp)))
             return fail(PCX_ERR_INVALID_ARGUMENT, "batch distance is smaller than one vector");
-        if ((const float*)dst == src && dst_dist != src_dist)
+        if ((const T*)dst == src && dst_dist != src_dist)
             return fail(PCX_ERR_INVALID_ARGUMENT, "in-place batch needs equal source and destination distances");
     }
     return run_batch(p, inverse, normalized, dst, dst_pack, dst_dist, src, src_pack, src_dist,
                      inverse ? p->n : src_size, batch, (hipStream_t)stream);
+}
+} // namespace
+
+extern "C" int pcx_fft_batch(pcx_plan_t p, int inverse, int normalized, float* dst, int64_t dst_pack,
+                             int64_t dst_dist, const float* src, int64_t src_pack, int64_t src_dist,
+                             int64_t src_size, int64_t batch, void* stream) {
+    return fft_batch(p, inverse, normalized, dst, dst_pack, dst_dist, src, src_pack, src_dist, src_size, batch,
+                     stream);
+}
+
+// ---------------------------------------------------------------- FP64 transforms
+extern "C" int pcx_fft_f64(pcx_plan_t p, double* dst, int64_t dst_pack, const double* src, int64_t src_pack,
+                           int64_t src_size, void* stream) {
+    if (!p)
+        return fail(PCX_ERR_INVALID_ARGUMENT, "plan is null");
+    return run_batch(p, 0, 0, dst, dst_pack, 0, src, src_pack, 0, src_size, 1, (hipStream_t)stream);
+}
+
+extern "C" int pcx_ifft_f64(pcx_plan_t p, int normalized, double* dst, int64_t dst_pack, const double* src,
+                            int64_t src_pack, void* stream) {
+    if (!p)
+        return fail(PCX_ERR_INVALID_ARGUMENT, "plan is null");
+    return run_batch(p, 1, normalized, dst, dst_pack, 0, src, src_pack, 0, p->n, 1, (hipStream_t)stream);
+}
+
+extern "C" int pcx_fft_batch_f64(pcx_plan_t p, int inverse, int normalized, double* dst, int64_t dst_pack,
+                                 int64_t dst_dist, const double* src, int64_t src_pack, int64_t src_dist,
+                                 int64_t src_size, int64_t batch, void* stream) {
+    return fft_batch(p, inverse, normalized, dst, dst_pack, dst_dist, src, src_pack, src_dist, src_size, batch,
+                     stream);
 }
 
 // ---------------------------------------------------------------- cross-correlation
@@ -817,8 +1004,22 @@ extern "C" int pcx_generate(float* dst, int64_t pack, int64_t n, uint64_t seed, 
     if (n <= 0)
         return PCX_SUCCESS;
     int64_t nb = (n + 255) / 256;
-    hipLaunchKernelGGL(k_generate, dim3((unsigned)nb), dim3(256), 0, (hipStream_t)stream, dst, ilog2(pack), n, seed,
+    hipLaunchKernelGGL(k_generate<float>, dim3((unsigned)nb), dim3(256), 0, (hipStream_t)stream, dst, ilog2(pack), n, seed,
                        dist, sigma);
+    HIPCHK(hipGetLastError());
+    return PCX_SUCCESS;
+}
+extern "C" int pcx_generate_f64(double* dst, int64_t pack, int64_t n, uint64_t seed, int dist, double sigma,
+                                void* stream) {
+    if (!dst)
+        return fail(PCX_ERR_INVALID_ARGUMENT, "data pointer is null");
+    if (int e = check_pack(pack, "destination"))
+        return e;
+    if (n <= 0)
+        return PCX_SUCCESS;
+    int64_t nb = (n + 255) / 256;
+    hipLaunchKernelGGL(k_generate<double>, dim3((unsigned)nb), dim3(256), 0, (hipStream_t)stream, dst, ilog2(pack), n,
+                       seed, dist, sigma);
     HIPCHK(hipGetLastError());
     return PCX_SUCCESS;
 }
@@ -1273,6 +1474,8 @@ extern "C" int pcx_fft_across(pcx_plan_t p, int inverse, int normalized, float* 
                               const float* const* src_vecs, int64_t src_pack, int64_t length, void* stream) {
     if (!p)
         return fail(PCX_ERR_INVALID_ARGUMENT, "plan is null");
+    if (p->f64)
+        return fail(PCX_ERR_UNSUPPORTED, "transforms across vectors are built for float plans");
     if (!dst_vecs || !src_vecs)
         return fail(PCX_ERR_INVALID_ARGUMENT, "vector table is null");
     if (length < 0)

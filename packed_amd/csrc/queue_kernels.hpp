@@ -56,8 +56,8 @@ __device__ __forceinline__ void queue_item(const QueueArgs& q, const QItem& it, 
     float* ltw             = lds + RowShape<LA, G>::DATA;
     TwTable<LA>::fill(ltw, p.twA, threadIdx.x, kQueueThreads);
     __syncthreads();
-    row_group<LA, G, LD_CONTIG, ST_CONTIG, 0, (Plan<LA>::TA >= 64), 0>(p, 0, lds, RowShape<LA, G>::im(lds), ltw,
-                                                                         nullptr);
+    row_group<LA, G, LD_CONTIG, ST_CONTIG, 0, (Plan<LA>::TA >= 64), 0, PREC_FP32, float>(p, 0, lds, RowShape<LA, G>::im(lds),
+                                                                                         ltw, nullptr);
 }
 
 template<int... L>

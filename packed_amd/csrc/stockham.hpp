@@ -24,20 +24,23 @@
 
 namespace pcxk {
 
-template<int LA, bool REV = false>
+template<int LA, bool REV = false, class T = float>
 struct Plan {
+    static constexpr bool F64    = sizeof(T) == 8;
     static constexpr int A       = 1 << LA;
     static constexpr int NFULL   = LA / 4;           // radix-16 stages
     static constexpr int REM     = LA % 4;           // bits left for the small-radix stage
     static constexpr int NSTAGES = NFULL + (REM ? 1 : 0);
     // points per thread: more for the largest sizes, so that a 2^13 transform takes 256 threads
     // and a 2^14 transform 256 threads as well (two workgroups share a CU: one loads or stores
-    // while the other computes)
-    static constexpr int PPT     = LA >= 14 ? 64 : LA >= 13 ? 32 : (LA >= 4 ? 16 : A);
+    // while the other computes).  FP64 values take two VGPRs each: at most 32 points per thread.
+    static constexpr int PPT     = F64 ? (LA >= 13 ? 32 : (LA >= 4 ? 16 : A))
+                                       : (LA >= 14 ? 64 : LA >= 13 ? 32 : (LA >= 4 ? 16 : A));
     static constexpr int TA      = A / PPT;          // threads per transform
-    // HALF: the stage exchange moves real parts, then imaginary parts, through one A-float LDS
-    // array (2^14 points: 66 KiB instead of 132 KiB, so two workgroups fit a CU)
-    static constexpr bool HALF   = LA >= 14;
+    // HALF: the stage exchange moves real parts, then imaginary parts, through one A-element LDS
+    // array (2^14 FP32 points: 66 KiB instead of 132 KiB, so two workgroups fit a CU; FP64 from
+    // 2^12 points)
+    static constexpr bool HALF   = F64 ? LA >= 12 : LA >= 14;
     static constexpr int radix(int s) {
         if (REM == 0)
             return 16;
@@ -50,35 +53,35 @@ struct Plan {
     static constexpr int last_radix  = radix(NSTAGES - 1);
 };
 
-// LDS twiddle table of W_A^e = exp(-2*pi*i*e/A).
-template<int LA>
+// LDS twiddle table of W_A^e = exp(-2*pi*i*e/A), elements of type T.
+template<int LA, class T = float>
 struct TwTable {
-    static constexpr bool direct = LA <= 10;
+    static constexpr bool direct = LA <= (sizeof(T) == 8 ? 8 : 10);
     static constexpr int  LOB    = direct ? LA : 6;
     static constexpr int  NLO    = 1 << LOB;
     static constexpr int  NHI    = direct ? 0 : 1 << (LA - LOB);
-    static constexpr int  FLOATS = 2 * (NLO + NHI); // lo re, lo im, hi re, hi im
+    static constexpr int  FLOATS = 2 * (NLO + NHI); // elements: lo re, lo im, hi re, hi im
 
     // tw: global W_A table (interleaved re, im); nthreads threads cooperate.
-    __device__ static void fill(float* t, const float2* __restrict__ tw, int tid, int nthreads) {
+    __device__ static void fill(T* t, const vec2<T>* __restrict__ tw, int tid, int nthreads) {
         for (int e = tid; e < NLO; e += nthreads) {
-            float2 w       = tw[e];
+            vec2<T> w      = tw[e];
             t[e]           = w.x;
             t[NLO + e]     = w.y;
         }
         for (int h = tid; h < NHI; h += nthreads) {
-            float2 w                = tw[h << LOB];
+            vec2<T> w               = tw[h << LOB];
             t[2 * NLO + h]          = w.x;
             t[2 * NLO + NHI + h]    = w.y;
         }
     }
-    __device__ static cf get(const float* t, uint32_t e) {
+    __device__ static cx<T> get(const T* t, uint32_t e) {
         if constexpr (direct) {
             return {t[e], t[NLO + e]};
         } else {
             uint32_t lo = e & (NLO - 1), hi = e >> LOB;
-            cf       a  = {t[lo], t[NLO + lo]};
-            cf       b  = {t[2 * NLO + hi], t[2 * NLO + NHI + hi]};
+            cx<T>    a  = {t[lo], t[NLO + lo]};
+            cx<T>    b  = {t[2 * NLO + hi], t[2 * NLO + NHI + hi]};
             return cmul(a, b);
         }
     }
@@ -104,9 +107,9 @@ __device__ __forceinline__ int opaque_after_memory(int x) {
 }
 
 // Reads of stage s (component C: 0 real, 1 imaginary, 2 both) from LDS into v.
-template<int LA, bool REV, int s, int C, class Addr>
-__device__ __forceinline__ void stage_read(cf* v, const float* lre, const float* lim, int jt, const Addr& addr) {
-    using P          = Plan<LA, REV>;
+template<int LA, bool REV, int s, int C, class Addr, class T>
+__device__ __forceinline__ void stage_read(cx<T>* v, const T* lre, const T* lim, int jt, const Addr& addr) {
+    using P          = Plan<LA, REV, T>;
     constexpr int R  = P::radix(s);
     constexpr int NB = P::PPT / R;
 #pragma unroll
@@ -123,9 +126,9 @@ __device__ __forceinline__ void stage_read(cf* v, const float* lre, const float*
     }
 }
 // Writes of stage s (component C) from v to LDS.
-template<int LA, bool REV, int s, int C, class Addr>
-__device__ __forceinline__ void stage_write(const cf* v, float* lre, float* lim, int jt, const Addr& addr) {
-    using P          = Plan<LA, REV>;
+template<int LA, bool REV, int s, int C, class Addr, class T>
+__device__ __forceinline__ void stage_write(const cx<T>* v, T* lre, T* lim, int jt, const Addr& addr) {
+    using P          = Plan<LA, REV, T>;
     constexpr int R  = P::radix(s);
     constexpr int Ns = P::ns(s);
     constexpr int NB = P::PPT / R;
@@ -144,12 +147,21 @@ __device__ __forceinline__ void stage_write(const cf* v, float* lre, float* lim,
     }
 }
 
+// Matrix-core DFT16 of v (FP32 only; the FP64 path never selects it).
+template<class T>
+__device__ __forceinline__ void dft16_mfma(const MfmaDft16* mf, cx<T>* v) {
+    if constexpr (sizeof(T) == 4)
+        mf->dft16(v);
+}
+
 // PREC_BF16X3 runs the radix-16 butterflies on the matrix cores (mf holds the operands).
 // With Plan::HALF, lre and lim are the same array and each exchange runs in two halves.
-template<int LA, bool REV, class Addr, int PREC = PREC_FP32>
-__device__ __forceinline__ void stockham(cf* v, float* lre, float* lim, int jt0, const Addr& addr, const float* ltw,
+// T = double runs the same schedule in FP64 (PREC is then PREC_FP32, the plain butterflies).
+template<int LA, bool REV, class Addr, int PREC = PREC_FP32, class T = float>
+__device__ __forceinline__ void stockham(cx<T>* v, T* lre, T* lim, int jt0, const Addr& addr, const T* ltw,
                                          const MfmaDft16* mf = nullptr) {
-    using P           = Plan<LA, REV>;
+    using P           = Plan<LA, REV, T>;
+    static_assert(PREC == PREC_FP32 || sizeof(T) == 4, "matrix-core butterflies are FP32 only");
     constexpr int TA  = P::TA;
     constexpr int PPT = P::PPT;
     sfor<0, P::NSTAGES>([&](auto ss) {
@@ -172,10 +184,10 @@ __device__ __forceinline__ void stockham(cf* v, float* lre, float* lim, int jt0,
                 uint32_t      k     = (uint32_t)(jt + m * TA) & (Ns - 1);
 #pragma unroll
                 for (int r = 1; r < R; ++r)
-                    v[m * R + r] = cmul(v[m * R + r], TwTable<LA>::get(ltw, (uint32_t)(r * k) << shift));
+                    v[m * R + r] = cmul(v[m * R + r], TwTable<LA, T>::get(ltw, (uint32_t)(r * k) << shift));
             }
             if constexpr (R == 16 && PREC == PREC_BF16X3)
-                mf->dft16(v + m * R);
+                dft16_mfma(mf, v + m * R);
             else
                 dft<R>(v + m * R);
             if constexpr (NB > 1)
@@ -200,14 +212,14 @@ __device__ __forceinline__ void stockham(cf* v, float* lre, float* lim, int jt0,
 }
 
 // Element index held by register slot m*R + r on entry (first stage) and on exit (last stage).
-template<int LA, bool REV = false>
+template<int LA, bool REV = false, class T = float>
 __device__ __forceinline__ int in_elem(int jt, int m, int r) {
-    using P = Plan<LA, REV>;
+    using P = Plan<LA, REV, T>;
     return jt + m * P::TA + r * (P::A / P::first_radix);
 }
-template<int LA, bool REV = false>
+template<int LA, bool REV = false, class T = float>
 __device__ __forceinline__ int out_elem(int jt, int m, int r) {
-    using P = Plan<LA, REV>;
+    using P = Plan<LA, REV, T>;
     return jt + m * P::TA + r * (P::A / P::last_radix);
 }
 

@@ -1,11 +1,12 @@
 """Python mirror of the reference's FFT-unit and cross-correlation interfaces over the C ABI.
 
-``vector``      pcxo::vector<float, P> (vector.hpp:26-382): n complex values in packs of P reals
-                then P imaginaries, ceil(n/P)*2P floats, zero-filled -- here resident in device
-                memory (a torch float32 tensor on the GPU; torch is only the allocator).
-``fft_unit``    pcxo::fft_unit<float, Order> (fft.hpp:565-915): size(), operator() (forward, in or
-                out of place, zero-extended source), ifft<Normalized>(), fft_raw / ifft_raw on
-                raw device pointers.
+``vector``      pcxo::vector<T, P> (vector.hpp:26-382): n complex values in packs of P reals
+                then P imaginaries, ceil(n/P)*2P elements, zero-filled -- here resident in device
+                memory (a torch float32 / float64 tensor on the GPU; torch is only the allocator).
+                The default pack is 32 / sizeof(T) (types.hpp:70-72): 8 for float, 4 for double.
+``fft_unit``    pcxo::fft_unit<T, Order> (fft.hpp:565-915), T = float or double: size(),
+                operator() (forward, in or out of place, zero-extended source), ifft<Normalized>(),
+                fft_raw / ifft_raw on raw device pointers.
 ``fxcorr_unit`` pcxo::fxcorr_unit<float> (fxcorr.hpp:44-147): construct from the kernel g and an
                 FFT size, call on a signal (in place), or out of place.
 
@@ -48,17 +49,32 @@ def _stream_ptr(stream) -> int | None:
     return getattr(stream, "cuda_stream", stream)
 
 
-class vector:
-    """Device-resident packed complex vector (pcxo::vector<float, P> layout)."""
+DTYPES = ("float32", "float64")
 
-    def __init__(self, n: int, pack: int = 8, data=None):
+
+def default_pack(dtype: str = "float32") -> int:
+    """types.hpp:70-72 default_pack_size<T> = 32 / sizeof(T)."""
+    return 8 if dtype == "float32" else 4
+
+
+class vector:
+    """Device-resident packed complex vector (pcxo::vector<T, P> layout), T = float or double."""
+
+    def __init__(self, n: int, pack: int | None = None, data=None, dtype: str | None = None):
+        if data is not None and dtype is None:
+            dtype = str(data.dtype).replace("torch.", "")
+        dtype = dtype or "float32"
+        if dtype not in DTYPES:
+            raise PcxError(_lib.ERR_INVALID_ARGUMENT, f"element type {dtype!r} is not float32 or float64")
+        pack = default_pack(dtype) if pack is None else pack
         if pack < 1 or pack & (pack - 1):
             raise PcxError(_lib.ERR_INVALID_PACK, f"pack size (which is {pack}) is not a power of two")
         torch = _torch()
         self.n = int(n)
         self.pack = int(pack)
+        self.dtype = dtype
         if data is None:
-            data = torch.zeros(real_size(self.n, self.pack), dtype=torch.float32, device="cuda")
+            data = torch.zeros(real_size(self.n, self.pack), dtype=getattr(torch, dtype), device="cuda")
         self.data = data
 
     def size(self) -> int:
@@ -71,14 +87,15 @@ class vector:
         return self.data.data_ptr()
 
     @classmethod
-    def from_complex(cls, z, pack: int = 8) -> "vector":
+    def from_complex(cls, z, pack: int | None = None, dtype: str = "float32") -> "vector":
         torch = _torch()
         z = np.asarray(z)
-        buf = np.zeros(real_size(len(z), pack), np.float32)
+        pack = default_pack(dtype) if pack is None else pack
+        buf = np.zeros(real_size(len(z), pack), np.dtype(dtype))
         idx = pidx(np.arange(len(z)), pack)
         buf[idx] = np.real(z)
         buf[idx + pack] = np.imag(z)
-        return cls(len(z), pack, torch.from_numpy(buf).cuda())
+        return cls(len(z), pack, torch.from_numpy(buf).cuda(), dtype)
 
     # vector_arithm.hpp:450-495: lazy expressions, evaluated on the GPU by store()
     def _expr(self):
@@ -120,17 +137,26 @@ class vector:
 
 
 class fft_unit:
-    """fft.hpp:565-915 fft_unit<float, Order> on the GPU."""
+    """fft.hpp:565-915 fft_unit<T, Order> on the GPU; dtype "float64" makes fft_unit<double, Order>."""
 
-    def __init__(self, fft_size: int, sub_size: int = 2048, order: str = "normal", precision: str = "fp32"):
+    def __init__(self, fft_size: int, sub_size: int = 2048, order: str = "normal", precision: str = "fp32",
+                 dtype: str = "float32"):
         if order not in ORDER:
             raise PcxError(_lib.ERR_INVALID_ARGUMENT, f"unknown fft_order {order!r}")
+        if dtype not in DTYPES:
+            raise PcxError(_lib.ERR_INVALID_ARGUMENT, f"element type {dtype!r} is not float32 or float64")
         import ctypes as C
 
         h = C.c_void_p()
-        check(lib().pcx_plan_create(int(fft_size), int(sub_size), ORDER[order], C.byref(h)))
+        create = lib().pcx_plan_create_f64 if dtype == "float64" else lib().pcx_plan_create
+        check(create(int(fft_size), int(sub_size), ORDER[order], C.byref(h)))
         self._h = h
         self.order = order
+        self.dtype = dtype
+        f64 = dtype == "float64"
+        self._fft = lib().pcx_fft_f64 if f64 else lib().pcx_fft
+        self._ifft = lib().pcx_ifft_f64 if f64 else lib().pcx_ifft
+        self._batch = lib().pcx_fft_batch_f64 if f64 else lib().pcx_fft_batch
         if precision != "fp32":
             check(lib().pcx_plan_set_precision(h, PRECISION[precision]))
         self.precision = precision
@@ -155,23 +181,29 @@ class fft_unit:
         src = dst if src is None else src
         src_pack = dst_pack if src_pack is None else src_pack
         src_size = self.size() if src_size is None else src_size
-        check(lib().pcx_fft(self._h, dst, dst_pack, src, src_pack, src_size, _stream_ptr(stream)))
+        check(self._fft(self._h, dst, dst_pack, src, src_pack, src_size, _stream_ptr(stream)))
 
     def ifft_raw(self, dst: int, src: int | None = None, normalized: bool = True, dst_pack: int = 1,
                  src_pack: int | None = None, stream=None) -> None:
         """ifft_raw<Normalized,PD,PS>(dst[, src]) (fft.hpp:689-721) on device pointers."""
         src = dst if src is None else src
         src_pack = dst_pack if src_pack is None else src_pack
-        check(lib().pcx_ifft(self._h, int(normalized), dst, dst_pack, src, src_pack, _stream_ptr(stream)))
+        check(self._ifft(self._h, int(normalized), dst, dst_pack, src, src_pack, _stream_ptr(stream)))
 
     def batch(self, dst: int, src: int, batch: int, dst_pack: int, dst_dist: int, src_pack: int,
               src_dist: int, inverse: bool = False, normalized: bool = True, src_size: int | None = None,
               stream=None) -> None:
         src_size = self.size() if src_size is None else src_size
-        check(lib().pcx_fft_batch(self._h, int(inverse), int(normalized), dst, dst_pack, dst_dist, src, src_pack,
-                                  src_dist, src_size, batch, _stream_ptr(stream)))
+        check(self._batch(self._h, int(inverse), int(normalized), dst, dst_pack, dst_dist, src, src_pack,
+                          src_dist, src_size, batch, _stream_ptr(stream)))
 
     # ---------------------------------------------------------------- vector forms
+    def _check_type(self, *vs: vector) -> None:
+        for v in vs:
+            if v.dtype != self.dtype:
+                raise PcxError(_lib.ERR_INVALID_ARGUMENT,
+                               f"vector element type {v.dtype} does not match the unit's {self.dtype}")
+
     def _check_dst(self, v: vector, what: str) -> None:
         if v.size() != self.size():
             raise PcxError(_lib.ERR_SIZE_MISMATCH, f"{what} size (which is {v.size()}) is not equal to fft size "
@@ -198,6 +230,7 @@ class fft_unit:
 
     def __call__(self, dst: vector, src: vector | None = None, stream=None) -> None:
         """operator()(v) in place, or operator()(dst, src) with src zero-extended (fft.hpp:728-820)."""
+        self._check_type(dst, *([] if src is None else [src]))
         if src is None:
             if dst.size() != self.size():
                 raise PcxError(_lib.ERR_SIZE_MISMATCH, f"input size (which is {dst.size()}) is not equal to fft "
@@ -212,6 +245,7 @@ class fft_unit:
 
     def ifft(self, dst: vector, src: vector | None = None, normalized: bool = True, stream=None) -> None:
         """ifft<Normalized>(v) / ifft<Normalized>(dst, src) (fft.hpp:828-915)."""
+        self._check_type(dst, *([] if src is None else [src]))
         if src is None:
             if dst.size() != self.size():
                 raise PcxError(_lib.ERR_SIZE_MISMATCH, f"input size (which is {dst.size()} is not equal to fft "

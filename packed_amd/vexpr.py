@@ -174,6 +174,12 @@ def compile_expr(expr: Expr):
     return prog, srcs
 
 
+def _float_only(*vs) -> None:
+    for v in vs:
+        if getattr(v, "dtype", "float32") != "float32":
+            raise PcxError(_lib.ERR_UNSUPPORTED, "vector expressions and repacking are built for float vectors")
+
+
 def store(expr, dst, stream=None) -> None:
     """pcxo::store(expr, dst) (vector_arithm.hpp:405-447): evaluate into dst in its pack size."""
     import ctypes as C
@@ -185,6 +191,7 @@ def store(expr, dst, stream=None) -> None:
         raise PcxError(_lib.ERR_SIZE_MISMATCH,
                        f"expression size (which is {expr.size()}) differs from destination size (which is {dst.size()})")
     prog, srcs = compile_expr(expr)
+    _float_only(dst, *srcs)
     S = (_lib.VSource * max(1, len(srcs)))(*[_lib.VSource(v.data_ptr(), v.pack) for v in srcs])
     P = (_lib.VInstr * len(prog))(*[_lib.VInstr(*i) for i in prog])
     check(lib().pcx_vexpr_eval(S, len(srcs), P, len(prog), dst.data_ptr(), dst.pack, dst.size(), _stream_ptr(stream)))
@@ -196,4 +203,5 @@ def repack(dst, src, stream=None) -> None:
 
     if src.size() != dst.size():
         raise PcxError(_lib.ERR_SIZE_MISMATCH, "repack needs equal sizes")
+    _float_only(dst, src)
     check(lib().pcx_repack(dst.data_ptr(), dst.pack, src.data_ptr(), src.pack, dst.size(), _stream_ptr(stream)))

@@ -65,3 +65,22 @@ def test_null_arguments():
     assert L.pcx_fxcorr_exec(None, None, 8, None, 8, 8, None) == _lib.ERR_INVALID_ARGUMENT
     assert L.pcx_plan_size(None) == 0
     assert L.pcx_version() >= 1
+
+
+def test_f64_plan_validation():
+    """fft_unit<double, Order> checks (fft.hpp:1806-1826 messages); refused orders and sizes fail
+    before any device work."""
+    from packed_amd import _lib
+
+    L = _lib.lib()
+    h = C.c_void_p()
+    assert L.pcx_plan_create_f64(1000, 2048, 0, C.byref(h)) == _lib.ERR_INVALID_SIZE
+    assert L.pcx_last_error().decode() == "fft_size (which is  1000) is not an integer power of two"
+    assert L.pcx_plan_create_f64(1024, 4, 0, C.byref(h)) == _lib.ERR_INVALID_SUB_SIZE
+    assert L.pcx_plan_create_f64(1024, 2048, 2, C.byref(h)) == _lib.ERR_UNSUPPORTED  # no double unordered
+    assert L.pcx_plan_create_f64(1 << 24, 2048, 0, C.byref(h)) == _lib.ERR_UNSUPPORTED
+    assert L.pcx_plan_create_f64(1024, 2048, 0, None) == _lib.ERR_INVALID_ARGUMENT
+    assert L.pcx_fft_f64(None, None, 4, None, 4, 4, None) == _lib.ERR_INVALID_ARGUMENT
+    assert L.pcx_ifft_f64(None, 1, None, 4, None, 4, None) == _lib.ERR_INVALID_ARGUMENT
+    assert L.pcx_fft_batch_f64(None, 0, 1, None, 4, 8, None, 4, 8, 4, 2, None) == _lib.ERR_INVALID_ARGUMENT
+    assert L.pcx_plan_element_size(None) == 0

@@ -84,6 +84,9 @@ def test_order_maps(orc):
         un = orc.slot_freq("unordered", n)
         assert sorted(un) == list(range(n)) and not np.array_equal(un, br)
         assert np.array_equal(orc.slot_freq("normal", n), np.arange(n))
+        for order in ["normal", "bit_reversed", "unordered"]:  # array form == scalar form
+            assert list(orc.slot_freq(order, n)) == [orc.lib().orc_slot_freq(orc.ORDERS[order], i, n)
+                                                     for i in range(n)]
 
 
 def test_pack_layout(orc):
