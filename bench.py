@@ -40,6 +40,8 @@ def parse():
     ap.add_argument("--batch", type=int, default=BATCH)
     ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16x3"],
                     help="arithmetic of the default workload (bf16x3: split-BF16 matrix-core butterflies)")
+    ap.add_argument("--dtype", default="float32", choices=["float32", "float64"],
+                    help="element type of the default workload (float64: the double-precision path)")
     ap.add_argument("--copy", action="store_true",
                     help="time a plain device copy of the same bytes instead (calibrates the HBM counters)")
     ap.add_argument("--config", type=int, default=2, choices=[1, 2, 3, 4, 5],
@@ -59,10 +61,11 @@ def profile_figure(name: str, key: str):
     return None
 
 
-def measured_traffic(log2n: int, batch: int):
+def measured_traffic(log2n: int, batch: int, dtype: str = "float32"):
     """HBM bytes per step from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this
     workload (counters scaled by a plain copy of the same bytes); None for other shapes."""
-    return profile_figure(f"fft_fourstep_n{1 << log2n}_b{batch}_hbm_traffic.md", "traffic_bytes_per_step")
+    tag = "" if dtype == "float32" else "f64_"
+    return profile_figure(f"fft_fourstep_{tag}n{1 << log2n}_b{batch}_hbm_traffic.md", "traffic_bytes_per_step")
 
 
 def time_steps(fn, steps, warmup):
@@ -142,17 +145,23 @@ def main():
     n = 1 << args.log2n
     batch = args.batch
     pts = n * batch
+    f64 = args.dtype == "float64"
+    if f64 and args.precision != "fp32":
+        raise SystemExit("bench.py: --precision applies to float32 data only")
+    esize = 8 if f64 else 4
     lib()  # load the HIP library (fails loudly if it is not built)
-    x = torch.empty(2 * pts, dtype=torch.float32, device="cuda")
-    y = torch.empty(2 * pts, dtype=torch.float32, device="cuda")
-    z = torch.empty(2 * pts, dtype=torch.float32, device="cuda")
+    tdt = torch.float64 if f64 else torch.float32
+    x = torch.empty(2 * pts, dtype=tdt, device="cuda")
+    y = torch.empty(2 * pts, dtype=tdt, device="cuda")
+    z = torch.empty(2 * pts, dtype=tdt, device="cuda")
     stream = torch.cuda.current_stream().cuda_stream
-    # standard-normal re/im; split layout: transform b = [re N floats][im N floats]
-    check(lib().pcx_generate(x.data_ptr(), n, pts, SEED, 1, 1.0, stream))
+    # standard-normal re/im; split layout: transform b = [re N values][im N values]
+    gen = lib().pcx_generate_f64 if f64 else lib().pcx_generate
+    check(gen(x.data_ptr(), n, pts, SEED, 1, 1.0, stream))
     torch.cuda.synchronize()
 
-    fwd = pa.fft_unit(n, order="normal", precision=args.precision)
-    fwd_br = pa.fft_unit(n, order="bit_reversed")
+    fwd = pa.fft_unit(n, order="normal", precision=args.precision, dtype=args.dtype)
+    fwd_br = pa.fft_unit(n, order="bit_reversed", dtype=args.dtype)
 
     def step():
         fwd.batch(y.data_ptr(), x.data_ptr(), batch, n, 2 * n, n, 2 * n)
@@ -172,7 +181,7 @@ def main():
 
     flops = 5.0 * n * math.log2(n) * batch
     gflops = flops / (ms * 1e-3) / 1e9
-    algo_bytes = 16.0 * pts
+    algo_bytes = 4.0 * esize * pts  # read and write one complex value per point
     achieved = algo_bytes / (ms * 1e-3) / 1e9
 
     # parity spot check on the timed outputs: transform 0 and the last one vs the FP64 oracle
@@ -190,9 +199,9 @@ def main():
     want_br = np.fft.fft(xin[:n].astype(np.float64) + 1j * xin[n:])[O.slot_freq("bit_reversed", n)]
     err_br = O.rel_l2(zin[:n] + 1j * zin[n:], want_br)
 
-    traffic = measured_traffic(args.log2n, batch) if args.precision == "fp32" else None
+    traffic = measured_traffic(args.log2n, batch, args.dtype) if args.precision == "fp32" else None
     out = {
-        "metric": "C2C FFT GFLOP/s (5N*log2N), N=2^16 batch 4096, FP32 split re/im",
+        "metric": f"C2C FFT GFLOP/s (5N*log2N), N=2^16 batch 4096, {'FP64' if f64 else 'FP32'} split re/im",
         "value": round(gflops, 1),
         "unit": "GFLOP/s",
         "n_gpus": 1,
@@ -202,11 +211,13 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32" if args.precision == "fp32" else "f32 (bf16x3 split operands, f32 accumulation)",
+        "dtype": "f64" if f64 else ("f32" if args.precision == "fp32" else
+                                    "f32 (bf16x3 split operands, f32 accumulation)"),
         "data": "synthetic (standard-normal re/im from splitmix64 seed 0x5EED0002, generated in HBM)",
         "config": {"workload": f"batched C2C FFT forward, N=2^{args.log2n}, batch {batch}, split re/im "
                                f"(pack=N), natural order, out of place",
-                   "fft_size": n, "batch": batch, "order": "normal", "precision": args.precision},
+                   "fft_size": n, "batch": batch, "order": "normal", "precision": args.precision,
+                   "element_type": args.dtype},
         "roofline": {
             "bound": "hbm",
             "achieved": round(achieved, 1),
@@ -221,7 +232,7 @@ def main():
                          "rel_l2_vs_fp64": float(f"{err_br:.3e}")},
         "rel_l2_vs_fp64": float(f"{max(errs):.3e}"),
     }
-    if not args.no_cpu:
+    if not args.no_cpu and not f64:  # the host-core port is FP32
         out["cpu_baseline"] = cpu_baseline(n, args.cpu_seconds)
     print(json.dumps(out))
 

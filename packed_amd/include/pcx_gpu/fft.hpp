@@ -4,9 +4,10 @@
 // as `void*` (a hipStream_t).
 //
 // Data lives in device memory in the packed split-complex layout of vector.hpp: complex i of a
-// pack-P vector has its real part at float offset i + (i / P) * P, its imaginary part P floats
-// later.  pcx_gpu::vector<T, P> is that container with a device allocator; raw-pointer members
-// (fft_raw / ifft_raw) take device pointers.
+// pack-P vector has its real part at element offset i + (i / P) * P, its imaginary part P
+// elements later.  pcx_gpu::vector<T, P> is that container with a device allocator; raw-pointer
+// members (fft_raw / ifft_raw) take device pointers.  T is float or double (fft_unit<double>
+// covers the normal and bit_reversed orders).
 //
 // Errors throw what the reference throws: std::invalid_argument with the reference's message for
 // a bad size, sub-size, size mismatch or in-place repack; std::runtime_error for a HIP failure and
@@ -46,6 +47,43 @@ inline void throw_on(int rc) {
 template<uZ P>
 concept pack_size = P > 0 && (P & (P - 1)) == 0;
 
+template<typename T>
+concept gpu_real = std::same_as<T, float> || std::same_as<T, double>;
+
+// types.hpp:70-72
+template<gpu_real T>
+constexpr uZ default_pack_size = 32 / sizeof(T);
+
+// The C ABI by element type.
+namespace detail {
+inline int plan_create(float*, int64_t n, int64_t sub, int order, pcx_plan_t* p) {
+    return pcx_plan_create(n, sub, order, p);
+}
+inline int plan_create(double*, int64_t n, int64_t sub, int order, pcx_plan_t* p) {
+    return pcx_plan_create_f64(n, sub, order, p);
+}
+inline int fft(pcx_plan_t p, float* d, int64_t dp, const float* s, int64_t sp, int64_t n, void* st) {
+    return pcx_fft(p, d, dp, s, sp, n, st);
+}
+inline int fft(pcx_plan_t p, double* d, int64_t dp, const double* s, int64_t sp, int64_t n, void* st) {
+    return pcx_fft_f64(p, d, dp, s, sp, n, st);
+}
+inline int ifft(pcx_plan_t p, int nz, float* d, int64_t dp, const float* s, int64_t sp, void* st) {
+    return pcx_ifft(p, nz, d, dp, s, sp, st);
+}
+inline int ifft(pcx_plan_t p, int nz, double* d, int64_t dp, const double* s, int64_t sp, void* st) {
+    return pcx_ifft_f64(p, nz, d, dp, s, sp, st);
+}
+inline int batch(pcx_plan_t p, int inv, int nz, float* d, int64_t dp, int64_t dd, const float* s, int64_t sp,
+                 int64_t sd, int64_t n, int64_t b, void* st) {
+    return pcx_fft_batch(p, inv, nz, d, dp, dd, s, sp, sd, n, b, st);
+}
+inline int batch(pcx_plan_t p, int inv, int nz, double* d, int64_t dp, int64_t dd, const double* s, int64_t sp,
+                 int64_t sd, int64_t n, int64_t b, void* st) {
+    return pcx_fft_batch_f64(p, inv, nz, d, dp, dd, s, sp, sd, n, b, st);
+}
+} // namespace detail
+
 // Allocator of device memory (the reference's Allocator parameter decides where buffers live,
 // allocators.hpp:11-48).
 template<typename T>
@@ -73,8 +111,8 @@ struct is_vexpr : std::false_type {};
 template<typename E, typename V>
 void store(const E& expr, V& dest);
 
-template<typename T, uZ PackSize = 8, typename Allocator = device_allocator<T>>
-    requires std::same_as<T, float> && pack_size<PackSize>
+template<typename T, uZ PackSize = default_pack_size<T>, typename Allocator = device_allocator<T>>
+    requires gpu_real<T> && pack_size<PackSize>
 class vector {
 public:
     using real_type      = T;
@@ -159,20 +197,26 @@ private:
     T*                                   m_ptr  = nullptr;
 };
 
-// The layout contract of cx_vector_traits (types.hpp:113-148): pack size, size and data.
+// The layout contract of cx_vector_traits (types.hpp:113-148): element type, pack size, size
+// and data.
 template<typename V>
 concept device_complex_vector = requires(V v, const V cv) {
+    typename V::real_type;
     { cv.size() } -> std::convertible_to<uZ>;
-    { v.data() } -> std::convertible_to<float*>;
+    { v.data() } -> std::convertible_to<typename V::real_type*>;
     V::pack_size;
 };
 
-// fft_unit<float, Order, Allocator> (fft.hpp:565-915) on the GPU.  The twiddle tables live in the
-// plan; copies share the (immutable) plan.  Calls are asynchronous on stream(); the raw forms take
-// device pointers.
+// fft_unit<T, Order, Allocator> (fft.hpp:565-915) on the GPU, T = float or double.  The twiddle
+// tables live in the plan; copies share the (immutable) plan.  Calls are asynchronous on
+// stream(); the raw forms take device pointers.
 template<typename T, fft_order Order = fft_order::normal, typename Allocator = device_allocator<T>>
-    requires std::same_as<T, float>
+    requires gpu_real<T>
 class fft_unit {
+    static_assert(std::same_as<T, float> || Order != fft_order::unordered,
+                  "the unordered order is defined for float only (the reference's unsorted kernels are "
+                  "float-only)");
+
 public:
     using real_type      = T;
     using allocator_type = Allocator;
@@ -194,32 +238,33 @@ public:
     template<uZ PackSize = 1>
         requires pack_size<PackSize>
     void fft_raw(T* data) {
-        throw_on(pcx_fft(m_plan.get(), data, PackSize, data, PackSize, size(), m_stream));
+        throw_on(detail::fft(m_plan.get(), data, PackSize, data, PackSize, size(), m_stream));
     }
     template<uZ PackSizeDest = 1, uZ PackSizeSrc = 1>
         requires pack_size<PackSizeDest> && pack_size<PackSizeSrc>
     void fft_raw(T* dest, const T* source) {
-        throw_on(pcx_fft(m_plan.get(), dest, PackSizeDest, source, PackSizeSrc, size(), m_stream));
+        throw_on(detail::fft(m_plan.get(), dest, PackSizeDest, source, PackSizeSrc, size(), m_stream));
     }
     template<uZ PackSizeDest = 1, uZ PackSizeSrc = 1>
         requires pack_size<PackSizeDest> && pack_size<PackSizeSrc>
     void fft_raw(T* dest, const T* source, uZ source_size) {
-        throw_on(pcx_fft(m_plan.get(), dest, PackSizeDest, source, PackSizeSrc, source_size, m_stream));
+        throw_on(detail::fft(m_plan.get(), dest, PackSizeDest, source, PackSizeSrc, source_size, m_stream));
     }
     // fft.hpp:689-721
     template<bool Normalized = true, uZ PackSize = 1>
         requires pack_size<PackSize>
     void ifft_raw(T* data) {
-        throw_on(pcx_ifft(m_plan.get(), Normalized, data, PackSize, data, PackSize, m_stream));
+        throw_on(detail::ifft(m_plan.get(), Normalized, data, PackSize, data, PackSize, m_stream));
     }
     template<bool Normalized = true, uZ PackSizeDest = 1, uZ PackSizeSrc = 1>
         requires pack_size<PackSizeDest> && pack_size<PackSizeSrc>
     void ifft_raw(T* dest, const T* source) {
-        throw_on(pcx_ifft(m_plan.get(), Normalized, dest, PackSizeDest, source, PackSizeSrc, m_stream));
+        throw_on(detail::ifft(m_plan.get(), Normalized, dest, PackSizeDest, source, PackSizeSrc, m_stream));
     }
 
     // fft.hpp:728-750
     template<device_complex_vector Vect_>
+        requires std::same_as<typename Vect_::real_type, T>
     void operator()(Vect_& vector) {
         if (vector.size() != size())
             throw std::invalid_argument("input size (which is " + std::to_string(vector.size()) +
@@ -228,6 +273,7 @@ public:
     }
     // fft.hpp:759-820 (source may be shorter: zero-extended)
     template<device_complex_vector DestVect_, device_complex_vector SrcVect_>
+        requires std::same_as<typename DestVect_::real_type, T> && std::same_as<typename SrcVect_::real_type, T>
     void operator()(DestVect_& dest, const SrcVect_& source) {
         if (dest.size() != size())
             throw std::invalid_argument("destination size (which is " + std::to_string(dest.size()) +
@@ -239,6 +285,7 @@ public:
     }
     // fft.hpp:828-853
     template<bool Normalized = true, device_complex_vector Vect_>
+        requires std::same_as<typename Vect_::real_type, T>
     void ifft(Vect_& vector) {
         if (vector.size() != size())
             throw std::invalid_argument("input size (which is " + std::to_string(vector.size()) +
@@ -247,6 +294,7 @@ public:
     }
     // fft.hpp:863-915
     template<bool Normalized = true, device_complex_vector DestVect_, device_complex_vector SrcVect_>
+        requires std::same_as<typename DestVect_::real_type, T> && std::same_as<typename SrcVect_::real_type, T>
     void ifft(DestVect_& dest, const SrcVect_& source) {
         if (dest.size() != size())
             throw std::invalid_argument("destination size (which is " + std::to_string(dest.size()) +
@@ -257,11 +305,11 @@ public:
         ifft_raw<Normalized, DestVect_::pack_size, SrcVect_::pack_size>(dest.data(), source.data());
     }
 
-    // Batched form: `batch` vectors at a distance of dist floats (device pointers).
+    // Batched form: `batch` vectors at a distance of dist elements (device pointers).
     template<uZ PackSizeDest, uZ PackSizeSrc>
     void batch(T* dest, uZ dest_dist, const T* source, uZ source_dist, uZ count, bool inverse = false,
                bool normalized = true) {
-        throw_on(pcx_fft_batch(m_plan.get(), inverse, normalized, dest, PackSizeDest, dest_dist, source, PackSizeSrc,
+        throw_on(detail::batch(m_plan.get(), inverse, normalized, dest, PackSizeDest, dest_dist, source, PackSizeSrc,
                                source_dist, size(), count, m_stream));
     }
 
@@ -271,7 +319,8 @@ private:
     };
     static std::shared_ptr<pcx_plan_s> make_plan(uZ n, uZ sub) {
         pcx_plan_t p = nullptr;
-        throw_on(pcx_plan_create(static_cast<int64_t>(n), static_cast<int64_t>(sub), static_cast<int>(Order), &p));
+        throw_on(detail::plan_create(static_cast<T*>(nullptr), static_cast<int64_t>(n), static_cast<int64_t>(sub),
+                                     static_cast<int>(Order), &p));
         return {p, plan_deleter{}};
     }
     std::shared_ptr<pcx_plan_s> m_plan;

@@ -59,9 +59,11 @@ struct is_node<sca<Op, E>> : std::true_type {};
 template<typename E>
 struct is_node<cj<E>> : std::true_type {};
 
-// any pcx_gpu::vector<float, P> or expression node
+// any pcx_gpu::vector<float, P> or expression node (expressions are evaluated in FP32)
 template<typename X>
-concept operand = is_node<std::remove_cvref_t<X>>::value || device_complex_vector<std::remove_cvref_t<X>>;
+concept operand = is_node<std::remove_cvref_t<X>>::value ||
+                  (device_complex_vector<std::remove_cvref_t<X>> &&
+                   std::same_as<typename std::remove_cvref_t<X>::real_type, float>);
 template<typename S>
 concept scalar = std::same_as<S, float> || std::same_as<S, std::complex<float>>;
 

@@ -334,6 +334,54 @@ static void test_fft_par() {
     CHECK(thrown, "fft_unit_par size check");
 }
 
+// fft_unit<double, Order> (the reference declares it with pack 4, types.hpp:70-72): forward in
+// normal and bit_reversed order against the FP64 oracle, in and out of place, and the round trip.
+template<size_t PackSize>
+static void test_fft_double(size_t size) {
+    std::vector<cxd> in(size);
+    for (size_t i = 0; i < size; ++i) // a leaky tone in double precision (test_fft.cpp:299-302)
+        in[i] = std::exp(cxd(0., 2 * M_PI * (double)i / (double)size * 13.37));
+    std::vector<double> xr(size), xi(size), yr(size), yi(size);
+    for (size_t i = 0; i < size; ++i)
+        xr[i] = in[i].real(), xi[i] = in[i].imag();
+    orc_fft(xr.data(), xi.data(), yr.data(), yi.data(), (int64_t)size, 0);
+    int bits = 0;
+    while (((size_t)1 << bits) < size)
+        ++bits;
+    const double tol = 4.0 * bits * std::numeric_limits<double>::epsilon();
+    auto         err = [&](const std::vector<cxd>& got, bool brev) {
+        double num = 0, den = 0;
+        for (size_t i = 0; i < size; ++i) {
+            size_t k = brev ? (size_t)orc_brev((int64_t)i, bits) : i;
+            num += std::norm(got[i] - cxd(yr[k], yi[k]));
+            den += yr[k] * yr[k] + yi[k] * yi[k];
+        }
+        return std::sqrt(num / den);
+    };
+    pcx_gpu::vector<double, PackSize>                      v(in);
+    pcx_gpu::vector<double, PackSize>                      out(size);
+    pcx_gpu::fft_unit<double>                              unit(size);
+    pcx_gpu::fft_unit<double, pcx_gpu::fft_order::bit_reversed> unit_br(size);
+    unit(out, v);
+    unit_br(v);
+    pcx_synchronize(nullptr);
+    double e1 = err(out.to_host(), false), e2 = err(v.to_host(), true);
+    CHECK(e1 < tol, "fft<double> size %zu pack %zu: rel L2 %.3g", size, PackSize, e1);
+    CHECK(e2 < tol, "fft<double, bit_reversed> size %zu pack %zu: rel L2 %.3g", size, PackSize, e2);
+    unit.ifft(out); // back to the input
+    unit_br.ifft(v);
+    pcx_synchronize(nullptr);
+    auto back1 = out.to_host(), back2 = v.to_host();
+    double n1 = 0, n2 = 0, d = 0;
+    for (size_t i = 0; i < size; ++i) {
+        n1 += std::norm(back1[i] - in[i]);
+        n2 += std::norm(back2[i] - in[i]);
+        d += std::norm(in[i]);
+    }
+    CHECK(std::sqrt(n1 / d) < 2 * tol && std::sqrt(n2 / d) < 2 * tol, "ifft<double> round trip size %zu: %.3g %.3g",
+          size, std::sqrt(n1 / d), std::sqrt(n2 / d));
+}
+
 int main() {
     for (size_t n = 64; n <= 16384; n *= 2) { // test_fft.cpp:779-798 runs 2^6 .. 2^14
         test_fft_float<8>(n);
@@ -342,6 +390,10 @@ int main() {
     }
     test_fft_float<8>(1 << 16);
     test_fftu_float<8>(1 << 16);
+    for (size_t n = 2; n <= (1 << 20); n *= 4) {
+        test_fft_double<4>(n);
+        test_fft_double<1>(2 * n);
+    }
     test_zero_extension();
     test_errors();
     test_fxcorr();

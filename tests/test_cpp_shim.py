@@ -19,6 +19,33 @@ def test_shim_header_compiles_standalone(tmp_path):
                     str(src)], check=True)
 
 
+def _syntax(tmp_path, body):
+    root = os.path.dirname(HERE)
+    src = tmp_path / "t.cpp"
+    src.write_text('#include "pcx_gpu/fft.hpp"\nint main(){ ' + body + ' }\n')
+    return subprocess.run(["g++", "-std=c++20", "-fsyntax-only", f"-I{root}/include", f"-I{root}/packed_amd/include",
+                           str(src)], capture_output=True, text=True)
+
+
+def test_double_unit_instantiates(tmp_path):
+    """fft_unit<double, Order> with the reference's default pack 4 (types.hpp:70-72)."""
+    body = ("pcx_gpu::fft_unit<double, pcx_gpu::fft_order::bit_reversed>* u = nullptr; "
+            "pcx_gpu::vector<double>* v = nullptr; if (u) { (*u)(*v); u->ifft(*v); } "
+            "static_assert(pcx_gpu::vector<double>::pack_size == 4 && pcx_gpu::vector<float>::pack_size == 8);")
+    res = _syntax(tmp_path, body)
+    assert res.returncode == 0, res.stderr
+
+
+def test_double_unordered_is_a_compile_error(tmp_path):
+    res = _syntax(tmp_path, "pcx_gpu::fft_unit<double, pcx_gpu::fft_order::unordered> u(1024);")
+    assert res.returncode != 0 and "unordered order is defined for float only" in res.stderr
+
+
+def test_mixed_element_types_are_a_compile_error(tmp_path):
+    res = _syntax(tmp_path, "pcx_gpu::fft_unit<float>* u = nullptr; pcx_gpu::vector<double>* v = nullptr; (*u)(*v);")
+    assert res.returncode != 0
+
+
 @pytest.mark.gpu
 def test_cpp_drop_in():
     assert os.path.exists(BIN), "build the C++ test first: python __graft_entry__.py"
