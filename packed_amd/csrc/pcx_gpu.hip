@@ -229,8 +229,11 @@ const std::array<FourStep, 20> kShapes = {
     shape<16, 10, 6, 16, 64>(), shape<16, 7, 9, 64, 32>(), shape<16, 6, 10, 64, 16>(),
     shape<16, 9, 7, 32, 32>(), shape<16, 10, 6, 16, 32>(),
 };
-// FP64 shapes, 2^14 .. 2^23: LDS of about 64-80 KiB per workgroup (two workgroups per CU), 256
-// threads; C and G are 16 doubles (128-byte segments) up to 2^16, fewer above.  Pass-twiddle
+// FP64 shapes, 2^14 .. 2^23, 256-512 threads per workgroup.  C and G (doubles per column segment
+// and per row store segment) are 16 up to 2^16 with about 64-80 KiB of LDS per workgroup (two
+// workgroups per CU).  From 2^17 the defaults are the measured best of the tuning sweeps on
+// MI355X (2 GiB batches, forward): wider segments win over occupancy, e.g. 2^20 with C = G = 8
+// and 129-133 KiB per workgroup runs in 2.43 ms against 3.23 ms for C = 8, G = 2.  Pass-twiddle
 // tables above 512 entries are read from global memory.
 template<int LG>
 constexpr int tw_cap64() {
@@ -248,10 +251,31 @@ FourStepT<double> shape64() {
              &launch_row<N1, G, LD_PERM, ST_CONTIG, TW, true, ORD_BITREV, PREC_FP32, D>, nullptr},
             {&launch_col<N2, C, 0, PREC_FP32, D>, &launch_col<N2, C, TW, PREC_FP32, D>}};
 }
-const std::array<FourStepT<double>, 10> kShapes64 = {
-    shape64<14, 7, 7, 32, 32>(),  shape64<15, 8, 7, 16, 32>(), shape64<16, 8, 8, 16, 16>(), shape64<17, 9, 8, 8, 16>(),
-    shape64<18, 10, 8, 4, 16>(),  shape64<19, 10, 9, 4, 8>(),  shape64<20, 11, 9, 2, 8>(),  shape64<21, 12, 9, 2, 8>(),
-    shape64<22, 13, 9, 1, 8>(),   shape64<23, 13, 10, 1, 4>()};
+const std::array<FourStepT<double>, 20> kShapes64 = {
+    // defaults (first match per lg); forward ms for a 2 GiB batch measured on MI355X
+    shape64<14, 7, 7, 32, 32>(),  shape64<15, 8, 7, 16, 32>(), shape64<16, 8, 8, 16, 16>(),
+    shape64<17, 8, 9, 16, 8>(),   // 1.92 ms
+    shape64<18, 10, 8, 8, 16>(),  // 2.04
+    shape64<19, 10, 9, 8, 8>(),   // 2.52
+    shape64<20, 10, 10, 8, 8>(),  // 2.43
+    shape64<21, 12, 9, 2, 8>(),   shape64<22, 13, 9, 1, 8>(),  shape64<23, 13, 10, 1, 4>(),
+    // alternatives explored by the sweeps
+    shape64<17, 9, 8, 8, 16>(),   // 2.08
+    shape64<17, 9, 8, 16, 16>(),  // 1.90 (inverse 2.05)
+    shape64<18, 10, 8, 4, 16>(),  // 2.71
+    shape64<18, 9, 9, 8, 8>(),    // 2.22
+    shape64<18, 8, 10, 16, 4>(),  // 2.15
+    shape64<19, 10, 9, 4, 8>(),   // 3.25
+    shape64<19, 9, 10, 8, 4>(),   // 2.81
+    shape64<20, 11, 9, 2, 8>(),   // 3.23
+    shape64<20, 10, 10, 4, 4>(),  // 3.15
+    shape64<20, 11, 9, 4, 8>()};  // 2.63
+const FourStepT<double>* find_shape64(int lg, int n2, int g, int c) {
+    for (const auto& f : kShapes64)
+        if (f.lg == lg && (n2 == 0 || f.n2 == n2) && (g == 0 || f.g == g) && (c == 0 || f.c == c))
+            return &f;
+    return nullptr;
+}
 
 // split-BF16 matrix-core variants of the default shapes for 2^15 .. 2^20 (natural order):
 // rows {forward, inverse}, cols {inverse, forward}
@@ -484,9 +508,16 @@ extern "C" int pcx_plan_create_f64(int64_t fft_size, int64_t sub_size, int order
             he = gen_twiddles(p->tw64_row, fft_size, 1, fft_size);
     } else {
         p->two_pass = true;
-        for (const auto& f : kShapes64)
-            if (f.lg == lg && !p->shape64)
-                p->shape64 = &f;
+        // PCX_N2_LOG2 and PCX_TILE ("C,G") select another instantiated shape (tuning sweeps)
+        int n2s = 0, c = 0, g = 0;
+        if (const char* env = std::getenv("PCX_N2_LOG2"))
+            n2s = std::atoi(env);
+        if (const char* env = std::getenv("PCX_TILE"))
+            if (std::sscanf(env, "%d,%d", &c, &g) != 2)
+                c = g = 0;
+        p->shape64 = find_shape64(lg, n2s, g, c);
+        if (!p->shape64)
+            p->shape64 = find_shape64(lg, 0, 0, 0);
         p->n1_lg = p->shape64->n1;
         p->n2_lg = p->shape64->n2;
         p->tw_ls = (lg + 1) / 2;
