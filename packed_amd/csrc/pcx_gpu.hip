@@ -799,8 +799,10 @@ int fft_batch(pcx_plan_t p, int inverse, int normalized, T* dst, int64_t dst_pac
             if (inverse && sp < 8)
                 sp = 8;
         }
+        // a forward source holds src_size values (zero-extended to the fft size)
+        const int64_t src_n = inverse ? p->n : std::min(std::max<int64_t>(src_size, 0), p->n);
         if (is_pow2(dp) && is_pow2(sp) &&
-            (dst_dist < footprint(p->n, dp) || src_dist < footprint(p->n, sp)))
+            (dst_dist < footprint(p->n, dp) || src_dist < footprint(src_n, sp)))
             return fail(PCX_ERR_INVALID_ARGUMENT, "batch distance is smaller than one vector");
         if ((const T*)dst == src && dst_dist != src_dist)
             return fail(PCX_ERR_INVALID_ARGUMENT, "in-place batch needs equal source and destination distances");

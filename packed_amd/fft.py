@@ -20,7 +20,7 @@ import numpy as np
 from . import _lib
 from ._lib import ORDER, PRECISION, PcxError, check, lib
 
-__all__ = ["vector", "fft_unit", "fxcorr_unit", "fft_queue", "PcxError", "real_size", "pidx"]
+__all__ = ["vector", "md_storage", "fft_unit", "fxcorr_unit", "fft_queue", "PcxError", "real_size", "pidx"]
 
 
 def _torch():
@@ -136,6 +136,53 @@ class vector:
         return buf[idx].astype(np.float64) + 1j * buf[idx + pack].astype(np.float64)
 
 
+def md_row_stride(inner: int, pack: int, alignment: int) -> int:
+    """Reals between slices of a two-axis md::dynamic_storage (calc_storage_size,
+    mdstorage.hpp:879-885): ceil(2*inner / A) * A with A = lcm(2*pack, alignment)."""
+    import math
+
+    a = math.lcm(2 * pack, alignment)
+    return (2 * inner + a - 1) // a * a
+
+
+class md_storage:
+    """Two-axis md::dynamic_storage<T, basis, P, Alignment> (mdstorage.hpp:1478-1499) in device
+    memory: `outer` slices of `inner` packed complex values, row_stride reals apart, with
+    row_stride = ceil(2*inner / A)*A, A = lcm(2P, Alignment) (calc_storage_size,
+    mdstorage.hpp:879-885); zero-filled.  storage[i] is slice i as a `vector` view (no copy).
+    Extents are in layout order: the inner (contiguous) axis first."""
+
+    def __init__(self, inner: int, outer: int, pack: int | None = None, alignment: int | None = None,
+                 dtype: str = "float32"):
+        if dtype not in DTYPES:
+            raise PcxError(_lib.ERR_INVALID_ARGUMENT, f"element type {dtype!r} is not float32 or float64")
+        pack = default_pack(dtype) if pack is None else pack
+        alignment = default_pack(dtype) if alignment is None else alignment
+        for what, v in (("pack", pack), ("alignment", alignment)):
+            if v < 1 or v & (v - 1):
+                raise PcxError(_lib.ERR_INVALID_PACK, f"{what} size (which is {v}) is not a power of two")
+        torch = _torch()
+        self.inner, self.outer, self.pack, self.dtype = int(inner), int(outer), int(pack), dtype
+        self.row_stride = md_row_stride(self.inner, self.pack, int(alignment))
+        self.data = torch.zeros(self.row_stride * self.outer, dtype=getattr(torch, dtype), device="cuda")
+
+    def size(self) -> int:
+        """Extent of the outer axis (mdstorage.hpp:1466-1468)."""
+        return self.outer
+
+    def __len__(self) -> int:
+        return self.outer
+
+    def data_ptr(self) -> int:
+        return self.data.data_ptr()
+
+    def __getitem__(self, i: int) -> vector:
+        if not 0 <= i < self.outer:
+            raise IndexError(i)
+        view = self.data[i * self.row_stride: i * self.row_stride + real_size(self.inner, self.pack)]
+        return vector(self.inner, self.pack, view, self.dtype)
+
+
 class fft_unit:
     """fft.hpp:565-915 fft_unit<T, Order> on the GPU; dtype "float64" makes fft_unit<double, Order>."""
 
@@ -228,8 +275,29 @@ class fft_unit:
                                    sp.data_ptr(), src_vecs[0].pack, length, _stream_ptr(stream)))
         self._across_tables = (sp, dp)  # kept alive until the next call (asynchronous launch)
 
+    def _storage_call(self, dst: md_storage, src: md_storage | None, inverse: bool, normalized: bool,
+                      stream) -> None:
+        """Every slice of a batch storage in one launch (the rows of a two-axis md::dynamic_storage)."""
+        src = dst if src is None else src
+        self._check_type(dst, src)
+        if dst.inner != self.size():
+            raise PcxError(_lib.ERR_SIZE_MISMATCH, f"destination size (which is {dst.inner}) is not equal to fft "
+                                                   f"size (which is {self.size()})")
+        if src.inner > self.size() or (inverse and src.inner != self.size()):
+            raise PcxError(_lib.ERR_SIZE_MISMATCH, f"source size (which is {src.inner}) does not fit fft size "
+                                                   f"(which is {self.size()})")
+        if dst.outer != src.outer:
+            raise PcxError(_lib.ERR_SIZE_MISMATCH, f"destination slice count (which is {dst.outer}) is not equal to "
+                                                   f"source slice count (which is {src.outer})")
+        self.batch(dst.data_ptr(), src.data_ptr(), dst.outer, dst.pack, dst.row_stride, src.pack, src.row_stride,
+                   inverse=inverse, normalized=normalized, src_size=src.inner, stream=stream)
+
     def __call__(self, dst: vector, src: vector | None = None, stream=None) -> None:
-        """operator()(v) in place, or operator()(dst, src) with src zero-extended (fft.hpp:728-820)."""
+        """operator()(v) in place, or operator()(dst, src) with src zero-extended (fft.hpp:728-820).
+        With md_storage arguments every slice is transformed in one launch."""
+        if isinstance(dst, md_storage):
+            self._storage_call(dst, src, False, True, stream)
+            return
         self._check_type(dst, *([] if src is None else [src]))
         if src is None:
             if dst.size() != self.size():
@@ -244,7 +312,10 @@ class fft_unit:
         self.fft_raw(dst.data_ptr(), src.data_ptr(), dst.pack, src.pack, src.size(), stream)
 
     def ifft(self, dst: vector, src: vector | None = None, normalized: bool = True, stream=None) -> None:
-        """ifft<Normalized>(v) / ifft<Normalized>(dst, src) (fft.hpp:828-915)."""
+        """ifft<Normalized>(v) / ifft<Normalized>(dst, src) (fft.hpp:828-915); md_storage: every slice."""
+        if isinstance(dst, md_storage):
+            self._storage_call(dst, src, True, normalized, stream)
+            return
         self._check_type(dst, *([] if src is None else [src]))
         if src is None:
             if dst.size() != self.size():

@@ -199,8 +199,12 @@ private:
 
 // The layout contract of cx_vector_traits (types.hpp:113-148): element type, pack size, size
 // and data.
+// A batch storage (pcx_gpu/mdstorage.hpp) is not a vector: its size() is the outer extent.
+template<typename S>
+concept md_batch = requires { typename S::md_storage_tag; };
+
 template<typename V>
-concept device_complex_vector = requires(V v, const V cv) {
+concept device_complex_vector = !md_batch<V> && requires(V v, const V cv) {
     typename V::real_type;
     { cv.size() } -> std::convertible_to<uZ>;
     { v.data() } -> std::convertible_to<typename V::real_type*>;
@@ -305,6 +309,44 @@ public:
         ifft_raw<Normalized, DestVect_::pack_size, SrcVect_::pack_size>(dest.data(), source.data());
     }
 
+    // Every slice of a batch storage (pcx_gpu/mdstorage.hpp, the layout of a two-axis
+    // md::dynamic_storage) in one launch: the slices are the rows of the batch, row_stride()
+    // reals apart.  The reference transforms md slices one call at a time.
+    template<md_batch S>
+        requires std::same_as<typename S::real_type, T>
+    void operator()(S& storage) {
+        check_inner(storage.inner_extent(), "input");
+        batch<S::pack_size, S::pack_size>(storage.data(), storage.row_stride(), storage.data(), storage.row_stride(),
+                                          storage.size());
+    }
+    template<md_batch D, md_batch S>
+        requires std::same_as<typename D::real_type, T> && std::same_as<typename S::real_type, T>
+    void operator()(D& dest, const S& source) {
+        check_inner(dest.inner_extent(), "destination");
+        check_batch(dest.size(), source.size());
+        if (source.inner_extent() > size())
+            throw std::invalid_argument("source size (which is " + std::to_string(source.inner_extent()) +
+                                        ") is bigger than fft size (which is " + std::to_string(size()) + ")");
+        throw_on(detail::batch(m_plan.get(), 0, 0, dest.data(), D::pack_size, dest.row_stride(), source.data(),
+                               S::pack_size, source.row_stride(), source.inner_extent(), dest.size(), m_stream));
+    }
+    template<bool Normalized = true, md_batch S>
+        requires std::same_as<typename S::real_type, T>
+    void ifft(S& storage) {
+        check_inner(storage.inner_extent(), "input");
+        batch<S::pack_size, S::pack_size>(storage.data(), storage.row_stride(), storage.data(), storage.row_stride(),
+                                          storage.size(), true, Normalized);
+    }
+    template<bool Normalized = true, md_batch D, md_batch S>
+        requires std::same_as<typename D::real_type, T> && std::same_as<typename S::real_type, T>
+    void ifft(D& dest, const S& source) {
+        check_inner(dest.inner_extent(), "destination");
+        check_inner(source.inner_extent(), "source");
+        check_batch(dest.size(), source.size());
+        batch<D::pack_size, S::pack_size>(dest.data(), dest.row_stride(), source.data(), source.row_stride(),
+                                          dest.size(), true, Normalized);
+    }
+
     // Batched form: `batch` vectors at a distance of dist elements (device pointers).
     template<uZ PackSizeDest, uZ PackSizeSrc>
     void batch(T* dest, uZ dest_dist, const T* source, uZ source_dist, uZ count, bool inverse = false,
@@ -314,6 +356,16 @@ public:
     }
 
 private:
+    void check_inner(uZ n, const char* what) const {
+        if (n != size())
+            throw std::invalid_argument(std::string(what) + " size (which is " + std::to_string(n) +
+                                        ") is not equal to fft size (which is " + std::to_string(size()) + ")");
+    }
+    static void check_batch(uZ a, uZ b) {
+        if (a != b)
+            throw std::invalid_argument("destination slice count (which is " + std::to_string(a) +
+                                        ") is not equal to source slice count (which is " + std::to_string(b) + ")");
+    }
     struct plan_deleter {
         void operator()(pcx_plan_s* p) const noexcept { (void)pcx_plan_destroy(p); }
     };

@@ -3,6 +3,7 @@
 // tests/test_fxcorr.cpp (61-139), with the FP64 oracle (oracle/pcx_oracle.h) as the checker.
 // Exit code = number of failed checks.
 #include "pcx_gpu/fft.hpp"
+#include "pcx_gpu/mdstorage.hpp"
 #include "pcx_oracle.h"
 
 #include <cmath>
@@ -382,6 +383,102 @@ static void test_fft_double(size_t size) {
           size, std::sqrt(n1 / d), std::sqrt(n2 / d));
 }
 
+// Batches as two-axis md::dynamic_storage (mdstorage.hpp:1478-1499): the row stride follows
+// calc_storage_size (mdstorage.hpp:879-885); one fft_unit call transforms every slice, with the
+// same results as one call per slice, and zero-extends from a shorter source storage.
+static void test_md_storage() {
+    using pcx_gpu::md::dynamic_storage;
+    // stride = ceil(2 * inner / lcm(2P, A)) * lcm(2P, A), values worked by hand
+    CHECK((dynamic_storage<float, 8, 8>(1001, 0).row_stride() == 2016), "stride 1001/8/8");
+    CHECK((dynamic_storage<float, 1, 8>(5, 0).row_stride() == 16), "stride 5/1/8");
+    CHECK((dynamic_storage<float, 16, 8>(17, 0).row_stride() == 64), "stride 17/16/8");
+    CHECK((dynamic_storage<double, 4, 4>(3, 0).row_stride() == 8), "stride 3/4/4");
+    CHECK((dynamic_storage<double, 2, 32>(33, 0).row_stride() == 96), "stride 33/2/32");
+
+    const size_t n = 1024, rows = 37, short_n = 1001;
+    dynamic_storage<float, 8, 8> src(short_n, rows); // padded rows: 2016 reals for 2002 used
+    dynamic_storage<float, 8, 8> dst(n, rows);
+    std::vector<std::vector<cxf>> host(rows);
+    for (size_t r = 0; r < rows; ++r) {
+        std::vector<float> re(short_n), im(short_n);
+        orc_gen_uniform(0x5EED0700 + r, re.data(), im.data(), (int64_t)short_n);
+        host[r].resize(short_n);
+        for (size_t i = 0; i < short_n; ++i)
+            host[r][i] = {re[i], im[i]};
+        src.assign(r, host[r].data(), short_n);
+    }
+    pcx_gpu::fft_unit<float> unit(n);
+    unit(dst, src); // every slice, zero-extended
+    pcx_synchronize(nullptr);
+    double worst = 0;
+    for (size_t r = 0; r < rows; ++r) {
+        std::vector<cxf> in(n, cxf(0, 0));
+        std::copy(host[r].begin(), host[r].end(), in.begin());
+        worst = std::max(worst, rel_l2(dst.to_host(r), oracle_fft(0, false, true, in)));
+    }
+    CHECK(worst < 2 * 10 * std::numeric_limits<float>::epsilon(), "md storage batch vs oracle: %.3g", worst);
+    // the batched call equals one call per slice, bit for bit
+    dynamic_storage<float, 8, 8> one(n, rows);
+    for (size_t r = 0; r < rows; ++r) {
+        auto d = one[r];
+        auto s = src[r];
+        unit(d, s);
+    }
+    pcx_synchronize(nullptr);
+    bool same = true;
+    for (size_t r = 0; r < rows; ++r)
+        same = same && one.to_host(r) == dst.to_host(r);
+    CHECK(same, "md storage batch differs from per-slice calls");
+    unit.ifft(dst); // back, in place, every slice
+    pcx_synchronize(nullptr);
+    worst = 0;
+    for (size_t r = 0; r < rows; ++r) {
+        auto back = dst.to_host(r);
+        double num = 0, den = 0;
+        for (size_t i = 0; i < n; ++i) {
+            cxd want = i < short_n ? cxd(host[r][i]) : cxd(0, 0);
+            num += std::norm(cxd(back[i]) - want);
+            den += std::norm(want);
+        }
+        worst = std::max(worst, std::sqrt(num / den));
+    }
+    CHECK(worst < 4 * 10 * std::numeric_limits<float>::epsilon(), "md storage round trip: %.3g", worst);
+    // double slices through the four-step
+    const size_t nd = 1 << 16, rd = 3;
+    dynamic_storage<double> sd(nd, rd);
+    std::vector<std::vector<cxd>> hd(rd, std::vector<cxd>(nd));
+    for (size_t r = 0; r < rd; ++r) {
+        for (size_t i = 0; i < nd; ++i)
+            hd[r][i] = std::exp(cxd(0., 2 * M_PI * (double)i * (13.37 + (double)r) / (double)nd));
+        sd.assign(r, hd[r].data(), nd);
+    }
+    pcx_gpu::fft_unit<double> ud(nd);
+    ud(sd);
+    pcx_synchronize(nullptr);
+    for (size_t r = 0; r < rd; ++r) {
+        std::vector<double> xr(nd), xi(nd), yr(nd), yi(nd);
+        for (size_t i = 0; i < nd; ++i)
+            xr[i] = hd[r][i].real(), xi[i] = hd[r][i].imag();
+        orc_fft(xr.data(), xi.data(), yr.data(), yi.data(), (int64_t)nd, 0);
+        auto   got = sd.to_host(r);
+        double num = 0, den = 0;
+        for (size_t i = 0; i < nd; ++i) {
+            num += std::norm(got[i] - cxd(yr[i], yi[i]));
+            den += yr[i] * yr[i] + yi[i] * yi[i];
+        }
+        CHECK(std::sqrt(num / den) < 4 * 16 * std::numeric_limits<double>::epsilon(), "md storage<double> row %zu: %.3g",
+              r, std::sqrt(num / den));
+    }
+    bool threw = false;
+    try {
+        dynamic_storage<float> wrong(512, 2);
+        unit(wrong);
+    } catch (const std::invalid_argument&) {
+        threw = true;
+    }
+    CHECK(threw, "slice size mismatch must throw");
+}
+
 int main() {
     for (size_t n = 64; n <= 16384; n *= 2) { // test_fft.cpp:779-798 runs 2^6 .. 2^14
         test_fft_float<8>(n);
@@ -394,6 +491,7 @@ int main() {
         test_fft_double<4>(n);
         test_fft_double<1>(2 * n);
     }
+    test_md_storage();
     test_zero_extension();
     test_errors();
     test_fxcorr();
