@@ -295,6 +295,15 @@ __device__ __forceinline__ void row_group(const RowArgsT<T>& p, int64_t wg, T* l
     // transform of this thread's row; with UNI the same for the whole wavefront
     const int64_t  btu    = (UNI && TA < 64) ? (row0 >> p.rows_lg) : bt;
     auto           addr   = [&](int b, int o) { return g * STR + b + (b >> 5) + o + (o >> 5); };
+    // The permuted loads and stores exchange through LDS with one pad per 2^PP elements,
+    // PP = max(LA - 6, 5).  In the permuted order lane jt touches element brev(jt) * PPT + c,
+    // and the elements of a wave's 64 lanes differ by multiples of 2^(LA - 6): with this pad
+    // they fall in 64 distinct banks from 2^11 on (one pad per 32 gives 2-, 4- and 8-way
+    // conflicts at 2^12, 2^13, 2^14; 2^10 keeps it and two-way conflicts, as the row stride is
+    // sized for one pad per 32).  Offsets stay carry-free: every offset added to a base is a
+    // multiple of 2^PP, or smaller than the room left below the next multiple.
+    constexpr int  PP     = LA - 6 > 5 ? LA - 6 : 5;
+    auto           paddr  = [&](int b, int o) { return g * STR + b + (b >> PP) + o + (o >> PP); };
 
     cx<T> v[PPT];
     // ------------------------------------------------------------------ load
@@ -337,9 +346,9 @@ __device__ __forceinline__ void row_group(const RowArgsT<T>& p, int64_t wg, T* l
         auto pos = [&](int u) {
             if constexpr (LD == LD_PERM) {
                 if constexpr (TA >= 32)
-                    return addr(perm_base<LA, ORD, T>(jt), brevc(u, ilog2c(PPT)));
+                    return paddr(perm_base<LA, ORD, T>(jt), brevc(u, ilog2c(PPT)));
                 else
-                    return addr(perm_elem<LA, ORD, T>(jt, u), 0);
+                    return paddr(perm_elem<LA, ORD, T>(jt, u), 0);
             } else {
                 int idx = t + u * NT;
                 int k1  = idx / G;
@@ -367,35 +376,46 @@ __device__ __forceinline__ void row_group(const RowArgsT<T>& p, int64_t wg, T* l
                 v[u]        = ok ? x : cx<T>{T(0), T(0)};
             }
         }
+        // the first stage's layout, in the padding that pos() wrote
+        auto rd = [&](int b, int o) {
+            if constexpr (LD == LD_PERM)
+                return paddr(b, o);
+            else
+                return addr(b, o);
+        };
 #pragma unroll
         for (int u = 0; u < PPT; ++u)
             lre[pos(u)] = v[u].x;
         __syncthreads();
-        stage_read<LA, false, 0, 0>(v, lre, lim, jt, addr);
+        stage_read<LA, false, 0, 0>(v, lre, lim, jt, rd);
         __syncthreads();
 #pragma unroll
         for (int u = 0; u < PPT; ++u)
             lre[pos(u)] = v[u].y;
         __syncthreads();
-        stage_read<LA, false, 0, 1>(v, lre, lim, jt, addr);
+        stage_read<LA, false, 0, 1>(v, lre, lim, jt, rd);
     } else {
         if constexpr (LD == LD_PERM) {
             // this row's spectrum occupies slots [A*brev(k2), A*brev(k2) + A); local slot s holds
-            // element slot_freq(order, s)
+            // element slot_freq(order, s).  All loads first, then the LDS writes: interleaving
+            // them let the compiler wait for each load before the next one (2^11: 1.27 ms per
+            // 2 GiB against 0.76 for the forward transform).
             const RdView<UNI, T> src(tbase(p.src, p.src_offs, p.src_dist, btu), p.src_lp);
             uint32_t          base = brev_bits(k2, p.rows_lg) << LA;
 #pragma unroll
             for (int u = 0; u < PPT; ++u) {
-                int   s  = jt + u * TA;
-                cx<T> x  = src.ld(row_ok ? poff(base + s, p.src_lp) : 0u, p.nt_load);
-                x      = row_ok ? x : cx<T>{T(0), T(0)};
+                cx<T> x = src.ld(row_ok ? poff(base + jt + u * TA, p.src_lp) : 0u, p.nt_load);
+                v[u]    = row_ok ? x : cx<T>{T(0), T(0)};
+            }
+#pragma unroll
+            for (int u = 0; u < PPT; ++u) {
                 int o;
                 if constexpr (TA >= 32)
-                    o = addr(perm_base<LA, ORD, T>(jt), brevc(u, ilog2c(PPT)));
+                    o = paddr(perm_base<LA, ORD, T>(jt), brevc(u, ilog2c(PPT)));
                 else
-                    o = addr(perm_elem<LA, ORD, T>(jt, u), 0);
-                lre[o] = x.x;
-                lim[o] = x.y;
+                    o = paddr(perm_elem<LA, ORD, T>(jt, u), 0);
+                lre[o] = v[u].x;
+                lim[o] = v[u].y;
             }
         } else { // LD_STRIDED: element k1 of row k2 sits at slot k2 + (k1 << rows_lg)
             // (the rows of a workgroup lie in one transform: G divides the rows per transform)
@@ -420,7 +440,7 @@ __device__ __forceinline__ void row_group(const RowArgsT<T>& p, int64_t wg, T* l
         for (int m = 0; m < PPT / R0; ++m)
 #pragma unroll
             for (int r = 0; r < R0; ++r) {
-                int o         = addr(jt + m * TA, r * (P::A / R0));
+                int o         = LD == LD_PERM ? paddr(jt + m * TA, r * (P::A / R0)) : addr(jt + m * TA, r * (P::A / R0));
                 v[m * R0 + r] = cx<T>{lre[o], lim[o]};
             }
     }
@@ -483,9 +503,9 @@ __device__ __forceinline__ void row_group(const RowArgsT<T>& p, int64_t wg, T* l
         auto pos = [&](int u) {
             if constexpr (ST == ST_PERM) {
                 if constexpr (TA >= 32)
-                    return addr(perm_base<LA, ORD, T>(jt), brevc(u, ilog2c(PPT)));
+                    return paddr(perm_base<LA, ORD, T>(jt), brevc(u, ilog2c(PPT)));
                 else
-                    return addr(perm_elem<LA, ORD, T>(jt, u), 0);
+                    return paddr(perm_elem<LA, ORD, T>(jt, u), 0);
             } else {
                 int idx = t + u * NT;
                 int k1  = idx / G;
@@ -496,8 +516,10 @@ __device__ __forceinline__ void row_group(const RowArgsT<T>& p, int64_t wg, T* l
 #pragma unroll
             for (int m = 0; m < PPT / RL; ++m)
 #pragma unroll
-                for (int r = 0; r < RL; ++r)
-                    lre[addr(jt + m * TA, r * (P::A / RL))] = decltype(comp)::value ? v[m * RL + r].y : v[m * RL + r].x;
+                for (int r = 0; r < RL; ++r) {
+                    const int o = ST == ST_PERM ? paddr(jt + m * TA, r * (P::A / RL)) : addr(jt + m * TA, r * (P::A / RL));
+                    lre[o]      = decltype(comp)::value ? v[m * RL + r].y : v[m * RL + r].x;
+                }
         };
         __syncthreads(); // the last stage's LDS reads are done
         put(std::false_type{});
@@ -536,7 +558,7 @@ __device__ __forceinline__ void row_group(const RowArgsT<T>& p, int64_t wg, T* l
         for (int m = 0; m < PPT / RL; ++m)
 #pragma unroll
             for (int r = 0; r < RL; ++r) {
-                int o  = addr(jt + m * TA, r * (P::A / RL));
+                int o  = ST == ST_PERM ? paddr(jt + m * TA, r * (P::A / RL)) : addr(jt + m * TA, r * (P::A / RL));
                 lre[o] = v[m * RL + r].x;
                 lim[o] = v[m * RL + r].y;
             }
@@ -550,9 +572,9 @@ __device__ __forceinline__ void row_group(const RowArgsT<T>& p, int64_t wg, T* l
                     int s = jt + u * TA;
                     int o;
                     if constexpr (TA >= 32)
-                        o = addr(perm_base<LA, ORD, T>(jt), brevc(u, ilog2c(PPT)));
+                        o = paddr(perm_base<LA, ORD, T>(jt), brevc(u, ilog2c(PPT)));
                     else
-                        o = addr(perm_elem<LA, ORD, T>(jt, u), 0);
+                        o = paddr(perm_elem<LA, ORD, T>(jt, u), 0);
                     dst.st(poff(base + s, p.dst_lp), cx<T>{lre[o], lim[o]}, p.nt_store);
                 }
             }
@@ -742,12 +764,16 @@ void k_dif(RowArgs p) {
             }
         }
     } else {
+        // the permuted exchange's padding (one pad per 2^(LM - 6) elements: conflict-free in
+        // both orders, as in row_group)
+        constexpr int PP    = LM - 6 > 5 ? LM - 6 : 5;
+        auto          paddr = [&](int bb, int o) { return bb + (bb >> PP) + o + (o >> PP); };
         __syncthreads(); // the last stage's LDS reads are done
 #pragma unroll
         for (int m = 0; m < PPT / RL; ++m)
 #pragma unroll
             for (int r = 0; r < RL; ++r) {
-                int o  = addr(jt + m * TA, r * (M / RL));
+                int o  = paddr(jt + m * TA, r * (M / RL));
                 lre[o] = v[m * RL + r].x;
                 lim[o] = v[m * RL + r].y;
             }
@@ -756,7 +782,7 @@ void k_dif(RowArgs p) {
 #pragma unroll
         for (int u = 0; u < PPT; ++u) {
             int s = jt + u * TA;
-            int o = addr(perm_base<LM, ORD>(jt), brevc(u, ilog2c(PPT)));
+            int o = paddr(perm_base<LM, ORD>(jt), brevc(u, ilog2c(PPT)));
             dst.st(poff(base + s, p.dst_lp), cf{lre[o], lim[o]}, p.nt_store);
         }
     }

@@ -102,6 +102,15 @@ def test_inverse_all_sizes(orc, pa, lg, order):
     assert err < fft_tol(n), (n, order, err)
 
 
+@pytest.mark.parametrize("lg", [21, 22, 23, 24])
+@pytest.mark.parametrize("order", ORDERS)
+def test_largest_sizes(orc, pa, lg, order):
+    """One transform at each four-step size above 2^20, up to the 2^24 maximum (split layout)."""
+    n = 1 << lg
+    assert _fwd_case(orc, pa, n, order, n, n) < fft_tol(n)
+    assert _inv_case(orc, pa, n, order, n, n) < fft_tol(n)
+
+
 @pytest.mark.parametrize("n", [64, 1024, 4096, 1 << 15, 1 << 16])
 @pytest.mark.parametrize("packs", [(1, 1), (8, 1), (1, 8), (4, 2), ("N", "N"), ("N", 8)])
 @pytest.mark.parametrize("order", ORDERS)

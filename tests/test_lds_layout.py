@@ -45,3 +45,67 @@ def test_a_carrying_layout_is_caught():
     o = np.arange(4)[None, :] * 24
     p16 = lambda x: x + (x >> 4)  # noqa: E731
     assert not np.array_equal(p16(j + o), p16(j) + o + (o >> 4))
+
+
+# ---- the permuted exchange of row_group / k_dif (bit_reversed and unordered loads and stores)
+def plan(la, f64):
+    a = 1 << la
+    if f64:
+        ppt = 32 if la >= 13 else (16 if la >= 4 else a)
+    else:
+        ppt = 64 if la >= 14 else 32 if la >= 13 else (16 if la >= 4 else a)
+    return ppt, a // ppt
+
+
+def brev(v, bits):
+    return int(format(v, f"0{bits}b")[::-1], 2) if bits else 0
+
+
+def uq(i):  # the unordered lane permutation of the low five bits
+    return (i & ~31) | (i & 16) | ((i & 4) << 1) | ((i & 1) << 2) | (i & 2) | ((i >> 3) & 1)
+
+
+def perm_base(jt, la, ppt, ta, unordered):
+    lt, lpp = ta.bit_length() - 1, ppt.bit_length() - 1
+    return brev(uq(jt) if unordered else jt, lt) << lpp
+
+
+CASES = [(la, False) for la in range(1, 15)] + [(la, True) for la in range(1, 14)]
+
+
+@pytest.mark.parametrize("la,f64", CASES)
+def test_permuted_padding_is_carry_free_and_conflict_free(la, f64):
+    a = 1 << la
+    ppt, ta = plan(la, f64)
+    pp = max(la - 6, 5)
+    pad_p = lambda x: x + (x >> pp)  # noqa: E731
+    stride = a + (a >> 5) + 1  # the row stride the LDS array is sized for
+    assert pad_p(a - 1) < stride
+    rs = radices(la, False)
+    for rad in (rs[0], rs[-1]):  # first-stage loads, last-stage stores: b = jt + m*TA, o = r*A/R
+        b = (np.arange(ta)[:, None] + np.arange(ppt // rad)[None, :] * ta).reshape(-1, 1)
+        o = np.arange(rad)[None, :] * (a // rad)
+        assert np.array_equal(pad_p(b + o), pad_p(b) + o + (o >> pp))
+    if ta >= 32:  # permuted side: b = perm_base(jt), o = brev(u) < PPT
+        lpp = ppt.bit_length() - 1
+        for unordered in (False, True):
+            if unordered and la < 5:
+                continue
+            b = np.array([perm_base(j, la, ppt, ta, unordered) for j in range(ta)])[:, None]
+            o = np.array([brev(u, lpp) for u in range(ppt)])[None, :]
+            assert np.array_equal(pad_p(b + o), pad_p(b) + o + (o >> pp))
+            if ta >= 64:  # one wave = 64 lanes of one transform: distinct banks per access
+                # (from 2^11 on; at 2^10 the pad stays one per 32 elements, which the row stride
+                # is sized for, and two lanes can share a bank)
+                need = 64 if la >= 11 else 32
+                for w0 in range(0, ta, 64):
+                    banks = pad_p(b[w0:w0 + 64] + o) % 64
+                    assert all(len(set(banks[:, u])) >= need for u in range(ppt))
+
+
+def test_old_padding_conflicts_in_permuted_order():
+    """With one pad per 32 elements the permuted reads of a 2^14-point FP32 plan hit 8 banks:
+    the check above would catch a return to that layout."""
+    la, (ppt, ta) = 14, plan(14, False)
+    b = np.array([perm_base(j, la, ppt, ta, False) for j in range(64)])
+    assert len(set((b + (b >> 5)) % 64)) == 8
