@@ -655,12 +655,21 @@ template<int LM, int LR>
 constexpr int dif_lds_floats() {
     return RowShape<LM, 1>::DATA + TwTable<LM>::FLOATS + 2 * pass_tw_entries(LM + LR);
 }
+// Points per thread of k_dif: 32 for M = 2^14 (512 threads; 1.32 against 1.43 ms per 2 GiB at
+// N = 2^15 with 64), else the default rule.
+template<int LM>
+constexpr int dif_ppt() {
+    return LM >= 14 ? 32 : 0;
+}
 
 template<int LM, int LR, int ORD, int PREC = PREC_FP32>
-__global__ __launch_bounds__(Plan<LM>::TA,
-                             mfma_waves(waves_per_simd(Plan<LM>::TA, 4 * dif_lds_floats<LM, LR>()), PREC))
+__global__ __launch_bounds__((Plan<LM, false, float, dif_ppt<LM>()>::TA),
+                             mfma_waves(waves_per_simd(Plan<LM, false, float, dif_ppt<LM>()>::TA,
+                                                       4 * dif_lds_floats<LM, LR>()),
+                                        PREC))
 void k_dif(RowArgs p) {
-    using P           = Plan<LM>;
+    constexpr int PX  = dif_ppt<LM>();
+    using P           = Plan<LM, false, float, PX>;
     constexpr int TA  = P::TA;
     constexpr int PPT = P::PPT;
     constexpr int R0  = P::first_radix;
@@ -718,7 +727,7 @@ void k_dif(RowArgs p) {
             for (int m = 0; m < PPT / R0; ++m)
 #pragma unroll
                 for (int r = 0; r < R0; ++r) {
-                    uint32_t c = (uint32_t)(in_elem<LM>(jt, m, r) - jt);
+                    uint32_t c = (uint32_t)(in_elem<LM, false, float, PX>(jt, m, r) - jt);
                     cf       x = seg.ld(vo + poff(c, p.src_lp), 0);
                     x.y *= cs;
                     if (n1 == 0)
@@ -734,12 +743,12 @@ void k_dif(RowArgs p) {
         for (int m = 0; m < PPT / R0; ++m)
 #pragma unroll
             for (int r = 0; r < R0; ++r) {
-                uint32_t e    = (uint32_t)in_elem<LM>(jt, m, r) * (uint32_t)k1;
+                uint32_t e    = (uint32_t)in_elem<LM, false, float, PX>(jt, m, r) * (uint32_t)k1;
                 v[m * R0 + r] = cmul(v[m * R0 + r], PassTw<TWN>::get(lptw, p.twl, p.twh, p.tw_ls, p.tw_lg, e));
             }
     }
     // ------------------------------------------------------------------ M-point transform
-    stockham<LM, false, decltype(addr), PREC>(v, lre, lim, jt, addr, ltw, &mf);
+    stockham<LM, false, decltype(addr), PREC, float, PX>(v, lre, lim, jt, addr, ltw, &mf);
     const float sc = p.scale;
     const float so = p.conj_out ? -sc : sc;
 #pragma unroll
@@ -759,7 +768,7 @@ void k_dif(RowArgs p) {
             const int lp = opaque_s(p.dst_lp);
 #pragma unroll
             for (int r = 0; r < RL; ++r) {
-                uint32_t c = (uint32_t)(out_elem<LM>(jt, m, r) - jt);
+                uint32_t c = (uint32_t)(out_elem<LM, false, float, PX>(jt, m, r) - jt);
                 dst.st(vo, poff(c << LR, lp), v[m * RL + r], p.nt_store);
             }
         }

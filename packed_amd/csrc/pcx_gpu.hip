@@ -116,15 +116,15 @@ void launch_col(const ColArgsT<T>& a, int64_t grid_cap, hipStream_t st) {
 }
 template<int LA, int PREC = PREC_FP32>
 void launch_xc(const XcArgs& a, hipStream_t st) {
-    constexpr int G   = default_rows<LA>();
+    constexpr int G   = xc_rows<LA>();
     int64_t       nwg = grid8((a.nblocks * a.channels + G - 1) / G);
     // 32-bit buffer offsets cover a channel when its packed footprint stays below 2^29 floats
     int  lp    = a.src_lp > a.dst_lp ? a.src_lp : a.dst_lp;
     bool small = pidx64(a.len + (int64_t)(1 << LA), lp) < ((int64_t)1 << 29) && a.src_lp < 29 && a.dst_lp < 29;
     if (small)
-        hipLaunchKernelGGL((k_fxcorr<LA, G, true, PREC>), dim3((unsigned)nwg), dim3(G * Plan<LA>::TA), 0, st, a);
+        hipLaunchKernelGGL((k_fxcorr<LA, G, true, PREC>), dim3((unsigned)nwg), dim3(G * xc_threads<LA>()), 0, st, a);
     else
-        hipLaunchKernelGGL((k_fxcorr<LA, G, false, PREC>), dim3((unsigned)nwg), dim3(G * Plan<LA>::TA), 0, st, a);
+        hipLaunchKernelGGL((k_fxcorr<LA, G, false, PREC>), dim3((unsigned)nwg), dim3(G * xc_threads<LA>()), 0, st, a);
 }
 
 // Single-pass variants: 0 natural in/out; 1, 2 permuted store (forward, bit_reversed /
@@ -304,7 +304,8 @@ using DifLaunch = void (*)(const RowArgs&, int64_t batch, hipStream_t);
 template<int LM, int LR, int ORD>
 void launch_dif(const RowArgs& a, int64_t batch, hipStream_t st) {
     int64_t nwg = ((batch + 7) / 8) * 8 * ((int64_t)1 << LR);
-    hipLaunchKernelGGL((k_dif<LM, LR, ORD>), dim3((unsigned)nwg), dim3(Plan<LM>::TA), 0, st, a);
+    hipLaunchKernelGGL((k_dif<LM, LR, ORD>), dim3((unsigned)nwg), dim3(Plan<LM, false, float, dif_ppt<LM>()>::TA), 0, st,
+                       a);
 }
 constexpr int kDifLog2 = 15;
 const std::array<DifLaunch, 3> kDif = {&launch_dif<14, 1, ORD_NORMAL>, &launch_dif<13, 2, ORD_BITREV>,

@@ -24,7 +24,8 @@
 
 namespace pcxk {
 
-template<int LA, bool REV = false, class T = float>
+// PX > 0 overrides the points per thread (the latency-bound fused kernels run 2^14 with 32).
+template<int LA, bool REV = false, class T = float, int PX = 0>
 struct Plan {
     static constexpr bool F64    = sizeof(T) == 8;
     static constexpr int A       = 1 << LA;
@@ -34,8 +35,9 @@ struct Plan {
     // points per thread: more for the largest sizes, so that a 2^13 transform takes 256 threads
     // and a 2^14 transform 256 threads as well (two workgroups share a CU: one loads or stores
     // while the other computes).  FP64 values take two VGPRs each: at most 32 points per thread.
-    static constexpr int PPT     = F64 ? (LA >= 13 ? 32 : (LA >= 4 ? 16 : A))
-                                       : (LA >= 14 ? 64 : LA >= 13 ? 32 : (LA >= 4 ? 16 : A));
+    static constexpr int PPT     = PX > 0 ? PX
+                                 : F64  ? (LA >= 13 ? 32 : (LA >= 4 ? 16 : A))
+                                        : (LA >= 14 ? 64 : LA >= 13 ? 32 : (LA >= 4 ? 16 : A));
     static constexpr int TA      = A / PPT;          // threads per transform
     // HALF: the stage exchange moves real parts, then imaginary parts, through one A-element LDS
     // array (2^14 FP32 points: 66 KiB instead of 132 KiB, so two workgroups fit a CU; FP64 from
@@ -107,9 +109,9 @@ __device__ __forceinline__ int opaque_after_memory(int x) {
 }
 
 // Reads of stage s (component C: 0 real, 1 imaginary, 2 both) from LDS into v.
-template<int LA, bool REV, int s, int C, class Addr, class T>
+template<int LA, bool REV, int s, int C, int PX = 0, class Addr, class T>
 __device__ __forceinline__ void stage_read(cx<T>* v, const T* lre, const T* lim, int jt, const Addr& addr) {
-    using P          = Plan<LA, REV, T>;
+    using P          = Plan<LA, REV, T, PX>;
     constexpr int R  = P::radix(s);
     constexpr int NB = P::PPT / R;
 #pragma unroll
@@ -126,9 +128,9 @@ __device__ __forceinline__ void stage_read(cx<T>* v, const T* lre, const T* lim,
     }
 }
 // Writes of stage s (component C) from v to LDS.
-template<int LA, bool REV, int s, int C, class Addr, class T>
+template<int LA, bool REV, int s, int C, int PX = 0, class Addr, class T>
 __device__ __forceinline__ void stage_write(const cx<T>* v, T* lre, T* lim, int jt, const Addr& addr) {
-    using P          = Plan<LA, REV, T>;
+    using P          = Plan<LA, REV, T, PX>;
     constexpr int R  = P::radix(s);
     constexpr int Ns = P::ns(s);
     constexpr int NB = P::PPT / R;
@@ -157,10 +159,10 @@ __device__ __forceinline__ void dft16_mfma(const MfmaDft16* mf, cx<T>* v) {
 // PREC_BF16X3 runs the radix-16 butterflies on the matrix cores (mf holds the operands).
 // With Plan::HALF, lre and lim are the same array and each exchange runs in two halves.
 // T = double runs the same schedule in FP64 (PREC is then PREC_FP32, the plain butterflies).
-template<int LA, bool REV, class Addr, int PREC = PREC_FP32, class T = float>
+template<int LA, bool REV, class Addr, int PREC = PREC_FP32, class T = float, int PX = 0>
 __device__ __forceinline__ void stockham(cx<T>* v, T* lre, T* lim, int jt0, const Addr& addr, const T* ltw,
                                          const MfmaDft16* mf = nullptr) {
-    using P           = Plan<LA, REV, T>;
+    using P           = Plan<LA, REV, T, PX>;
     static_assert(PREC == PREC_FP32 || sizeof(T) == 4, "matrix-core butterflies are FP32 only");
     constexpr int TA  = P::TA;
     constexpr int PPT = P::PPT;
@@ -174,7 +176,7 @@ __device__ __forceinline__ void stockham(cx<T>* v, T* lre, T* lim, int jt0, cons
         const int     jt = P::HALF ? opaque(jt0) : jt0;
         if constexpr (s > 0 && !P::HALF) {
             __syncthreads(); // previous stage's LDS writes are visible
-            stage_read<LA, REV, s, 2>(v, lre, lim, jt, addr);
+            stage_read<LA, REV, s, 2, PX>(v, lre, lim, jt, addr);
         }
         // twiddle and butterfly one radix-R group at a time, so only one group's twiddles are live
 #pragma unroll
@@ -196,30 +198,30 @@ __device__ __forceinline__ void stockham(cx<T>* v, T* lre, T* lim, int jt0, cons
         if constexpr (s + 1 < P::NSTAGES) {
             __syncthreads(); // every read of this stage is done before LDS is overwritten
             if constexpr (!P::HALF) {
-                stage_write<LA, REV, s, 2>(v, lre, lim, jt, addr);
+                stage_write<LA, REV, s, 2, PX>(v, lre, lim, jt, addr);
             } else {
                 // real parts out and back in the next stage's layout, then imaginary parts
-                stage_write<LA, REV, s, 0>(v, lre, lim, jt, addr);
+                stage_write<LA, REV, s, 0, PX>(v, lre, lim, jt, addr);
                 __syncthreads();
-                stage_read<LA, REV, s + 1, 0>(v, lre, lim, jt, addr);
+                stage_read<LA, REV, s + 1, 0, PX>(v, lre, lim, jt, addr);
                 __syncthreads();
-                stage_write<LA, REV, s, 1>(v, lre, lim, jt, addr);
+                stage_write<LA, REV, s, 1, PX>(v, lre, lim, jt, addr);
                 __syncthreads();
-                stage_read<LA, REV, s + 1, 1>(v, lre, lim, jt, addr);
+                stage_read<LA, REV, s + 1, 1, PX>(v, lre, lim, jt, addr);
             }
         }
     });
 }
 
 // Element index held by register slot m*R + r on entry (first stage) and on exit (last stage).
-template<int LA, bool REV = false, class T = float>
+template<int LA, bool REV = false, class T = float, int PX = 0>
 __device__ __forceinline__ int in_elem(int jt, int m, int r) {
-    using P = Plan<LA, REV, T>;
+    using P = Plan<LA, REV, T, PX>;
     return jt + m * P::TA + r * (P::A / P::first_radix);
 }
-template<int LA, bool REV = false, class T = float>
+template<int LA, bool REV = false, class T = float, int PX = 0>
 __device__ __forceinline__ int out_elem(int jt, int m, int r) {
-    using P = Plan<LA, REV, T>;
+    using P = Plan<LA, REV, T, PX>;
     return jt + m * P::TA + r * (P::A / P::last_radix);
 }
 
