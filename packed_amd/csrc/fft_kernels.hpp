@@ -292,8 +292,10 @@ __device__ __forceinline__ void row_group(const RowArgsT<T>& p, int64_t wg, T* l
     const int64_t  rmask  = ((int64_t)1 << p.rows_lg) - 1;
     const int64_t  bt     = row >> p.rows_lg;
     const uint32_t k2     = (uint32_t)(row & rmask);
-    // transform of this thread's row; with UNI the same for the whole wavefront
-    const int64_t  btu    = (UNI && TA < 64) ? (row0 >> p.rows_lg) : bt;
+    // transform of this thread's row; with UNI the same for the whole wavefront.  Rows past the
+    // batch (the last group may be partial) take the group's first row, which always exists:
+    // their loads stay inside the source (values masked) and they store nothing.
+    const int64_t  btu    = (UNI && TA < 64) || !row_ok ? (row0 >> p.rows_lg) : bt;
     auto           addr   = [&](int b, int o) { return g * STR + b + (b >> 5) + o + (o >> 5); };
     // The permuted loads and stores exchange through LDS with one pad per 2^PP elements,
     // PP = max(LA - 6, 5).  In the permuted order lane jt touches element brev(jt) * PPT + c,
