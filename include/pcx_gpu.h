@@ -211,6 +211,19 @@ int pcx_vexpr_eval(const pcx_vsource* srcs, int32_t nsrc, const pcx_vinstr* prog
 /* Pack conversion: dst (pack dst_pack) = src (pack src_pack), n elements; in place only when the
  * packs are equal (vector.hpp copy between pack sizes). */
 int pcx_repack(float* dst, int64_t dst_pack, const float* src, int64_t src_pack, int64_t n, void* stream);
+/* The same for double vectors and scalars (tests/test_op.cpp runs every operator for float and
+ * double): programs of pcx_vinstr_f64 over pcx_vsource_f64, double arithmetic throughout. */
+typedef struct {
+    int32_t op, d, a, b; /* as in pcx_vinstr */
+    double  s_re, s_im;
+} pcx_vinstr_f64;
+typedef struct {
+    const double* data;
+    int64_t       pack;
+} pcx_vsource_f64;
+int pcx_vexpr_eval_f64(const pcx_vsource_f64* srcs, int32_t nsrc, const pcx_vinstr_f64* prog, int32_t nprog,
+                       double* dst, int64_t dst_pack, int64_t n, void* stream);
+int pcx_repack_f64(double* dst, int64_t dst_pack, const double* src, int64_t src_pack, int64_t n, void* stream);
 
 /* ------------------------------------------------------------------ memory and utilities
  * The reference's Allocator template parameter (allocators.hpp:11-48) decides where buffers

@@ -46,6 +46,19 @@ class VInstr(C.Structure):
                 ("s_re", C.c_float), ("s_im", C.c_float)]
 
 
+class VSource64(C.Structure):
+    """pcx_vsource_f64 (include/pcx_gpu.h)."""
+
+    _fields_ = [("data", C.c_void_p), ("pack", C.c_int64)]
+
+
+class VInstr64(C.Structure):
+    """pcx_vinstr_f64 (include/pcx_gpu.h)."""
+
+    _fields_ = [("op", C.c_int32), ("d", C.c_int32), ("a", C.c_int32), ("b", C.c_int32),
+                ("s_re", C.c_double), ("s_im", C.c_double)]
+
+
 # pcx_vop (include/pcx_gpu.h)
 VOP = {name: i for i, name in enumerate(
     ["LOAD", "ADD", "SUB", "MUL", "DIV", "CONJ", "ADDS", "SUBS", "RSUBS", "MULS", "DIVS", "RDIVS",
@@ -95,6 +108,9 @@ SIGNATURES = {
     "pcx_generate_f64": (_int, [_vp, _i64, _i64, _u64, _int, _dbl, _vp]),
     "pcx_vexpr_eval": (_int, [C.POINTER(VSource), C.c_int32, C.POINTER(VInstr), C.c_int32, _vp, _i64, _i64, _vp]),
     "pcx_repack": (_int, [_vp, _i64, _vp, _i64, _i64, _vp]),
+    "pcx_vexpr_eval_f64": (_int, [C.POINTER(VSource64), C.c_int32, C.POINTER(VInstr64), C.c_int32, _vp, _i64, _i64,
+                                  _vp]),
+    "pcx_repack_f64": (_int, [_vp, _i64, _vp, _i64, _i64, _vp]),
     "pcx_fft_across": (_int, [_vp, _int, _int, _vp, _i64, _vp, _i64, _i64, _vp]),
 }
 

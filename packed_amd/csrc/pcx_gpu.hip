@@ -1429,8 +1429,12 @@ extern "C" int pcx_queue_autotune(pcx_queue_t q, float* dst, const float* src, v
 }
 
 // ---------------------------------------------------------------- vector expressions
-extern "C" int pcx_vexpr_eval(const pcx_vsource* srcs, int32_t nsrc, const pcx_vinstr* prog, int32_t nprog, float* dst,
-                              int64_t dst_pack, int64_t n, void* stream) {
+namespace {
+// One validation and launch path for both element types (S, I: the C ABI's source and
+// instruction structs of that type).
+template<class T, class S, class I>
+int vexpr_eval(const S* srcs, int32_t nsrc, const I* prog, int32_t nprog, T* dst, int64_t dst_pack, int64_t n,
+               void* stream) {
     if (nsrc < 0 || nsrc > PCX_VEXPR_MAX_SRC || (nsrc > 0 && !srcs))
         return fail(PCX_ERR_INVALID_ARGUMENT, "source count (which is " + s(nsrc) + ") is not in [0, 16]");
     if (nprog < 1 || nprog > PCX_VEXPR_MAX_PROG || !prog)
@@ -1441,13 +1445,13 @@ extern "C" int pcx_vexpr_eval(const pcx_vsource* srcs, int32_t nsrc, const pcx_v
         return fail(PCX_ERR_INVALID_ARGUMENT, "size (which is " + s(n) + ") is negative");
     if (int e = check_pack(dst_pack, "destination"))
         return e;
-    VExprArgs a{};
+    VExprArgsT<T> a{};
     for (int k = 0; k < nsrc; ++k) {
         if (!srcs[k].data)
             return fail(PCX_ERR_INVALID_ARGUMENT, "source " + s(k) + " is null");
         if (int e = check_pack(srcs[k].pack, "source"))
             return e;
-        if ((const float*)dst == srcs[k].data && srcs[k].pack != dst_pack)
+        if ((const T*)dst == srcs[k].data && srcs[k].pack != dst_pack)
             return fail(PCX_ERR_INPLACE_REPACK, "cannot perform in-place repack from source pack size (which is " +
                                                     s(srcs[k].pack) + ") to destination pack size (which is " +
                                                     s(dst_pack) + ")");
@@ -1455,15 +1459,15 @@ extern "C" int pcx_vexpr_eval(const pcx_vsource* srcs, int32_t nsrc, const pcx_v
         a.src[k].lp   = ilog2(srcs[k].pack);
     }
     for (int k = 0; k < nprog; ++k) {
-        const pcx_vinstr& in = prog[k];
-        bool              ok = in.op >= 0 && in.op < VOP_COUNT && in.d >= 0 && in.d < PCX_VEXPR_REGS;
+        const I& in = prog[k];
+        bool     ok = in.op >= 0 && in.op < VOP_COUNT && in.d >= 0 && in.d < PCX_VEXPR_REGS;
         if (in.op == VOP_LOAD)
             ok = ok && in.a >= 0 && in.a < nsrc;
         else
             ok = ok && in.a >= 0 && in.a < PCX_VEXPR_REGS && (in.b >= 0 && in.b < PCX_VEXPR_REGS);
         if (!ok)
             return fail(PCX_ERR_INVALID_ARGUMENT, "instruction " + s(k) + " is malformed");
-        a.prog[k] = VInstr{in.op, in.d, in.a, in.b, in.s_re, in.s_im};
+        a.prog[k] = VInstrT<T>{in.op, in.d, in.a, in.b, in.s_re, in.s_im};
     }
     a.nprog  = nprog;
     a.dst    = dst;
@@ -1474,15 +1478,33 @@ extern "C" int pcx_vexpr_eval(const pcx_vsource* srcs, int32_t nsrc, const pcx_v
     int64_t nwg = (n + 255) / 256;
     if (nwg > 16384)
         nwg = 16384;
-    hipLaunchKernelGGL(k_vexpr, dim3((unsigned)nwg), dim3(256), 0, (hipStream_t)stream, a);
+    hipLaunchKernelGGL(k_vexpr<T>, dim3((unsigned)nwg), dim3(256), 0, (hipStream_t)stream, a);
     HIPCHK(hipGetLastError());
     return PCX_SUCCESS;
+}
+} // namespace
+
+extern "C" int pcx_vexpr_eval(const pcx_vsource* srcs, int32_t nsrc, const pcx_vinstr* prog, int32_t nprog, float* dst,
+                              int64_t dst_pack, int64_t n, void* stream) {
+    return vexpr_eval(srcs, nsrc, prog, nprog, dst, dst_pack, n, stream);
 }
 
 extern "C" int pcx_repack(float* dst, int64_t dst_pack, const float* src, int64_t src_pack, int64_t n, void* stream) {
     pcx_vsource sv{src, src_pack};
     pcx_vinstr  ld{PCX_VOP_LOAD, 0, 0, 0, 0.f, 0.f};
     return pcx_vexpr_eval(&sv, 1, &ld, 1, dst, dst_pack, n, stream);
+}
+
+extern "C" int pcx_vexpr_eval_f64(const pcx_vsource_f64* srcs, int32_t nsrc, const pcx_vinstr_f64* prog, int32_t nprog,
+                                  double* dst, int64_t dst_pack, int64_t n, void* stream) {
+    return vexpr_eval(srcs, nsrc, prog, nprog, dst, dst_pack, n, stream);
+}
+
+extern "C" int pcx_repack_f64(double* dst, int64_t dst_pack, const double* src, int64_t src_pack, int64_t n,
+                              void* stream) {
+    pcx_vsource_f64 sv{src, src_pack};
+    pcx_vinstr_f64  ld{PCX_VOP_LOAD, 0, 0, 0, 0.0, 0.0};
+    return pcx_vexpr_eval_f64(&sv, 1, &ld, 1, dst, dst_pack, n, stream);
 }
 
 // ---------------------------------------------------------------- FFT across vectors

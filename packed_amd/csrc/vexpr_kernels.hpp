@@ -11,22 +11,26 @@
 
 namespace pcxk {
 
-struct VInstr {
+// T = float or double: the element type of every vector and scalar of the program.
+template<class T>
+struct VInstrT {
     int32_t op, d, a, b;
-    float   s_re, s_im;
+    T       s_re, s_im;
 };
-struct VSource {
-    const float* data;
-    int64_t      lp; // log2 of the pack size
+template<class T>
+struct VSourceT {
+    const T* data;
+    int64_t  lp; // log2 of the pack size
 };
 constexpr int kVMaxProg = 64, kVMaxSrc = 16, kVRegs = 4;
-struct VExprArgs {
-    VSource src[kVMaxSrc];
-    VInstr  prog[kVMaxProg];
-    int32_t nprog;
-    float*  dst;
-    int     dst_lp;
-    int64_t n;
+template<class T>
+struct VExprArgsT {
+    VSourceT<T> src[kVMaxSrc];
+    VInstrT<T>  prog[kVMaxProg];
+    int32_t     nprog;
+    T*          dst;
+    int         dst_lp;
+    int64_t     n;
 };
 
 enum : int32_t {
@@ -35,39 +39,43 @@ enum : int32_t {
 };
 
 // simd_common.hpp:296-309: (a conj(b)) / (br^2 + bi^2)
-__device__ __forceinline__ cf vdiv(cf a, cf b) {
-    float den = __builtin_fmaf(b.y, b.y, b.x * b.x);
-    float re  = __builtin_fmaf(a.y, b.y, a.x * b.x);
-    float im  = __builtin_fmaf(a.y, b.x, -(a.x * b.y));
+template<class T>
+__device__ __forceinline__ cx<T> vdiv(cx<T> a, cx<T> b) {
+    T den = fmat(b.y, b.y, b.x * b.x);
+    T re  = fmat(a.y, b.y, a.x * b.x);
+    T im  = fmat(a.y, b.x, -(a.x * b.y));
     return {re / den, im / den};
 }
 // simd_common.hpp:372-383: real s / b
-__device__ __forceinline__ cf vrdiv_real(float s, cf b) {
-    float den = __builtin_fmaf(b.y, b.y, b.x * b.x);
+template<class T>
+__device__ __forceinline__ cx<T> vrdiv_real(T s, cx<T> b) {
+    T den = fmat(b.y, b.y, b.x * b.x);
     return {(s * b.x) / den, -(s * b.y) / den};
 }
 
-__device__ __forceinline__ cf vsel(const cf (&r)[kVRegs], int i) {
-    cf v = r[0];
+template<class T>
+__device__ __forceinline__ cx<T> vsel(const cx<T> (&r)[kVRegs], int i) {
+    cx<T> v = r[0];
     v    = i == 1 ? r[1] : v;
     v    = i == 2 ? r[2] : v;
     v    = i == 3 ? r[3] : v;
     return v;
 }
 
-__global__ __launch_bounds__(256) void k_vexpr(VExprArgs p) {
+template<class T>
+__global__ __launch_bounds__(256) void k_vexpr(VExprArgsT<T> p) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < p.n; i += stride) {
-        cf r[kVRegs] = {};
+        cx<T> r[kVRegs] = {};
         for (int k = 0; k < p.nprog; ++k) {
-            const VInstr in = p.prog[k]; // uniform: kernel arguments
-            const cf     x  = vsel(r, in.a);
-            const cf     y  = vsel(r, in.b);
-            const cf     s  = {in.s_re, in.s_im};
-            cf           z;
+            const VInstrT<T> in = p.prog[k]; // uniform: kernel arguments
+            const cx<T>      x  = vsel(r, in.a);
+            const cx<T>      y  = vsel(r, in.b);
+            const cx<T>      s  = {in.s_re, in.s_im};
+            cx<T>            z;
             switch (in.op) {
             case VOP_LOAD: {
-                const VSource src = p.src[in.a];
+                const VSourceT<T> src = p.src[in.a];
                 const int64_t o   = pidx(i, (int)src.lp);
                 z                 = {src.data[o], src.data[o + ((int64_t)1 << src.lp)]};
                 break;

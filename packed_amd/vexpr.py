@@ -174,10 +174,12 @@ def compile_expr(expr: Expr):
     return prog, srcs
 
 
-def _float_only(*vs) -> None:
-    for v in vs:
-        if getattr(v, "dtype", "float32") != "float32":
-            raise PcxError(_lib.ERR_UNSUPPORTED, "vector expressions and repacking are built for float vectors")
+def _common_dtype(*vs) -> str:
+    """The element type shared by every vector of an evaluation (float32 or float64)."""
+    kinds = {getattr(v, "dtype", "float32") for v in vs}
+    if len(kinds) != 1:
+        raise PcxError(_lib.ERR_INVALID_ARGUMENT, "an expression mixes float and double vectors")
+    return kinds.pop()
 
 
 def store(expr, dst, stream=None) -> None:
@@ -191,10 +193,12 @@ def store(expr, dst, stream=None) -> None:
         raise PcxError(_lib.ERR_SIZE_MISMATCH,
                        f"expression size (which is {expr.size()}) differs from destination size (which is {dst.size()})")
     prog, srcs = compile_expr(expr)
-    _float_only(dst, *srcs)
-    S = (_lib.VSource * max(1, len(srcs)))(*[_lib.VSource(v.data_ptr(), v.pack) for v in srcs])
-    P = (_lib.VInstr * len(prog))(*[_lib.VInstr(*i) for i in prog])
-    check(lib().pcx_vexpr_eval(S, len(srcs), P, len(prog), dst.data_ptr(), dst.pack, dst.size(), _stream_ptr(stream)))
+    f64 = _common_dtype(dst, *srcs) == "float64"
+    VS, VI = (_lib.VSource64, _lib.VInstr64) if f64 else (_lib.VSource, _lib.VInstr)
+    S = (VS * max(1, len(srcs)))(*[VS(v.data_ptr(), v.pack) for v in srcs])
+    P = (VI * len(prog))(*[VI(*i) for i in prog])
+    ev = lib().pcx_vexpr_eval_f64 if f64 else lib().pcx_vexpr_eval
+    check(ev(S, len(srcs), P, len(prog), dst.data_ptr(), dst.pack, dst.size(), _stream_ptr(stream)))
 
 
 def repack(dst, src, stream=None) -> None:
@@ -203,5 +207,5 @@ def repack(dst, src, stream=None) -> None:
 
     if src.size() != dst.size():
         raise PcxError(_lib.ERR_SIZE_MISMATCH, "repack needs equal sizes")
-    _float_only(dst, src)
-    check(lib().pcx_repack(dst.data_ptr(), dst.pack, src.data_ptr(), src.pack, dst.size(), _stream_ptr(stream)))
+    rp = lib().pcx_repack_f64 if _common_dtype(dst, src) == "float64" else lib().pcx_repack
+    check(rp(dst.data_ptr(), dst.pack, src.data_ptr(), src.pack, dst.size(), _stream_ptr(stream)))

@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <limits>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 using cxf = std::complex<float>;
@@ -247,49 +248,57 @@ static void test_fxcorr() {
 
 // tests/test_op.cpp:89-226 test_bin_ops: the 16 vector-vector operations, then complex- and
 // real-scalar operations on either side, checked with equal_eps (10 FLT_EPSILON relative).
-template<size_t PackSize>
+// tests/test_op.cpp test_bin_ops, for float and double (the reference runs both): element type
+// T, host result in a wider type W, every element within 10 epsilon(T) relative.
+template<typename T, size_t PackSize>
 static void test_bin_ops(size_t length) {
     using namespace pcx_gpu;
-    const float        eps = std::numeric_limits<float>::epsilon();
-    std::vector<cxf>   lhs(length), rhs(length);
+    using W             = std::conditional_t<std::is_same_v<T, float>, double, long double>;
+    using cT            = std::complex<T>;
+    using cW            = std::complex<W>;
+    const T         eps = std::numeric_limits<T>::epsilon();
+    std::vector<cT> lhs(length), rhs(length);
     for (size_t i = 0; i < length; ++i) {
-        lhs[i] = cxf(i * eps, -1.f - (float)i);
-        rhs[i] = cxf((float)i + 1, i * eps);
+        lhs[i] = cT((T)i * eps, T(-1) - (T)i);
+        rhs[i] = cT((T)i + 1, (T)i * eps);
     }
-    vector<float, PackSize> L(lhs), R(rhs), out(length);
-    const cxf               cx(512 * (eps + 1), -512 * eps);
-    const float             re = 1024 * (eps - 1);
+    vector<T, PackSize> L(lhs), R(rhs), out(length);
+    const cT            cx(512 * (eps + 1), -512 * eps);
+    const T             re = 1024 * (eps - 1);
     auto check = [&](const char* what, auto ref) {
         auto got = out.to_host();
         for (size_t i = 0; i < length; ++i) {
-            cxd want = ref(i);
-            cxd g    = cxd(got[i]);
-            double big = std::max(std::abs(g), std::abs(want));
-            if (!(g == want || std::abs(g - want) < big * 10 * eps)) {
-                CHECK(false, "%s pack %zu length %zu index %zu", what, PackSize, length, i);
+            cW want = ref(i);
+            cW g    = cW(got[i]);
+            W  big  = std::max(std::abs(g), std::abs(want));
+            if (!(g == want || std::abs(g - want) < big * 10 * (W)eps)) {
+                CHECK(false, "%s %s pack %zu length %zu index %zu", sizeof(T) == 4 ? "float" : "double", what, PackSize,
+                      length, i);
                 return;
             }
         }
     };
-    auto l = [&](size_t i) { return cxd(lhs[i]); };
-    auto r = [&](size_t i) { return cxd(rhs[i]); };
+    auto l  = [&](size_t i) { return cW(lhs[i]); };
+    auto r  = [&](size_t i) { return cW(rhs[i]); };
+    auto cs = cW(cx);
+    auto rs = (W)re;
     out = L + R;                 check("a+b", [&](size_t i) { return l(i) + r(i); });
     out = L - conj(R);           check("a-conj(b)", [&](size_t i) { return l(i) - std::conj(r(i)); });
     out = conj(L) * R;           check("conj(a)*b", [&](size_t i) { return std::conj(l(i)) * r(i); });
     out = conj(L) / conj(R);     check("conj(a)/conj(b)", [&](size_t i) { return std::conj(l(i)) / std::conj(r(i)); });
     out = L / R;                 check("a/b", [&](size_t i) { return l(i) / r(i); });
-    out = L * cx;                check("a*s", [&](size_t i) { return l(i) * cxd(cx); });
-    out = cx / R;                check("s/b", [&](size_t i) { return cxd(cx) / r(i); });
-    out = cx - conj(R);          check("s-conj(b)", [&](size_t i) { return cxd(cx) - std::conj(r(i)); });
-    out = L + re;                check("a+re", [&](size_t i) { return l(i) + (double)re; });
-    out = re - R;                check("re-b", [&](size_t i) { return (double)re - r(i); });
-    out = re / R;                check("re/b", [&](size_t i) { return (double)re / r(i); });
-    out = L / re;                check("a/re", [&](size_t i) { return l(i) / (double)re; });
+    out = L * cx;                check("a*s", [&](size_t i) { return l(i) * cs; });
+    out = cx / R;                check("s/b", [&](size_t i) { return cs / r(i); });
+    out = cx - conj(R);          check("s-conj(b)", [&](size_t i) { return cs - std::conj(r(i)); });
+    out = L + re;                check("a+re", [&](size_t i) { return l(i) + rs; });
+    out = re - R;                check("re-b", [&](size_t i) { return rs - r(i); });
+    out = re / R;                check("re/b", [&](size_t i) { return rs / r(i); });
+    out = L / re;                check("a/re", [&](size_t i) { return l(i) / rs; });
     // a fused pipeline across pack sizes: one kernel
-    vector<float, 1> L1(lhs);
-    out = (L1 + R) * conj(L - R) / (R + 2.f) - cx * L1;
+    vector<T, 1> L1(lhs);
+    out = (L1 + R) * conj(L - R) / (R + T(2)) - cx * L1;
     check("fused", [&](size_t i) {
-        return (l(i) + r(i)) * std::conj(l(i) - r(i)) / (r(i) + 2.0) - cxd(cx) * l(i);
+        return (l(i) + r(i)) * std::conj(l(i) - r(i)) / (r(i) + W(2)) - cs * l(i);
     });
     // repack through assignment of a vector of another pack size
     store(L1, out);
@@ -497,9 +506,12 @@ int main() {
     test_fxcorr();
     test_fft_par();
     for (size_t n = 1; n < 32; ++n) { // test_op.cpp:241-252
-        test_bin_ops<1>(n);
-        test_bin_ops<8>(n);
-        test_bin_ops<32>(n);
+        test_bin_ops<float, 1>(n);
+        test_bin_ops<float, 8>(n);
+        test_bin_ops<float, 32>(n);
+        test_bin_ops<double, 1>(n);
+        test_bin_ops<double, 4>(n);
+        test_bin_ops<double, 32>(n);
     }
     std::printf("%s: %d failed checks\n", g_fail ? "FAIL" : "OK", g_fail);
     return g_fail;

@@ -132,3 +132,64 @@ def test_fused_expression_and_errors(orc, pa):
     too_deep = too_deep + ((a + b) * (c + d)) / ((a - b) * (c - d))
     with pytest.raises(pa.PcxError):
         pa.store(too_deep, out)
+
+
+EPS64 = float(np.finfo(np.float64).eps)
+
+
+@pytest.mark.parametrize("pack", [1, 4, 8, 32])
+def test_reference_bin_ops_double(orc, pa, pack):
+    """test_bin_ops for double (tests/test_op.cpp runs it for float and double): the same operands
+    with DBL_EPSILON, every element within 10 DBL_EPSILON relative of an extended-precision
+    (80-bit long double) host result."""
+    import torch
+
+    def eq(got, want):
+        want = np.asarray(want, np.clongdouble)
+        got = np.asarray(got, np.clongdouble)
+        big = np.maximum(np.abs(got), np.abs(want))
+        return np.all((got == want) | (np.abs(got - want) < big * 10 * EPS64))
+
+    for length in list(range(1, 32)) + [1000]:
+        i = np.arange(length, dtype=np.float64)
+        lhs = i * EPS64 + 1j * (-1.0 - i)
+        rhs = (i + 1) + 1j * (i * EPS64)
+        L = pa.vector.from_complex(lhs, pack, dtype="float64")
+        R = pa.vector.from_complex(rhs, pack, dtype="float64")
+        out = pa.vector(length, pack, dtype="float64")
+        lh, rh = lhs.astype(np.clongdouble), rhs.astype(np.clongdouble)
+        cx = complex(512 * (EPS64 + 1), -512 * EPS64)
+        re = 1024 * (EPS64 - 1)
+        for name, dev, host in _vec_ops(pa):
+            pa.store(dev(L, R), out)
+            torch.cuda.synchronize()
+            assert eq(out.to_complex(), host(lh, rh)), (pack, length, name)
+            pa.store(dev(L, cx), out)
+            torch.cuda.synchronize()
+            assert eq(out.to_complex(), host(lh, np.clongdouble(cx))), (pack, length, name, "scalar rhs")
+            pa.store(dev(cx, R), out)
+            torch.cuda.synchronize()
+            assert eq(out.to_complex(), host(np.clongdouble(cx), rh)), (pack, length, name, "scalar lhs")
+        for (name, f) in OPS:
+            for left in (False, True):
+                for s in (cx, re):
+                    pa.store(f(s, L) if left else f(L, s), out)
+                    torch.cuda.synchronize()
+                    sh = np.clongdouble(s) if isinstance(s, complex) else np.longdouble(s)
+                    want = f(sh, lh) if left else f(lh, sh)
+                    assert eq(out.to_complex(), want), (pack, length, name, left, s)
+
+
+@pytest.mark.parametrize("sp,dp", [(1, 4), (4, 1), (4, 32), (2, 8)])
+def test_repack_double_bit_exact(orc, pa, sp, dp):
+    import torch
+
+    n = 1000
+    z = orc.gen_normal(0x5EED0051, n).astype(np.complex128) * (1 + 2.0 ** -40)
+    src = pa.vector.from_complex(z, sp, dtype="float64")
+    dst = pa.vector(n, dp, dtype="float64")
+    pa.repack(dst, src)
+    torch.cuda.synchronize()
+    assert np.array_equal(dst.to_complex(), z)
+    with pytest.raises(pa.PcxError):
+        pa.repack(dst, pa.vector(n, sp))  # float and double do not mix
