@@ -359,7 +359,18 @@ class fxcorr_unit:
         return int(lib().pcx_fxcorr_step(self._h, pack))
 
     def __call__(self, f: vector, out: vector | None = None, stream=None) -> None:
-        """operator()(vector&) in place (fxcorr.hpp:102-136), or into `out` (the out-of-place overload)."""
+        """operator()(vector&) in place (fxcorr.hpp:102-136), or into `out` (the out-of-place overload).
+        With md_storage arguments every slice is a channel: one launch, one shared kernel spectrum."""
+        if isinstance(f, md_storage):
+            dst = f if out is None else out
+            if not isinstance(dst, md_storage) or dst.outer != f.outer or dst.inner < f.inner:
+                raise PcxError(_lib.ERR_SIZE_MISMATCH, "destination storage must have the source's slice count "
+                                                       "and at least its slice size")
+            if f.dtype != "float32" or dst.dtype != "float32":
+                raise PcxError(_lib.ERR_UNSUPPORTED, "cross-correlation is built for float vectors")
+            check(lib().pcx_fxcorr_exec_batch(self._h, dst.data_ptr(), dst.pack, dst.row_stride, f.data_ptr(), f.pack,
+                                              f.row_stride, f.inner, f.outer, _stream_ptr(stream)))
+            return
         dst = f if out is None else out
         if dst.size() < f.size():
             raise PcxError(_lib.ERR_SIZE_MISMATCH, f"destination size (which is {dst.size()}) is smaller than "

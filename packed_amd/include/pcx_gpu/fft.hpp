@@ -502,6 +502,29 @@ public:
         throw_on(pcx_fxcorr_exec(m_xc.get(), dest.data(), DestVect_::pack_size, const_cast<T*>(source.data()),
                                  SrcVect_::pack_size, static_cast<int64_t>(source.size()), m_stream));
     }
+    // Every slice of a batch storage (pcx_gpu/mdstorage.hpp) is a channel: one launch, one shared
+    // kernel spectrum (the reference correlates one vector per call).
+    template<md_batch S>
+        requires std::same_as<typename S::real_type, T>
+    void operator()(S& storage) {
+        throw_on(pcx_fxcorr_exec_batch(m_xc.get(), storage.data(), S::pack_size, storage.row_stride(), storage.data(),
+                                       S::pack_size, storage.row_stride(),
+                                       static_cast<int64_t>(storage.inner_extent()),
+                                       static_cast<int64_t>(storage.size()), m_stream));
+    }
+    template<md_batch D, md_batch S>
+        requires std::same_as<typename D::real_type, T> && std::same_as<typename S::real_type, T>
+    void operator()(D& dest, const S& source) {
+        if (dest.size() != source.size() || dest.inner_extent() < source.inner_extent())
+            throw std::invalid_argument("destination storage (which is " + std::to_string(dest.size()) + " x " +
+                                        std::to_string(dest.inner_extent()) + ") does not hold source storage (which is " +
+                                        std::to_string(source.size()) + " x " + std::to_string(source.inner_extent()) +
+                                        ")");
+        throw_on(pcx_fxcorr_exec_batch(m_xc.get(), dest.data(), D::pack_size, dest.row_stride(), source.data(),
+                                       S::pack_size, source.row_stride(),
+                                       static_cast<int64_t>(source.inner_extent()), static_cast<int64_t>(source.size()),
+                                       m_stream));
+    }
     [[nodiscard]] uZ step(uZ pack = 8) const { return static_cast<uZ>(pcx_fxcorr_step(m_xc.get(), pack)); }
 
 private:

@@ -486,6 +486,42 @@ static void test_md_storage() {
         threw = true;
     }
     CHECK(threw, "slice size mismatch must throw");
+
+    // correlation: every slice a channel, one shared kernel spectrum, in place and out of place
+    const size_t L = 5001, M = 100, ch = 4;
+    std::vector<float> gr(M), gi(M);
+    orc_gen_uniform(0x5EED0B00, gr.data(), gi.data(), (int64_t)M);
+    std::vector<cxf> g(M);
+    for (size_t i = 0; i < M; ++i)
+        g[i] = {gr[i], gi[i]};
+    pcx_gpu::vector<float>       gv(g);
+    pcx_gpu::fxcorr_unit<float>  xc(gv, 512);
+    dynamic_storage<float, 8, 8> sig(L, ch), outp(L, ch);
+    std::vector<std::vector<cxf>> hs(ch);
+    for (size_t c = 0; c < ch; ++c) {
+        std::vector<float> fr(L), fi(L);
+        orc_gen_normal(0x5EED0C00 + c, 1.0, fr.data(), fi.data(), (int64_t)L);
+        hs[c].resize(L);
+        for (size_t i = 0; i < L; ++i)
+            hs[c][i] = {fr[i], fi[i]};
+        sig.assign(c, hs[c].data(), L);
+    }
+    xc(outp, sig);
+    xc(sig);
+    pcx_synchronize(nullptr);
+    for (size_t c = 0; c < ch; ++c) {
+        std::vector<double> fr(L), fi(L), g2r(M), g2i(M), outr(L), outi(L);
+        for (size_t i = 0; i < L; ++i)
+            fr[i] = hs[c][i].real(), fi[i] = hs[c][i].imag();
+        for (size_t i = 0; i < M; ++i)
+            g2r[i] = gr[i], g2i[i] = gi[i];
+        orc_xcorr_naive(fr.data(), fi.data(), (int64_t)L, g2r.data(), g2i.data(), (int64_t)M, outr.data(), outi.data());
+        std::vector<cxd> want(L);
+        for (size_t i = 0; i < L; ++i)
+            want[i] = {outr[i], outi[i]};
+        double e1 = rel_l2(outp.to_host(c), want), e2 = rel_l2(sig.to_host(c), want);
+        CHECK(e1 < 4e-6 && e2 < 4e-6, "md storage fxcorr channel %zu: %.3g %.3g", c, e1, e2);
+    }
 }
 
 int main() {
