@@ -951,6 +951,8 @@ extern "C" int pcx_fxcorr_exec_batch(pcx_fxcorr_t x, float* dst, int64_t dst_pac
         return fail(PCX_ERR_SIZE_MISMATCH, "fft size (which is " + s(x->n) + ") leaves no output per block for a kernel of " +
                                                s(x->m) + " taps at pack " + s(src_pack));
     const bool inplace = (const float*)dst == src;
+    if (channels > 1 && (src_dist < footprint(len, src_pack) || dst_dist < footprint(len, dst_pack)))
+        return fail(PCX_ERR_INVALID_ARGUMENT, "channel distance is smaller than one signal");
     if (inplace && (dst_pack != src_pack || dst_dist != src_dist))
         return fail(PCX_ERR_INPLACE_REPACK, "cannot perform in-place repack from source pack size (which is " +
                                                 s(src_pack) + ") to destination pack size (which is " + s(dst_pack) + ")");

@@ -156,3 +156,20 @@ def test_bf16x3_correlation(orc, pa):
     xc(fv, out)
     torch.cuda.synchronize()
     assert orc.rel_l2(out.to_complex(), orc.xcorr_naive(f, g)) < 4e-6
+
+
+def test_multichannel_distance_is_checked(orc, pa):
+    """Channels closer than one signal's footprint (or at a negative distance) are refused before
+    any launch."""
+    import torch
+
+    from packed_amd import PcxError
+
+    g = pa.vector.from_complex(orc.gen_uniform(0x5EED0F00, 64))
+    xc = pa.fxcorr_unit(g, 512)
+    buf = torch.zeros(4 * 2 * 1000, dtype=torch.float32, device="cuda")
+    for dist in (1000, -2000):  # footprint of 1000 values at pack 8 is 2000 floats
+        with pytest.raises(PcxError):
+            xc.batch(buf.data_ptr(), buf.data_ptr(), 1000, 2, 8, dist)
+    xc.batch(buf.data_ptr(), buf.data_ptr(), 1000, 2, 8, 2000)  # exactly one footprint apart: fine
+    torch.cuda.synchronize()
