@@ -216,8 +216,8 @@ def main():
         "data": "synthetic (standard-normal re/im from splitmix64 seed 0x5EED0002, generated in HBM)",
         "config": {"workload": f"batched C2C FFT forward, N=2^{args.log2n}, batch {batch}, split re/im "
                                f"(pack=N), natural order, out of place",
-                   "fft_size": n, "batch": batch, "order": "normal", "precision": args.precision,
-                   "element_type": args.dtype},
+                   "fft_size": n, "batch": batch, "order": "normal",
+                   "precision": "fp64" if f64 else args.precision, "element_type": args.dtype},
         "roofline": {
             "bound": "hbm",
             "achieved": round(achieved, 1),
