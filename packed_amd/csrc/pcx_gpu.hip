@@ -251,14 +251,16 @@ FourStepT<double> shape64() {
              &launch_row<N1, G, LD_PERM, ST_CONTIG, TW, true, ORD_BITREV, PREC_FP32, D>, nullptr},
             {&launch_col<N2, C, 0, PREC_FP32, D>, &launch_col<N2, C, TW, PREC_FP32, D>}};
 }
-const std::array<FourStepT<double>, 20> kShapes64 = {
+const std::array<FourStepT<double>, 23> kShapes64 = {
     // defaults (first match per lg); forward ms for a 2 GiB batch measured on MI355X
     shape64<14, 7, 7, 32, 32>(),  shape64<15, 8, 7, 16, 32>(), shape64<16, 8, 8, 16, 16>(),
     shape64<17, 8, 9, 16, 8>(),   // 1.92 ms
     shape64<18, 10, 8, 8, 16>(),  // 2.04
     shape64<19, 10, 9, 8, 8>(),   // 2.52
     shape64<20, 10, 10, 8, 8>(),  // 2.43
-    shape64<21, 12, 9, 2, 8>(),   shape64<22, 13, 9, 1, 8>(),  shape64<23, 13, 10, 1, 4>(),
+    shape64<21, 11, 10, 4, 8>(),  // 3.11
+    shape64<22, 12, 10, 4, 8>(),  // 3.12
+    shape64<23, 13, 10, 2, 4>(),  // 4.91
     // alternatives explored by the sweeps
     shape64<17, 9, 8, 8, 16>(),   // 2.08
     shape64<17, 9, 8, 16, 16>(),  // 1.90 (inverse 2.05)
@@ -269,7 +271,10 @@ const std::array<FourStepT<double>, 20> kShapes64 = {
     shape64<19, 9, 10, 8, 4>(),   // 2.81
     shape64<20, 11, 9, 2, 8>(),   // 3.23
     shape64<20, 10, 10, 4, 4>(),  // 3.15
-    shape64<20, 11, 9, 4, 8>()};  // 2.63
+    shape64<20, 11, 9, 4, 8>(),   // 2.63
+    shape64<21, 12, 9, 2, 8>(),   // 3.59
+    shape64<22, 13, 9, 1, 8>(),   // 4.49
+    shape64<23, 13, 10, 1, 4>()}; // 5.91
 const FourStepT<double>* find_shape64(int lg, int n2, int g, int c) {
     for (const auto& f : kShapes64)
         if (f.lg == lg && (n2 == 0 || f.n2 == n2) && (g == 0 || f.g == g) && (c == 0 || f.c == c))
