@@ -40,7 +40,7 @@ int fail(int code, std::string msg) {
     } while (0)
 
 constexpr int kMaxSinglePass = 14; // N <= 16384 runs in one LDS-resident pass
-constexpr int kMaxLog2       = 24; // four-step limit: N2 <= 1024 columns, N1 <= 16384
+constexpr int kMaxLog2       = 24; // four-step limit: N2 <= 2048 columns, N1 <= 16384
 constexpr int kMaxSinglePass64 = 13; // FP64: N <= 8192 in one pass
 constexpr int kMaxLog2_64      = 23; // FP64 four-step limit
 
@@ -219,15 +219,20 @@ FourStep shape() {
     static_assert(N1 + N2 == LG, "shape");
     return {LG, N1, N2, G, C, pass_set<N1, G, LG>(), col_set<N2, C, LG>()};
 }
-const std::array<FourStep, 20> kShapes = {
+const std::array<FourStep, 23> kShapes = {
     // defaults (first match per lg): measured best on MI355X for 2^16 is N1 = 512, N2 = 128
     shape<15, 8, 7, 32, 64>(), shape<16, 9, 7, 32, 64>(), shape<16, 8, 8, 32, 64>(), shape<17, 9, 8, 32, 64>(), shape<18, 9, 9, 32, 32>(),
     shape<19, 10, 9, 16, 32>(), shape<20, 10, 10, 16, 16>(), shape<21, 11, 10, 8, 16>(), shape<22, 12, 10, 4, 16>(),
-    shape<23, 13, 10, 2, 16>(), shape<24, 14, 10, 1, 16>(),
+    // 2^23, 2^24: more rows per workgroup than the LDS of two workgroups per CU allows (they
+    // spill, but store 4 / 2 floats per segment instead of 2 / 1): 5.03 / 5.87 ms per 2 GiB
+    // forward against 7.47 / 8.94
+    shape<23, 12, 11, 4, 8>(), shape<24, 14, 10, 2, 16>(),
     // alternatives for N = 2^16 explored by the tuning sweeps
     shape<16, 8, 8, 16, 64>(), shape<16, 8, 8, 64, 64>(), shape<16, 8, 8, 32, 32>(), shape<16, 8, 8, 16, 32>(),
     shape<16, 10, 6, 16, 64>(), shape<16, 7, 9, 64, 32>(), shape<16, 6, 10, 64, 16>(),
     shape<16, 9, 7, 32, 32>(), shape<16, 10, 6, 16, 32>(),
+    // alternatives for the largest sizes explored by the sweeps
+    shape<23, 13, 10, 2, 16>(), shape<24, 14, 10, 1, 16>(), shape<24, 13, 11, 2, 8>(),
 };
 // FP64 shapes, 2^14 .. 2^23, 256-512 threads per workgroup.  C and G (doubles per column segment
 // and per row store segment) are 16 up to 2^16 with about 64-80 KiB of LDS per workgroup (two
