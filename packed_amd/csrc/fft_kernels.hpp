@@ -1004,26 +1004,25 @@ __device__ __forceinline__ uint64_t smix(uint64_t z) {
 }
 template<class T>
 __global__ void k_generate(T* dst, int lp, int64_t n, uint64_t seed, int dist, double sigma) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n)
-        return;
     const uint64_t gam = 0x9E3779B97F4A7C15ULL;
-    double         u1  = (double)(smix(seed + (uint64_t)(2 * i + 1) * gam) >> 11) * (1.0 / 9007199254740992.0);
-    double         u2  = (double)(smix(seed + (uint64_t)(2 * i + 2) * gam) >> 11) * (1.0 / 9007199254740992.0);
-    T              re, im;
-    if (dist == 0) {
-        re = (T)(2.0 * u1 - 1.0);
-        im = (T)(2.0 * u2 - 1.0);
-    } else {
-        double r = sqrt(-2.0 * log(1.0 - u1));
-        double s, c;
-        sincospi(2.0 * u2, &s, &c);
-        re = (T)(sigma * r * c);
-        im = (T)(sigma * r * s);
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        double u1 = (double)(smix(seed + (uint64_t)(2 * i + 1) * gam) >> 11) * (1.0 / 9007199254740992.0);
+        double u2 = (double)(smix(seed + (uint64_t)(2 * i + 2) * gam) >> 11) * (1.0 / 9007199254740992.0);
+        T      re, im;
+        if (dist == 0) {
+            re = (T)(2.0 * u1 - 1.0);
+            im = (T)(2.0 * u2 - 1.0);
+        } else {
+            double r = sqrt(-2.0 * log(1.0 - u1));
+            double s, c;
+            sincospi(2.0 * u2, &s, &c);
+            re = (T)(sigma * r * c);
+            im = (T)(sigma * r * s);
+        }
+        int64_t o                   = i + ((i >> lp) << lp);
+        dst[o]                      = re;
+        dst[o + ((int64_t)1 << lp)] = im;
     }
-    int64_t o                   = i + ((i >> lp) << lp);
-    dst[o]                      = re;
-    dst[o + ((int64_t)1 << lp)] = im;
 }
 
 } // namespace pcxk

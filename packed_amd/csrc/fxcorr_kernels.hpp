@@ -40,6 +40,7 @@ struct XcArgs {
     int          src_lp, dst_lp;
     int64_t      src_dist, dst_dist; // floats between channels
     int64_t      channels;           // blocks of channel c are nblocks * c .. nblocks * (c + 1) - 1
+    int64_t      blk0;               // first block of this launch (a launch covers < 2^32 work-items)
 };
 
 template<int LA, int G>
@@ -96,7 +97,7 @@ void k_fxcorr(XcArgs p) {
     // a wavefront lies in one block when TA >= 64: keep the block's offsets in SGPRs
     const int     g    = TA >= 64 ? __builtin_amdgcn_readfirstlane(t / TA) : t / TA;
     const int     jt   = t % TA;
-    const int64_t bg   = xcd_item(0) * G + g; // block over all channels; neighbours share one L2
+    const int64_t bg   = p.blk0 + xcd_item(0) * G + g; // block over all channels; neighbours share one L2
     const bool    ok   = bg < p.nblocks * p.channels;
     const int64_t ch   = ok ? bg / p.nblocks : 0;
     const int64_t b    = ok ? bg - ch * p.nblocks : 0;
@@ -199,15 +200,16 @@ void k_fxcorr(XcArgs p) {
 // when the correlation runs in place.
 __global__ void k_fxcorr_halo(const float* src, int src_lp, int64_t src_dist, int64_t len, int64_t step, int64_t hlen,
                               int64_t nblocks, int64_t channels, float* hre, float* him) {
-    int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= nblocks * channels * hlen)
-        return;
-    int64_t bg = idx / hlen, h = idx % hlen;
-    int64_t ch = bg / nblocks, b = bg % nblocks;
-    int64_t i  = b * step + step + h;
-    cf      x  = i < len ? ld_cx64(src + ch * src_dist, i, src_lp) : cf{0.f, 0.f};
-    hre[idx]  = x.x;
-    him[idx]  = x.y;
+    const int64_t total = nblocks * channels * hlen;
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (int64_t)gridDim.x * blockDim.x) {
+        int64_t bg = idx / hlen, h = idx % hlen;
+        int64_t ch = bg / nblocks, b = bg % nblocks;
+        int64_t i  = b * step + step + h;
+        cf      x  = i < len ? ld_cx64(src + ch * src_dist, i, src_lp) : cf{0.f, 0.f};
+        hre[idx]   = x.x;
+        him[idx]   = x.y;
+    }
 }
 
 } // namespace pcxk

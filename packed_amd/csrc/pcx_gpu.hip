@@ -96,13 +96,19 @@ using XcLaunch  = void (*)(const XcArgs&, hipStream_t);
 // grid_cap > 0 bounds the grid; the kernels then loop over their work (grid-stride).  Grids are
 // rounded up to a multiple of 8 for the XCD-aware item order (spare workgroups exit at once).
 int64_t grid8(int64_t n) { return (n + 7) & ~(int64_t)7; }
+// One dispatch covers at most 2^32 - 1 work-items (the 32-bit grid size of an AQL packet).
+// Grid-stride kernels take a grid capped to that (a multiple of 8) and loop over the rest.
+int64_t fit_grid(int64_t nwg, int64_t threads) {
+    const int64_t cap = ((int64_t)0xFFFFFFFF / threads) & ~(int64_t)7;
+    return nwg < cap ? nwg : cap;
+}
 template<int LA, int G, int LD, int ST, int TW, bool UNI = (Plan<LA>::TA >= 64), int ORD = 0, int PREC = PREC_FP32,
          class T = float>
 void launch_row(const RowArgsT<T>& a, int64_t nrows, int64_t grid_cap, hipStream_t st) {
     int64_t nwg = (nrows + G - 1) / G;
     if (grid_cap > 0 && nwg > grid_cap)
         nwg = grid_cap;
-    nwg = grid8(nwg);
+    nwg = fit_grid(grid8(nwg), G * Plan<LA, false, T>::TA);
     hipLaunchKernelGGL((k_row<LA, G, LD, ST, TW, UNI, ORD, PREC, T>), dim3((unsigned)nwg),
                        dim3(G * Plan<LA, false, T>::TA), 0, st, a);
 }
@@ -111,20 +117,27 @@ void launch_col(const ColArgsT<T>& a, int64_t grid_cap, hipStream_t st) {
     int64_t nwg = a.ntiles;
     if (grid_cap > 0 && nwg > grid_cap)
         nwg = grid_cap;
-    nwg = grid8(nwg);
+    nwg = fit_grid(grid8(nwg), C * Plan<LA, false, T>::TA);
     hipLaunchKernelGGL((k_col<LA, C, TW, PREC, T>), dim3((unsigned)nwg), dim3(C * Plan<LA, false, T>::TA), 0, st, a);
 }
 template<int LA, int PREC = PREC_FP32>
 void launch_xc(const XcArgs& a, hipStream_t st) {
-    constexpr int G   = xc_rows<LA>();
-    int64_t       nwg = grid8((a.nblocks * a.channels + G - 1) / G);
+    constexpr int G     = xc_rows<LA>();
+    constexpr int NT    = G * xc_threads<LA>();
+    const int64_t total = a.nblocks * a.channels;
     // 32-bit buffer offsets cover a channel when its packed footprint stays below 2^29 floats
     int  lp    = a.src_lp > a.dst_lp ? a.src_lp : a.dst_lp;
     bool small = pidx64(a.len + (int64_t)(1 << LA), lp) < ((int64_t)1 << 29) && a.src_lp < 29 && a.dst_lp < 29;
-    if (small)
-        hipLaunchKernelGGL((k_fxcorr<LA, G, true, PREC>), dim3((unsigned)nwg), dim3(G * xc_threads<LA>()), 0, st, a);
-    else
-        hipLaunchKernelGGL((k_fxcorr<LA, G, false, PREC>), dim3((unsigned)nwg), dim3(G * xc_threads<LA>()), 0, st, a);
+    // one workgroup per G blocks; launches of at most 2^32 - 1 work-items, in block order
+    const int64_t per = fit_grid(grid8((total + G - 1) / G), NT) * G;
+    XcArgs        s   = a;
+    for (s.blk0 = 0; s.blk0 < total; s.blk0 += per) {
+        const int64_t nwg = grid8((std::min(per, total - s.blk0) + G - 1) / G);
+        if (small)
+            hipLaunchKernelGGL((k_fxcorr<LA, G, true, PREC>), dim3((unsigned)nwg), dim3(NT), 0, st, s);
+        else
+            hipLaunchKernelGGL((k_fxcorr<LA, G, false, PREC>), dim3((unsigned)nwg), dim3(NT), 0, st, s);
+    }
 }
 
 // Single-pass variants: 0 natural in/out; 1, 2 permuted store (forward, bit_reversed /
@@ -991,7 +1004,7 @@ extern "C" int pcx_fxcorr_exec_batch(pcx_fxcorr_t x, float* dst, int64_t dst_pac
             x->halo_cap = need;
         }
         a.hre = x->halo, a.him = x->halo + need;
-        int64_t nb = (need + 255) / 256;
+        int64_t nb = std::min<int64_t>((need + 255) / 256, (int64_t)1 << 20);
         hipLaunchKernelGGL(k_fxcorr_halo, dim3((unsigned)nb), dim3(256), 0, st, src, a.src_lp, src_dist, len, step, a.hlen,
                            nblocks, channels, x->halo, x->halo + need);
         HIPCHK(hipGetLastError());
@@ -1049,7 +1062,7 @@ extern "C" int pcx_generate(float* dst, int64_t pack, int64_t n, uint64_t seed, 
         return e;
     if (n <= 0)
         return PCX_SUCCESS;
-    int64_t nb = (n + 255) / 256;
+    int64_t nb = std::min<int64_t>((n + 255) / 256, (int64_t)1 << 20);
     hipLaunchKernelGGL(k_generate<float>, dim3((unsigned)nb), dim3(256), 0, (hipStream_t)stream, dst, ilog2(pack), n, seed,
                        dist, sigma);
     HIPCHK(hipGetLastError());
@@ -1063,7 +1076,7 @@ extern "C" int pcx_generate_f64(double* dst, int64_t pack, int64_t n, uint64_t s
         return e;
     if (n <= 0)
         return PCX_SUCCESS;
-    int64_t nb = (n + 255) / 256;
+    int64_t nb = std::min<int64_t>((n + 255) / 256, (int64_t)1 << 20);
     hipLaunchKernelGGL(k_generate<double>, dim3((unsigned)nb), dim3(256), 0, (hipStream_t)stream, dst, ilog2(pack), n,
                        seed, dist, sigma);
     HIPCHK(hipGetLastError());
@@ -1528,7 +1541,8 @@ void launch_across(const AcrossArgs& a, int64_t grid, hipStream_t st) {
     int64_t       n = (a.len + C - 1) / C;
     if (n > grid)
         n = grid;
-    hipLaunchKernelGGL((k_across<LA, C>), dim3((unsigned)grid8(n)), dim3(C * Plan<LA>::TA), 0, st, a);
+    hipLaunchKernelGGL((k_across<LA, C>), dim3((unsigned)fit_grid(grid8(n), C * Plan<LA>::TA)), dim3(C * Plan<LA>::TA), 0,
+                       st, a);
 }
 template<int... L>
 constexpr auto across_table(std::integer_sequence<int, L...>) {
