@@ -38,8 +38,9 @@ def parse():
     ap.add_argument("--quick", action="store_true", help="time the forward step only (for profiler runs)")
     ap.add_argument("--log2n", type=int, default=N_LOG2)
     ap.add_argument("--batch", type=int, default=BATCH)
-    ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16x3"],
-                    help="arithmetic of the default workload (bf16x3: split-BF16 matrix-core butterflies)")
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16x3", "fp16x2"],
+                    help="arithmetic of the default workload (bf16x3 / fp16x2: split-BF16 / split-FP16 "
+                         "matrix-core butterflies)")
     ap.add_argument("--dtype", default="float32", choices=["float32", "float64"],
                     help="element type of the default workload (float64: the double-precision path)")
     ap.add_argument("--copy", action="store_true",
@@ -212,7 +213,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64" if f64 else ("f32" if args.precision == "fp32" else
-                                    "f32 (bf16x3 split operands, f32 accumulation)"),
+                                    f"f32 ({args.precision} split operands, f32 accumulation)"),
         "data": "synthetic (standard-normal re/im from splitmix64 seed 0x5EED0002, generated in HBM)",
         "config": {"workload": f"batched C2C FFT forward, N=2^{args.log2n}, batch {batch}, split re/im "
                                f"(pack=N), natural order, out of place",
@@ -484,8 +485,8 @@ def run_config4(args):
 
 
 def run_config5(args):
-    """Split-precision matrix-core form (three-term BF16 operands, FP32 accumulation) against the
-    plain FP32 path: batched C2C N in {2^12, 2^16, 2^20} filling 2 GiB, plus 512-channel
+    """Split-precision matrix-core forms (three-term BF16 operands, or hi + lo FP16 operands
+    scaled per butterfly; FP32 accumulation) against the plain FP32 path: batched C2C N in {2^12, 2^16, 2^20} filling 2 GiB, plus 512-channel
     correlation of 2^20 samples with a 1024-tap kernel (block size chosen by timing)."""
     import numpy as np
     import torch
@@ -499,14 +500,15 @@ def run_config5(args):
     y = torch.empty(2 * pts, dtype=torch.float32, device="cuda")
     st = torch.cuda.current_stream().cuda_stream
     check(lib().pcx_generate(x.data_ptr(), 1, pts, 0x5EED0005, 1, 1.0, st))
-    res = {"metric": "split-BF16 MFMA vs FP32 FFT, GFLOP/s (5N*log2N), 2 GiB batches", "unit": "GFLOP/s",
+    res = {"metric": "split-BF16 / split-FP16 MFMA vs FP32 FFT, GFLOP/s (5N*log2N), 2 GiB batches",
+           "unit": "GFLOP/s",
            "higher_is_better": True, "sizes": {},
            "config": {"workload": "batched C2C FFT forward, N in {2^12, 2^16, 2^20}, 2^28 points, split re/im"}}
     for lg in (12, 16, 20):
         n = 1 << lg
         batch = pts // n
         row = {}
-        for prec in ("fp32", "bf16x3"):
+        for prec in ("fp32", "bf16x3", "fp16x2"):
             u = pa.fft_unit(n, precision=prec)
             ms = time_steps(lambda: u.batch(y.data_ptr(), x.data_ptr(), batch, n, 2 * n, n, 2 * n), 10, 2)
             xin = x[:2 * n].cpu().numpy()
@@ -516,6 +518,7 @@ def run_config5(args):
             row[prec] = {"ms": round(ms, 4), "gflops": round(5 * n * lg * batch / (ms * 1e-3) / 1e9, 1),
                          "rel_l2_vs_fp64": float(f"{err:.3e}")}
         row["speedup_bf16x3_over_fp32"] = round(row["fp32"]["ms"] / row["bf16x3"]["ms"], 3)
+        row["speedup_fp16x2_over_fp32"] = round(row["fp32"]["ms"] / row["fp16x2"]["ms"], 3)
         res["sizes"][f"2^{lg}"] = row
     del x, y
     torch.cuda.empty_cache()
@@ -529,7 +532,7 @@ def run_config5(args):
     g = np.exp(1j * np.pi * (np.real(O.gen_uniform(0x5EED0016, M)).astype(np.float64) + 1.0)).astype(np.complex64)
     gv = pa.vector.from_complex(g, pack)
     xres = {}
-    for prec in ("fp32", "bf16x3"):
+    for prec in ("fp32", "bf16x3", "fp16x2"):
         best = None
         for N in (4096, 8192, 16384):
             xc = pa.fxcorr_unit(gv, N, precision=prec)

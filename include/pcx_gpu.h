@@ -61,9 +61,11 @@ int     pcx_plan_destroy(pcx_plan_t plan);
 /* fft.hpp:620-622 size() */
 int64_t pcx_plan_size(pcx_plan_t plan);
 int     pcx_plan_order(pcx_plan_t plan);
-/* Arithmetic of the plan's radix-16 butterflies: PCX_PREC_FP32 (default) or PCX_PREC_BF16X3
- * (matrix cores, three-term BF16 split operands, FP32 accumulation; normal order, N in
- * [2^4, 2^20]).  No reference counterpart: the reference computes in FP32 only. */
+/* Arithmetic of the plan's radix-16 butterflies: PCX_PREC_FP32 (default), PCX_PREC_BF16X3
+ * (matrix cores, three-term BF16 split operands, FP32 accumulation) or PCX_PREC_FP16X2 (matrix
+ * cores, hi + lo FP16 operands scaled by a power of two per butterfly, FP32 accumulation); the
+ * split forms cover the normal order, N in [2^4, 2^20].  No reference counterpart: the
+ * reference computes in FP32 only. */
 int     pcx_plan_set_precision(pcx_plan_t plan, int precision);
 int     pcx_plan_precision(pcx_plan_t plan);
 
@@ -133,8 +135,8 @@ int64_t pcx_fxcorr_step(pcx_fxcorr_t xc, int64_t src_pack);
  * dst + c*dst_dist floats, all correlated with the same kernel spectrum in one launch. */
 int pcx_fxcorr_exec_batch(pcx_fxcorr_t xc, float* dst, int64_t dst_pack, int64_t dst_dist, const float* src,
                           int64_t src_pack, int64_t src_dist, int64_t len, int64_t channels, void* stream);
-/* PCX_PREC_FP32 (default) or PCX_PREC_BF16X3 (matrix-core radix-16 butterflies in both transforms;
- * no reference counterpart). */
+/* PCX_PREC_FP32 (default), PCX_PREC_BF16X3 or PCX_PREC_FP16X2 (matrix-core radix-16 butterflies
+ * in both transforms; no reference counterpart). */
 int pcx_fxcorr_set_precision(pcx_fxcorr_t xc, int precision);
 
 /* ------------------------------------------------------------------ uneven batches

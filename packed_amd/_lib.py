@@ -24,7 +24,7 @@ ERR_HIP = 8
 ERR_OUT_OF_MEMORY = 9
 
 ORDER = {"normal": 0, "bit_reversed": 1, "unordered": 2}
-PRECISION = {"fp32": 0, "bf16x3": 1}
+PRECISION = {"fp32": 0, "bf16x3": 1, "fp16x2": 2}
 
 class Job(C.Structure):
     """pcx_job (include/pcx_gpu.h)."""

@@ -626,8 +626,8 @@ void k_row(RowArgsT<T> p) {
         PassTw<TW, T>::fill(lptw, p.twl, p.twh, p.tw_ls, p.tw_lg, threadIdx.x, NT);
     __syncthreads();
     MfmaDft16 mf;
-    if constexpr (PREC == PREC_BF16X3)
-        mf.init();
+    if constexpr (PREC != PREC_FP32)
+        mf.template init<PREC>();
     const int64_t nwg = (p.nrows + G - 1) / G;
     for (int64_t base = 0; base < nwg; base += gridDim.x) {
         const int64_t wg = xcd_item(base);
@@ -696,8 +696,8 @@ void k_dif(RowArgs p) {
     TwTable<LM>::fill(ltw, p.twA, jt, TA);
     PassTw<TWN>::fill(lptw, p.twl, p.twh, p.tw_ls, p.tw_lg, jt, TA);
     MfmaDft16 mf;
-    if constexpr (PREC == PREC_BF16X3)
-        mf.init();
+    if constexpr (PREC != PREC_FP32)
+        mf.template init<PREC>();
     auto addr = [&](int bb, int o) { return bb + (bb >> 5) + o + (o >> 5); };
 
     // ------------------------------------------------------------------ load + radix-R step
@@ -898,8 +898,8 @@ void k_col(ColArgsT<T> p) {
         PassTw<TW, T>::fill(lptw, p.twl, p.twh, p.tw_ls, p.tw_lg, threadIdx.x, NT);
     __syncthreads();
     MfmaDft16 mf;
-    if constexpr (PREC == PREC_BF16X3)
-        mf.init();
+    if constexpr (PREC != PREC_FP32)
+        mf.template init<PREC>();
     for (int64_t base = 0; base < p.ntiles; base += gridDim.x) {
         const int64_t tile = xcd_item(base);
         if (tile >= p.ntiles)

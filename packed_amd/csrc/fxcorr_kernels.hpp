@@ -91,8 +91,8 @@ void k_fxcorr(XcArgs p) {
     __syncthreads();
 
     MfmaDft16 mf;
-    if constexpr (PREC == PREC_BF16X3)
-        mf.init();
+    if constexpr (PREC != PREC_FP32)
+        mf.template init<PREC>();
     const int     t    = threadIdx.x;
     // a wavefront lies in one block when TA >= 64: keep the block's offsets in SGPRs
     const int     g    = TA >= 64 ? __builtin_amdgcn_readfirstlane(t / TA) : t / TA;

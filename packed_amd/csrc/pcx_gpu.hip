@@ -162,19 +162,24 @@ constexpr auto xc_table(std::integer_sequence<int, L...>) {
 }
 // index LA-1 for LA in [1, 14]
 const auto kSingle = single_table(std::make_integer_sequence<int, kMaxSinglePass>{});
-// split-BF16 matrix-core variants (natural order), index LA-4 for LA in [4, 14]
-template<int... L>
+// split-precision matrix-core variants (natural order): index [precision - 1][LA - 4] for
+// precision BF16X3 / FP16X2 and LA in [4, 14]
+template<int PREC, int... L>
 constexpr auto single_mfma_table(std::integer_sequence<int, L...>) {
     return std::array<RowLaunch, sizeof...(L)>{
-        &launch_row<L + 4, default_rows<L + 4>(), LD_CONTIG, ST_CONTIG, 0, (Plan<L + 4>::TA >= 64), 0, PREC_BF16X3>...};
+        &launch_row<L + 4, default_rows<L + 4>(), LD_CONTIG, ST_CONTIG, 0, (Plan<L + 4>::TA >= 64), 0, PREC>...};
 }
-const auto kSingleMfma = single_mfma_table(std::make_integer_sequence<int, kMaxSinglePass - 3>{});
-const auto kXc     = xc_table(std::make_integer_sequence<int, kMaxSinglePass>{});
-template<int... L>
+const std::array<std::array<RowLaunch, kMaxSinglePass - 3>, 2> kSingleMfma = {
+    single_mfma_table<PREC_BF16X3>(std::make_integer_sequence<int, kMaxSinglePass - 3>{}),
+    single_mfma_table<PREC_FP16X2>(std::make_integer_sequence<int, kMaxSinglePass - 3>{})};
+const auto kXc = xc_table(std::make_integer_sequence<int, kMaxSinglePass>{});
+template<int PREC, int... L>
 constexpr auto xc_mfma_table(std::integer_sequence<int, L...>) {
-    return std::array<XcLaunch, sizeof...(L)>{&launch_xc<L + 4, PREC_BF16X3>...};
+    return std::array<XcLaunch, sizeof...(L)>{&launch_xc<L + 4, PREC>...};
 }
-const auto kXcMfma = xc_mfma_table(std::make_integer_sequence<int, kMaxSinglePass - 3>{}); // LA 4 .. 14
+const std::array<std::array<XcLaunch, kMaxSinglePass - 3>, 2> kXcMfma = {
+    xc_mfma_table<PREC_BF16X3>(std::make_integer_sequence<int, kMaxSinglePass - 3>{}),
+    xc_mfma_table<PREC_FP16X2>(std::make_integer_sequence<int, kMaxSinglePass - 3>{})};
 
 using PassSet = std::array<RowLaunch, 6>;
 // Pass-twiddle table capacity in LDS for N = 2^LG (2: too large, read from global memory).
@@ -300,19 +305,22 @@ const FourStepT<double>* find_shape64(int lg, int n2, int g, int c) {
     return nullptr;
 }
 
-// split-BF16 matrix-core variants of the default shapes for 2^15 .. 2^20 (natural order):
-// rows {forward, inverse}, cols {inverse, forward}
+// split-precision matrix-core variants of the default shapes for 2^15 .. 2^20 (natural order):
+// index [precision - 1], rows {forward, inverse}, cols {inverse, forward}
 struct FourStepMfma {
-    int                      lg;
-    std::array<RowLaunch, 2> rows;
-    std::array<ColLaunch, 2> cols;
+    int                                     lg;
+    std::array<std::array<RowLaunch, 2>, 2> rows;
+    std::array<std::array<ColLaunch, 2>, 2> cols;
 };
 template<int LG, int N1, int N2, int G, int C>
 FourStepMfma shape_mfma() {
     return {LG,
-            {&launch_row<N1, G, LD_CONTIG, ST_STRIDED, 0, true, 0, PREC_BF16X3>,
-             &launch_row<N1, G, LD_STRIDED, ST_CONTIG, tw_cap<LG>(), true, 0, PREC_BF16X3>},
-            {&launch_col<N2, C, 0, PREC_BF16X3>, &launch_col<N2, C, tw_cap<LG>(), PREC_BF16X3>}};
+            {{{&launch_row<N1, G, LD_CONTIG, ST_STRIDED, 0, true, 0, PREC_BF16X3>,
+               &launch_row<N1, G, LD_STRIDED, ST_CONTIG, tw_cap<LG>(), true, 0, PREC_BF16X3>},
+              {&launch_row<N1, G, LD_CONTIG, ST_STRIDED, 0, true, 0, PREC_FP16X2>,
+               &launch_row<N1, G, LD_STRIDED, ST_CONTIG, tw_cap<LG>(), true, 0, PREC_FP16X2>}}},
+            {{{&launch_col<N2, C, 0, PREC_BF16X3>, &launch_col<N2, C, tw_cap<LG>(), PREC_BF16X3>},
+              {&launch_col<N2, C, 0, PREC_FP16X2>, &launch_col<N2, C, tw_cap<LG>(), PREC_FP16X2>}}}};
 }
 const std::array<FourStepMfma, 6> kShapesMfma = {shape_mfma<15, 8, 7, 32, 64>(),  shape_mfma<16, 9, 7, 32, 64>(),
                                                  shape_mfma<17, 9, 8, 32, 64>(),  shape_mfma<18, 9, 9, 32, 32>(),
@@ -592,12 +600,12 @@ extern "C" int pcx_plan_set_precision(pcx_plan_t p, int precision) {
         p->precision = precision;
         return PCX_SUCCESS;
     }
-    if (precision != PCX_PREC_BF16X3)
+    if (precision != PCX_PREC_BF16X3 && precision != PCX_PREC_FP16X2)
         return fail(PCX_ERR_UNSUPPORTED, "precision " + s(precision) + " is not implemented");
     if (p->order != PCX_ORDER_NORMAL)
-        return fail(PCX_ERR_UNSUPPORTED, "split-BF16 transforms are built for the normal order only");
+        return fail(PCX_ERR_UNSUPPORTED, "split-precision transforms are built for the normal order only");
     if (p->lg < 4 || p->lg > 20 || (p->two_pass && !find_mfma(p->lg, p->shape)))
-        return fail(PCX_ERR_UNSUPPORTED, "split-BF16 transforms cover 2^4 .. 2^20 (default four-step shapes)");
+        return fail(PCX_ERR_UNSUPPORTED, "split-precision transforms cover 2^4 .. 2^20 (default four-step shapes)");
     p->precision = precision;
     return PCX_SUCCESS;
 }
@@ -688,8 +696,8 @@ int run_batch(pcx_plan_t p, int inverse, int normalized, T* dst, int64_t dst_pac
         a.src_offs = src_offs, a.dst_offs = dst_offs;
         if constexpr (F64)
             kSingle64[p->lg - 1][single_variant(inverse, p->order)](a, batch, 0, st);
-        else if (p->precision == PCX_PREC_BF16X3)
-            kSingleMfma[p->lg - 4](a, batch, 0, st);
+        else if (p->precision != PCX_PREC_FP32)
+            kSingleMfma[p->precision - 1][p->lg - 4](a, batch, 0, st);
         else
             kSingle[p->lg - 1][single_variant(inverse, p->order)](a, batch, 0, st);
         HIPCHK(hipGetLastError());
@@ -736,12 +744,12 @@ int run_batch(pcx_plan_t p, int inverse, int normalized, T* dst, int64_t dst_pac
     ColLaunchT<T> colf = shape->cols[inverse ? 0 : 1];
     RowLaunchT<T> rowf = shape->rows[(inverse ? 3 : 0) + p->order];
     if constexpr (!F64)
-    if (p->precision == PCX_PREC_BF16X3) {
+    if (p->precision != PCX_PREC_FP32) {
         const FourStepMfma* fm = find_mfma(p->lg, p->shape);
         if (!fm)
-            return fail(PCX_ERR_UNSUPPORTED, "no split-BF16 kernels for this four-step shape");
-        colf = fm->cols[inverse ? 0 : 1];
-        rowf = fm->rows[inverse ? 1 : 0];
+            return fail(PCX_ERR_UNSUPPORTED, "no split-precision kernels for this four-step shape");
+        colf = fm->cols[p->precision - 1][inverse ? 0 : 1];
+        rowf = fm->rows[p->precision - 1][inverse ? 1 : 0];
     }
     const int64_t n1 = (int64_t)1 << p->n1_lg, n2 = (int64_t)1 << p->n2_lg;
     for (int64_t b0 = 0; b0 < batch; b0 += chunk) {
@@ -949,10 +957,10 @@ extern "C" int64_t pcx_fxcorr_step(pcx_fxcorr_t x, int64_t src_pack) {
 extern "C" int pcx_fxcorr_set_precision(pcx_fxcorr_t x, int precision) {
     if (!x)
         return fail(PCX_ERR_INVALID_ARGUMENT, "fxcorr handle is null");
-    if (precision != PCX_PREC_FP32 && precision != PCX_PREC_BF16X3)
+    if (precision != PCX_PREC_FP32 && precision != PCX_PREC_BF16X3 && precision != PCX_PREC_FP16X2)
         return fail(PCX_ERR_UNSUPPORTED, "precision " + s(precision) + " is not implemented");
-    if (precision == PCX_PREC_BF16X3 && x->lg < 4)
-        return fail(PCX_ERR_UNSUPPORTED, "split-BF16 correlation needs fft_size >= 16");
+    if (precision != PCX_PREC_FP32 && x->lg < 4)
+        return fail(PCX_ERR_UNSUPPORTED, "split-precision correlation needs fft_size >= 16");
     x->precision = precision;
     return PCX_SUCCESS;
 }
@@ -1009,8 +1017,8 @@ extern "C" int pcx_fxcorr_exec_batch(pcx_fxcorr_t x, float* dst, int64_t dst_pac
                            nblocks, channels, x->halo, x->halo + need);
         HIPCHK(hipGetLastError());
     }
-    if (x->precision == PCX_PREC_BF16X3)
-        kXcMfma[x->lg - 4](a, st);
+    if (x->precision != PCX_PREC_FP32)
+        kXcMfma[x->precision - 1][x->lg - 4](a, st);
     else
         kXc[x->lg - 1](a, st);
     HIPCHK(hipGetLastError());

@@ -150,13 +150,14 @@ __device__ __forceinline__ void stage_write(const cx<T>* v, T* lre, T* lim, int 
 }
 
 // Matrix-core DFT16 of v (FP32 only; the FP64 path never selects it).
-template<class T>
+template<int PREC, class T>
 __device__ __forceinline__ void dft16_mfma(const MfmaDft16* mf, cx<T>* v) {
     if constexpr (sizeof(T) == 4)
-        mf->dft16(v);
+        mf->template dft16<PREC>(v);
 }
 
-// PREC_BF16X3 runs the radix-16 butterflies on the matrix cores (mf holds the operands).
+// PREC_BF16X3 / PREC_FP16X2 run the radix-16 butterflies on the matrix cores (mf holds the
+// operands).
 // With Plan::HALF, lre and lim are the same array and each exchange runs in two halves.
 // T = double runs the same schedule in FP64 (PREC is then PREC_FP32, the plain butterflies).
 template<int LA, bool REV, class Addr, int PREC = PREC_FP32, class T = float, int PX = 0>
@@ -188,8 +189,8 @@ __device__ __forceinline__ void stockham(cx<T>* v, T* lre, T* lim, int jt0, cons
                 for (int r = 1; r < R; ++r)
                     v[m * R + r] = cmul(v[m * R + r], TwTable<LA, T>::get(ltw, (uint32_t)(r * k) << shift));
             }
-            if constexpr (R == 16 && PREC == PREC_BF16X3)
-                dft16_mfma(mf, v + m * R);
+            if constexpr (R == 16 && PREC != PREC_FP32)
+                dft16_mfma<PREC>(mf, v + m * R);
             else
                 dft<R>(v + m * R);
             if constexpr (NB > 1)

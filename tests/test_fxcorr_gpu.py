@@ -142,15 +142,16 @@ def test_multi_channel(orc, pa, inplace):
         assert orc.rel_l2(got, orc.xcorr_naive(fs[c], g)) < 4e-6, c
 
 
-def test_bf16x3_correlation(orc, pa):
-    """Split-BF16 matrix-core form of the fused correlation."""
+@pytest.mark.parametrize("precision", ["bf16x3", "fp16x2"])
+def test_split_correlation(orc, pa, precision):
+    """Split-precision matrix-core forms of the fused correlation."""
     import torch
 
     L, M, N = 70000, 4096, 16384
     f = orc.gen_normal(0x5EED0003, L, np.sqrt(0.5)).astype(np.complex128)
     g = np.exp(1j * np.pi * np.real(orc.gen_uniform(0x5EED0013, M)))
     gv = pa.vector.from_complex(g.astype(np.complex64), 8)
-    xc = pa.fxcorr_unit(gv, N, precision="bf16x3")
+    xc = pa.fxcorr_unit(gv, N, precision=precision)
     fv = pa.vector.from_complex(f.astype(np.complex64), 8)
     out = pa.vector(L, 8)
     xc(fv, out)

@@ -1,5 +1,8 @@
-"""GPU parity of the split-BF16 matrix-core form (pcx_plan_set_precision(PCX_PREC_BF16X3)):
-the radix-16 butterflies run as v_mfma_f32_32x32x16_bf16 products of three-term BF16 splits.
+"""GPU parity of the split-precision matrix-core forms (pcx_plan_set_precision):
+* PCX_PREC_BF16X3: the radix-16 butterflies as v_mfma_f32_32x32x16_bf16 products of three-term
+  BF16 splits;
+* PCX_PREC_FP16X2: v_mfma_f32_32x32x16_f16 products of hi + lo FP16 splits, each butterfly's
+  inputs scaled by a power of two into the FP16 range.
 Target: relative L2 error <= 1e-6 against the FP64 oracle, in both directions."""
 import numpy as np
 import pytest
@@ -16,8 +19,9 @@ def pa():
     return packed_amd
 
 
+@pytest.mark.parametrize("precision", ["bf16x3", "fp16x2"])
 @pytest.mark.parametrize("lg", list(range(4, 21)))
-def test_bf16x3_forward_inverse(orc, pa, lg):
+def test_split_forward_inverse(orc, pa, lg, precision):
     import torch
 
     n = 1 << lg
@@ -27,7 +31,7 @@ def test_bf16x3_forward_inverse(orc, pa, lg):
     s = to_device(buf)
     d = torch.zeros_like(s)
     r = torch.zeros_like(s)
-    unit = pa.fft_unit(n, precision="bf16x3")
+    unit = pa.fft_unit(n, precision=precision)
     unit.batch(d.data_ptr(), s.data_ptr(), batch, n, 2 * n, n, 2 * n)
     unit.batch(r.data_ptr(), d.data_ptr(), batch, n, 2 * n, n, 2 * n, inverse=True)
     torch.cuda.synchronize()
@@ -40,10 +44,38 @@ def test_bf16x3_forward_inverse(orc, pa, lg):
         assert orc.rel_l2(back[b], x) < 2e-6
 
 
-def test_bf16x3_rejects_unsupported(pa):
+@pytest.mark.parametrize("lg", [6, 12, 16])
+def test_fp16x2_dynamic_range(orc, pa, lg):
+    """The per-butterfly power-of-two scaling: transforms at 1e-30, 1e-5, 1 and 1e20 times a
+    normal input (far outside FP16's 6e-8 .. 65504), one that mixes 1e-6 and 1e3 entries, and an
+    all-zero one, each within the bound of its own norm."""
+    import torch
+
+    n = 1 << lg
+    base = [orc.gen_normal(0x5EED0900 + b, n).astype(np.complex128) for b in range(6)]
+    mix = base[4] * np.where(np.arange(n) % 3 == 0, 1e3, 1e-6)
+    xs = [base[0] * 1e-30, base[1] * 1e-5, base[2], base[3] * 1e20, mix, np.zeros(n)]
+    buf = batch_buffer([x.astype(np.complex64) for x in xs], n, 2 * n)
+    s = to_device(buf)
+    d = torch.zeros_like(s)
+    unit = pa.fft_unit(n, precision="fp16x2")
+    unit.batch(d.data_ptr(), s.data_ptr(), len(xs), n, 2 * n, n, 2 * n)
+    torch.cuda.synchronize()
+    got = unpack_batch(d.cpu().numpy(), n, n, 2 * n, len(xs))
+    for b, x in enumerate(xs):
+        x32 = x.astype(np.complex64).astype(np.complex128)
+        if b == len(xs) - 1:
+            assert np.all(got[b] == 0)
+            continue
+        err = orc.rel_l2(got[b], orc.fft64(x32))
+        assert err < 1e-6, (lg, b, err)
+
+
+@pytest.mark.parametrize("precision", ["bf16x3", "fp16x2"])
+def test_split_rejects_unsupported(pa, precision):
     from packed_amd import PcxError
 
     with pytest.raises(PcxError):
-        pa.fft_unit(1024, order="bit_reversed", precision="bf16x3")
+        pa.fft_unit(1024, order="bit_reversed", precision=precision)
     with pytest.raises(PcxError):
-        pa.fft_unit(8, precision="bf16x3")
+        pa.fft_unit(8, precision=precision)
