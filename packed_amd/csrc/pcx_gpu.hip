@@ -237,9 +237,12 @@ FourStep shape() {
     static_assert(N1 + N2 == LG, "shape");
     return {LG, N1, N2, G, C, pass_set<N1, G, LG>(), col_set<N2, C, LG>()};
 }
-const std::array<FourStep, 23> kShapes = {
-    // defaults (first match per lg): measured best on MI355X for 2^16 is N1 = 512, N2 = 128
-    shape<15, 8, 7, 32, 64>(), shape<16, 9, 7, 32, 64>(), shape<16, 8, 8, 32, 64>(), shape<17, 9, 8, 32, 64>(), shape<18, 9, 9, 32, 32>(),
+const std::array<FourStep, 25> kShapes = {
+    // defaults (first match per lg): measured best on MI355X for 2^16 is N1 = 512, N2 = 128 with
+    // 128-column tiles (512-byte load and store segments in the column pass; 128 KiB of LDS, one
+    // workgroup of 1024 threads per CU): 0.778 ms per 2 GiB against 0.855 ms with 64 columns;
+    // 2^15 (four-step forward) 1.61 against 1.70 ms per 2 GiB
+    shape<15, 8, 7, 32, 128>(), shape<16, 9, 7, 32, 128>(), shape<16, 8, 8, 32, 64>(), shape<17, 9, 8, 32, 64>(), shape<18, 9, 9, 32, 32>(),
     shape<19, 10, 9, 16, 32>(), shape<20, 10, 10, 16, 16>(), shape<21, 11, 10, 8, 16>(), shape<22, 12, 10, 4, 16>(),
     // 2^23, 2^24: more rows per workgroup than the LDS of two workgroups per CU allows (they
     // spill, but store 4 / 2 floats per segment instead of 2 / 1): 5.03 / 5.87 ms per 2 GiB
@@ -248,7 +251,7 @@ const std::array<FourStep, 23> kShapes = {
     // alternatives for N = 2^16 explored by the tuning sweeps
     shape<16, 8, 8, 16, 64>(), shape<16, 8, 8, 64, 64>(), shape<16, 8, 8, 32, 32>(), shape<16, 8, 8, 16, 32>(),
     shape<16, 10, 6, 16, 64>(), shape<16, 7, 9, 64, 32>(), shape<16, 6, 10, 64, 16>(),
-    shape<16, 9, 7, 32, 32>(), shape<16, 10, 6, 16, 32>(),
+    shape<16, 9, 7, 32, 32>(), shape<16, 10, 6, 16, 32>(), shape<16, 9, 7, 32, 64>(), shape<15, 8, 7, 32, 64>(),
     // alternatives for the largest sizes explored by the sweeps
     shape<23, 13, 10, 2, 16>(), shape<24, 14, 10, 1, 16>(), shape<24, 13, 11, 2, 8>(),
 };
@@ -309,12 +312,13 @@ const FourStepT<double>* find_shape64(int lg, int n2, int g, int c) {
 // index [precision - 1], rows {forward, inverse}, cols {inverse, forward}
 struct FourStepMfma {
     int                                     lg;
+    int                                     c; // columns per tile of its column kernels
     std::array<std::array<RowLaunch, 2>, 2> rows;
     std::array<std::array<ColLaunch, 2>, 2> cols;
 };
 template<int LG, int N1, int N2, int G, int C>
 FourStepMfma shape_mfma() {
-    return {LG,
+    return {LG, C,
             {{{&launch_row<N1, G, LD_CONTIG, ST_STRIDED, 0, true, 0, PREC_BF16X3>,
                &launch_row<N1, G, LD_STRIDED, ST_CONTIG, tw_cap<LG>(), true, 0, PREC_BF16X3>},
               {&launch_row<N1, G, LD_CONTIG, ST_STRIDED, 0, true, 0, PREC_FP16X2>,
@@ -740,7 +744,7 @@ int run_batch(pcx_plan_t p, int inverse, int normalized, T* dst, int64_t dst_pac
         shape = p->shape64;
     else
         shape = p->shape;
-    const int     C    = shape->c;
+    int           C    = shape->c;
     ColLaunchT<T> colf = shape->cols[inverse ? 0 : 1];
     RowLaunchT<T> rowf = shape->rows[(inverse ? 3 : 0) + p->order];
     if constexpr (!F64)
@@ -750,6 +754,7 @@ int run_batch(pcx_plan_t p, int inverse, int normalized, T* dst, int64_t dst_pac
             return fail(PCX_ERR_UNSUPPORTED, "no split-precision kernels for this four-step shape");
         colf = fm->cols[p->precision - 1][inverse ? 0 : 1];
         rowf = fm->rows[p->precision - 1][inverse ? 1 : 0];
+        C    = fm->c; // the split kernels keep their own column tile
     }
     const int64_t n1 = (int64_t)1 << p->n1_lg, n2 = (int64_t)1 << p->n2_lg;
     for (int64_t b0 = 0; b0 < batch; b0 += chunk) {
