@@ -277,9 +277,12 @@ FourStepT<double> shape64() {
              &launch_row<N1, G, LD_PERM, ST_CONTIG, TW, true, ORD_BITREV, PREC_FP32, D>, nullptr},
             {&launch_col<N2, C, 0, PREC_FP32, D>, &launch_col<N2, C, TW, PREC_FP32, D>}};
 }
-const std::array<FourStepT<double>, 23> kShapes64 = {
+const std::array<FourStepT<double>, 26> kShapes64 = {
     // defaults (first match per lg); forward ms for a 2 GiB batch measured on MI355X
-    shape64<14, 7, 7, 32, 32>(),  shape64<15, 8, 7, 16, 32>(), shape64<16, 8, 8, 16, 16>(),
+    // 2^14 .. 2^16: one workgroup per CU with 512-byte column segments (2^14, 2^15: 64 columns)
+    // and 256-byte segments in both passes at 2^16 (32 columns, 32 rows): 1.63 / 1.58 / 1.62 ms
+    // against 1.68 / 1.68 / 1.73 ms with the two-workgroups-per-CU tiles listed below
+    shape64<14, 7, 7, 32, 64>(),  shape64<15, 8, 7, 32, 64>(), shape64<16, 8, 8, 32, 32>(),
     shape64<17, 8, 9, 16, 8>(),   // 1.92 ms
     shape64<18, 10, 8, 8, 16>(),  // 2.04
     shape64<19, 10, 9, 8, 8>(),   // 2.52
@@ -300,7 +303,8 @@ const std::array<FourStepT<double>, 23> kShapes64 = {
     shape64<20, 11, 9, 4, 8>(),   // 2.63
     shape64<21, 12, 9, 2, 8>(),   // 3.59
     shape64<22, 13, 9, 1, 8>(),   // 4.49
-    shape64<23, 13, 10, 1, 4>()}; // 5.91
+    shape64<23, 13, 10, 1, 4>(),  // 5.91
+    shape64<14, 7, 7, 32, 32>(), shape64<15, 8, 7, 16, 32>(), shape64<16, 8, 8, 16, 16>()};
 const FourStepT<double>* find_shape64(int lg, int n2, int g, int c) {
     for (const auto& f : kShapes64)
         if (f.lg == lg && (n2 == 0 || f.n2 == n2) && (g == 0 || f.g == g) && (c == 0 || f.c == c))
