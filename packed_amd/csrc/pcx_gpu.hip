@@ -255,9 +255,8 @@ const std::array<FourStep, 25> kShapes = {
     // alternatives for the largest sizes explored by the sweeps
     shape<23, 13, 10, 2, 16>(), shape<24, 14, 10, 1, 16>(), shape<24, 13, 11, 2, 8>(),
 };
-// FP64 shapes, 2^14 .. 2^23, 256-512 threads per workgroup.  C and G (doubles per column segment
-// and per row store segment) are 16 up to 2^16 with about 64-80 KiB of LDS per workgroup (two
-// workgroups per CU).  From 2^17 the defaults are the measured best of the tuning sweeps on
+// FP64 shapes, 2^14 .. 2^23, 256-512 threads per workgroup.  C and G are the doubles per column
+// segment and per row store segment.  The defaults are the measured best of the tuning sweeps on
 // MI355X (2 GiB batches, forward): wider segments win over occupancy, e.g. 2^20 with C = G = 8
 // and 129-133 KiB per workgroup runs in 2.43 ms against 3.23 ms for C = 8, G = 2.  Pass-twiddle
 // tables above 512 entries are read from global memory.
