@@ -237,12 +237,12 @@ FourStep shape() {
     static_assert(N1 + N2 == LG, "shape");
     return {LG, N1, N2, G, C, pass_set<N1, G, LG>(), col_set<N2, C, LG>()};
 }
-const std::array<FourStep, 25> kShapes = {
+const std::array<FourStep, 26> kShapes = {
     // defaults (first match per lg): measured best on MI355X for 2^16 is N1 = 512, N2 = 128 with
     // 128-column tiles (512-byte load and store segments in the column pass; 128 KiB of LDS, one
     // workgroup of 1024 threads per CU): 0.778 ms per 2 GiB against 0.855 ms with 64 columns;
-    // 2^15 (four-step forward) 1.61 against 1.70 ms per 2 GiB
-    shape<15, 8, 7, 32, 128>(), shape<16, 9, 7, 32, 128>(), shape<16, 8, 8, 32, 64>(), shape<17, 9, 8, 32, 64>(), shape<18, 9, 9, 32, 32>(),
+    // 2^15 (four-step forward, 64 rows per row workgroup) 1.59 against 1.70 ms per 2 GiB
+    shape<15, 8, 7, 64, 128>(), shape<16, 9, 7, 32, 128>(), shape<16, 8, 8, 32, 64>(), shape<17, 9, 8, 32, 64>(), shape<18, 9, 9, 32, 32>(),
     shape<19, 10, 9, 16, 32>(), shape<20, 10, 10, 16, 16>(), shape<21, 11, 10, 8, 16>(), shape<22, 12, 10, 4, 16>(),
     // 2^23, 2^24: more rows per workgroup than the LDS of two workgroups per CU allows (they
     // spill, but store 4 / 2 floats per segment instead of 2 / 1): 5.03 / 5.87 ms per 2 GiB
@@ -251,7 +251,7 @@ const std::array<FourStep, 25> kShapes = {
     // alternatives for N = 2^16 explored by the tuning sweeps
     shape<16, 8, 8, 16, 64>(), shape<16, 8, 8, 64, 64>(), shape<16, 8, 8, 32, 32>(), shape<16, 8, 8, 16, 32>(),
     shape<16, 10, 6, 16, 64>(), shape<16, 7, 9, 64, 32>(), shape<16, 6, 10, 64, 16>(),
-    shape<16, 9, 7, 32, 32>(), shape<16, 10, 6, 16, 32>(), shape<16, 9, 7, 32, 64>(), shape<15, 8, 7, 32, 64>(),
+    shape<16, 9, 7, 32, 32>(), shape<16, 10, 6, 16, 32>(), shape<16, 9, 7, 32, 64>(), shape<15, 8, 7, 32, 64>(), shape<15, 8, 7, 32, 128>(),
     // alternatives for the largest sizes explored by the sweeps
     shape<23, 13, 10, 2, 16>(), shape<24, 14, 10, 1, 16>(), shape<24, 13, 11, 2, 8>(),
 };
@@ -276,15 +276,15 @@ FourStepT<double> shape64() {
              &launch_row<N1, G, LD_PERM, ST_CONTIG, TW, true, ORD_BITREV, PREC_FP32, D>, nullptr},
             {&launch_col<N2, C, 0, PREC_FP32, D>, &launch_col<N2, C, TW, PREC_FP32, D>}};
 }
-const std::array<FourStepT<double>, 26> kShapes64 = {
+const std::array<FourStepT<double>, 29> kShapes64 = {
     // defaults (first match per lg); forward ms for a 2 GiB batch measured on MI355X
     // 2^14 .. 2^16: one workgroup per CU with 512-byte column segments (2^14, 2^15: 64 columns)
     // and 256-byte segments in both passes at 2^16 (32 columns, 32 rows): 1.63 / 1.58 / 1.62 ms
     // against 1.68 / 1.68 / 1.73 ms with the two-workgroups-per-CU tiles listed below
     shape64<14, 7, 7, 32, 64>(),  shape64<15, 8, 7, 32, 64>(), shape64<16, 8, 8, 32, 32>(),
-    shape64<17, 8, 9, 16, 8>(),   // 1.92 ms
-    shape64<18, 10, 8, 8, 16>(),  // 2.04
-    shape64<19, 10, 9, 8, 8>(),   // 2.52
+    shape64<17, 8, 9, 32, 16>(),  // 1.84 ms
+    shape64<18, 10, 8, 8, 32>(),  // 1.89
+    shape64<19, 10, 9, 8, 16>(),  // 2.41
     shape64<20, 10, 10, 8, 8>(),  // 2.43
     shape64<21, 11, 10, 4, 8>(),  // 3.11
     shape64<22, 12, 10, 4, 8>(),  // 3.12
@@ -303,7 +303,10 @@ const std::array<FourStepT<double>, 26> kShapes64 = {
     shape64<21, 12, 9, 2, 8>(),   // 3.59
     shape64<22, 13, 9, 1, 8>(),   // 4.49
     shape64<23, 13, 10, 1, 4>(),  // 5.91
-    shape64<14, 7, 7, 32, 32>(), shape64<15, 8, 7, 16, 32>(), shape64<16, 8, 8, 16, 16>()};
+    shape64<14, 7, 7, 32, 32>(), shape64<15, 8, 7, 16, 32>(), shape64<16, 8, 8, 16, 16>(),
+    shape64<17, 8, 9, 16, 8>(),   // 1.88
+    shape64<18, 10, 8, 8, 16>(),  // 2.02
+    shape64<19, 10, 9, 8, 8>()};  // 2.44
 const FourStepT<double>* find_shape64(int lg, int n2, int g, int c) {
     for (const auto& f : kShapes64)
         if (f.lg == lg && (n2 == 0 || f.n2 == n2) && (g == 0 || f.g == g) && (c == 0 || f.c == c))
