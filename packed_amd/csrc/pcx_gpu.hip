@@ -237,13 +237,15 @@ FourStep shape() {
     static_assert(N1 + N2 == LG, "shape");
     return {LG, N1, N2, G, C, pass_set<N1, G, LG>(), col_set<N2, C, LG>()};
 }
-const std::array<FourStep, 26> kShapes = {
+const std::array<FourStep, 27> kShapes = {
     // defaults (first match per lg): measured best on MI355X for 2^16 is N1 = 512, N2 = 128 with
     // 128-column tiles (512-byte load and store segments in the column pass; 128 KiB of LDS, one
     // workgroup of 1024 threads per CU): 0.778 ms per 2 GiB against 0.855 ms with 64 columns;
     // 2^15 (four-step forward, 64 rows per row workgroup) 1.59 against 1.70 ms per 2 GiB
     shape<15, 8, 7, 64, 128>(), shape<16, 9, 7, 32, 128>(), shape<16, 8, 8, 32, 64>(), shape<17, 9, 8, 32, 64>(), shape<18, 9, 9, 32, 32>(),
-    shape<19, 10, 9, 16, 32>(), shape<20, 10, 10, 16, 16>(), shape<21, 11, 10, 8, 16>(), shape<22, 12, 10, 4, 16>(),
+    // 2^19: 512-point rows, 32 per workgroup (128-byte store segments), 1.91 against 2.03 ms per
+    // 2 GiB with 1024-point rows (the alternative below)
+    shape<19, 9, 10, 32, 16>(), shape<20, 10, 10, 16, 16>(), shape<21, 11, 10, 8, 16>(), shape<22, 12, 10, 4, 16>(),
     // 2^23, 2^24: more rows per workgroup than the LDS of two workgroups per CU allows (they
     // spill, but store 4 / 2 floats per segment instead of 2 / 1): 5.03 / 5.87 ms per 2 GiB
     // forward against 7.47 / 8.94
@@ -252,6 +254,7 @@ const std::array<FourStep, 26> kShapes = {
     shape<16, 8, 8, 16, 64>(), shape<16, 8, 8, 64, 64>(), shape<16, 8, 8, 32, 32>(), shape<16, 8, 8, 16, 32>(),
     shape<16, 10, 6, 16, 64>(), shape<16, 7, 9, 64, 32>(), shape<16, 6, 10, 64, 16>(),
     shape<16, 9, 7, 32, 32>(), shape<16, 10, 6, 16, 32>(), shape<16, 9, 7, 32, 64>(), shape<15, 8, 7, 32, 64>(), shape<15, 8, 7, 32, 128>(),
+    shape<19, 10, 9, 16, 32>(),
     // alternatives for the largest sizes explored by the sweeps
     shape<23, 13, 10, 2, 16>(), shape<24, 14, 10, 1, 16>(), shape<24, 13, 11, 2, 8>(),
 };
@@ -318,13 +321,15 @@ const FourStepT<double>* find_shape64(int lg, int n2, int g, int c) {
 // index [precision - 1], rows {forward, inverse}, cols {inverse, forward}
 struct FourStepMfma {
     int                                     lg;
-    int                                     c; // columns per tile of its column kernels
+    int                                     n1, n2; // the split its kernels are built for
+    int                                     c;      // columns per tile of its column kernels
     std::array<std::array<RowLaunch, 2>, 2> rows;
     std::array<std::array<ColLaunch, 2>, 2> cols;
 };
 template<int LG, int N1, int N2, int G, int C>
 FourStepMfma shape_mfma() {
-    return {LG, C,
+    static_assert(N1 + N2 == LG, "shape");
+    return {LG, N1, N2, C,
             {{{&launch_row<N1, G, LD_CONTIG, ST_STRIDED, 0, true, 0, PREC_BF16X3>,
                &launch_row<N1, G, LD_STRIDED, ST_CONTIG, tw_cap<LG>(), true, 0, PREC_BF16X3>},
               {&launch_row<N1, G, LD_CONTIG, ST_STRIDED, 0, true, 0, PREC_FP16X2>,
@@ -334,7 +339,7 @@ FourStepMfma shape_mfma() {
 }
 const std::array<FourStepMfma, 6> kShapesMfma = {shape_mfma<15, 8, 7, 32, 64>(),  shape_mfma<16, 9, 7, 32, 64>(),
                                                  shape_mfma<17, 9, 8, 32, 64>(),  shape_mfma<18, 9, 9, 32, 32>(),
-                                                 shape_mfma<19, 10, 9, 16, 32>(), shape_mfma<20, 10, 10, 16, 16>()};
+                                                 shape_mfma<19, 9, 10, 32, 16>(), shape_mfma<20, 10, 10, 16, 16>()};
 // One-pass decimation-in-frequency kernel (k_dif) for N = 2^15 = M * R: natural order with
 // M = 2^14, R = 2; bit_reversed / unordered with M = 2^13, R = 4 (two workgroups per CU; the
 // stores are contiguous there).  Measured on MI355X against the four-step for a 2 GiB batch:
@@ -354,9 +359,11 @@ const std::array<DifLaunch, 3> kDif = {&launch_dif<14, 1, ORD_NORMAL>, &launch_d
 constexpr int dif_core_log2(int order) { return order == 0 ? 14 : 13; }
 
 const FourStep* find_shape(int lg, int n2, int g, int c);
+// The split kernels for the plan's shape: same size and the same N1 x N2 split (the plan's
+// scratch layout and row / column counts follow its shape); only the default shapes have them.
 const FourStepMfma* find_mfma(int lg, const FourStep* sh) {
     for (const auto& f : kShapesMfma)
-        if (f.lg == lg && sh == find_shape(lg, 0, 0, 0))
+        if (f.lg == lg && sh == find_shape(lg, 0, 0, 0) && f.n1 == sh->n1 && f.n2 == sh->n2)
             return &f;
     return nullptr;
 }
