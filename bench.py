@@ -86,15 +86,37 @@ def time_steps(fn, steps, warmup):
     return t0.elapsed_time(t1) / steps
 
 
-def cpu_baseline(n: int, seconds: float):
-    """Times the oracle's FP32 port on host cores: bounded sample of the same workload."""
+def cpu_share() -> int:
+    """Host cores this process may use: its CPU affinity, bounded by OMP_NUM_THREADS when set (a
+    GPU job's host share)."""
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    if omp.isdigit() and int(omp) > 0:
+        cores = min(cores, int(omp))
+    return max(1, cores)
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _port_rate(n: int, seconds: float, threads: int):
+    """GFLOP/s of the oracle's FP32 port over a batch sized to about `seconds` of work, best of 3
+    after a warm-up, and the batch size."""
     import numpy as np
 
     from oracle import oracle as O
 
-    threads = max(1, min(os.cpu_count() or 1, 16))
     plan = O.PortPlan(n)
-    # calibrate on a small batch, then size the sample to ~`seconds` of CPU work
     cal = threads * 2
     re = np.empty(cal * n, np.float32)
     im = np.empty(cal * n, np.float32)
@@ -102,9 +124,8 @@ def cpu_baseline(n: int, seconds: float):
     plan.batch(re, im, n, cal, threads=threads)  # warm-up
     t = time.perf_counter()
     plan.batch(re, im, n, cal, threads=threads)
-    dt = time.perf_counter() - t
-    per = dt / cal
-    count = int(max(cal, min(4096, seconds / 3 / max(per, 1e-9))))
+    per = (time.perf_counter() - t) / cal
+    count = int(max(cal, min(4096, seconds / 4 / max(per, 1e-9))))
     count = (count // threads) * threads or threads
     re = np.empty(count * n, np.float32)
     im = np.empty(count * n, np.float32)
@@ -115,14 +136,34 @@ def cpu_baseline(n: int, seconds: float):
         t = time.perf_counter()
         plan.batch(re, im, n, count, threads=threads)
         best = min(best, time.perf_counter() - t)
-    flops = 5.0 * n * math.log2(n) * count
+    return 5.0 * n * math.log2(n) * count / best / 1e9, count
+
+
+CALIBRATION_NOTE = ("kind=port: the oracle's FP32 split-complex radix-4 restatement (C, OpenMP over the batch), "
+                    "not the reference's AVX2 code.  The reference cannot be compiled in this image (fft.hpp:15 "
+                    "includes <bits/utility.h>, which libstdc++ 11 does not ship), so the port is not calibrated "
+                    "against it.")
+
+
+def cpu_baseline(n: int, seconds: float):
+    """Times the oracle's FP32 port on host cores: bounded samples of the same workload on all of
+    this process's cores and on one."""
+    threads = cpu_share()
+    value, count = _port_rate(n, seconds, threads)
+    value1, count1 = _port_rate(n, seconds / 2, 1)
     return {
-        "value": round(flops / best / 1e9, 2),
+        "value": round(value, 2),
         "unit": "GFLOP/s",
         "cores": threads,
         "kind": "port",
         "sample": f"{count} forward FP32 C2C FFTs of N=2^{int(math.log2(n))}, split re/im, normal order, "
-                  f"best of 3, {threads} OpenMP threads",
+                  f"best of 3, {threads} OpenMP threads; 1 thread: {count1} FFTs",
+        "cores_1": 1,
+        "value_1": round(value1, 2),
+        "nproc": os.cpu_count(),
+        "cpu_model": cpu_model(),
+        "note": CALIBRATION_NOTE + f" cores = this process's CPU share (affinity {cpu_share()} of nproc "
+                                   f"{os.cpu_count()}, bounded by OMP_NUM_THREADS).",
     }
 
 
@@ -179,6 +220,27 @@ def main():
         return
     ms_inv = time_steps(step_inv, max(3, args.steps // 2), 1)
     ms_br = time_steps(step_br, max(3, args.steps // 2), 1)  # FP32 (the matrix-core form is natural order only)
+    from oracle import oracle as O
+
+    zin = z[: 2 * n].cpu().numpy()  # written by step_br (bit-reversed forward of x)
+    xin = x[: 2 * n].cpu().numpy()
+    want_br = np.fft.fft(xin[:n].astype(np.float64) + 1j * xin[n:])[O.slot_freq("bit_reversed", n)]
+    err_br = O.rel_l2(zin[:n] + 1j * zin[n:], want_br)
+    # the split-precision matrix-core forms on the same batch (the metric's "MFMA-split vs FP32")
+    split = {}
+    if not f64 and args.precision == "fp32":
+        for prec in ("bf16x3", "fp16x2"):
+            u = pa.fft_unit(n, precision=prec)
+            ms_p = time_steps(lambda: u.batch(z.data_ptr(), x.data_ptr(), batch, n, 2 * n, n, 2 * n),
+                              max(3, args.steps // 2), 1)
+            errs_p = []
+            for b in (0, batch - 1):
+                xin = x[b * 2 * n:(b + 1) * 2 * n].cpu().numpy()
+                want = np.fft.fft(xin[:n].astype(np.float64) + 1j * xin[n:].astype(np.float64))
+                zo = z[b * 2 * n:(b + 1) * 2 * n].cpu().numpy()
+                errs_p.append(O.rel_l2(zo[:n] + 1j * zo[n:], want))
+            split[prec] = {"ms": round(ms_p, 4), "gflops": round(5.0 * n * math.log2(n) * batch / (ms_p * 1e-3) / 1e9, 1),
+                           "speedup_vs_fp32": round(ms / ms_p, 3), "rel_l2_vs_fp64": float(f"{max(errs_p):.3e}")}
 
     flops = 5.0 * n * math.log2(n) * batch
     gflops = flops / (ms * 1e-3) / 1e9
@@ -186,8 +248,6 @@ def main():
     achieved = algo_bytes / (ms * 1e-3) / 1e9
 
     # parity spot check on the timed outputs: transform 0 and the last one vs the FP64 oracle
-    from oracle import oracle as O
-
     errs = []
     for b in (0, batch - 1):
         xin = x[b * 2 * n:(b + 1) * 2 * n].cpu().numpy()
@@ -195,14 +255,10 @@ def main():
         want = np.fft.fft(xc)
         yo = y[b * 2 * n:(b + 1) * 2 * n].cpu().numpy()
         errs.append(O.rel_l2(yo[:n] + 1j * yo[n:], want))
-    zin = z[: 2 * n].cpu().numpy()  # last written by step_br (bit-reversed forward of x)
-    xin = x[: 2 * n].cpu().numpy()
-    want_br = np.fft.fft(xin[:n].astype(np.float64) + 1j * xin[n:])[O.slot_freq("bit_reversed", n)]
-    err_br = O.rel_l2(zin[:n] + 1j * zin[n:], want_br)
-
     traffic = measured_traffic(args.log2n, batch, args.dtype) if args.precision == "fp32" else None
     out = {
-        "metric": f"C2C FFT GFLOP/s (5N*log2N), N=2^16 batch 4096, {'FP64' if f64 else 'FP32'} split re/im",
+        "metric": ("C2C FFT GFLOP/s (5N\u00b7log2N) & %HBM peak, N=2^16 batch; MFMA-split vs FP32" if not f64
+                   else "C2C FFT GFLOP/s (5N*log2N), N=2^16 batch 4096, FP64 split re/im"),
         "value": round(gflops, 1),
         "unit": "GFLOP/s",
         "n_gpus": 1,
@@ -226,6 +282,9 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
+            "traffic_source": (f"profiles/fft_fourstep_{'f64_' if f64 else ''}n{n}_b{batch}_hbm_traffic.md: "
+                               "rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE of this workload, scaled by a plain copy"
+                               if traffic else None),
             "kernel": "four-step column pass k_col + row pass k_row (one step = one launch of each)",
         },
         "inverse": {"ms": round(ms_inv, 4), "gflops": round(flops / (ms_inv * 1e-3) / 1e9, 1)},
@@ -233,6 +292,8 @@ def main():
                          "rel_l2_vs_fp64": float(f"{err_br:.3e}")},
         "rel_l2_vs_fp64": float(f"{max(errs):.3e}"),
     }
+    if split:
+        out["split"] = split
     if not args.no_cpu and not f64:  # the host-core port is FP32
         out["cpu_baseline"] = cpu_baseline(n, args.cpu_seconds)
     print(json.dumps(out))
@@ -380,7 +441,7 @@ def run_config3(args):
     }
     if not args.no_cpu:
         # host-core baseline: the oracle's FP32 port on a 2^20-sample slice of the same signal
-        threads = max(1, min(os.cpu_count() or 1, 16))
+        threads = cpu_share()
         fs = f[: 1 << 20]
         O.port_fxcorr(fs, g, N, threads)
         t = time.perf_counter()
@@ -388,8 +449,9 @@ def run_config3(args):
         dt = time.perf_counter() - t
         cpu_flops = nb * (2 * 5 * N * math.log2(N) + 6 * N)
         res["cpu_baseline"] = {"value": round(cpu_flops / dt / 1e9, 2), "unit": "GFLOP/s", "cores": threads,
-                               "kind": "port",
-                               "sample": f"2^20-sample slice ({nb} blocks), same kernel and block size, {threads} threads"}
+                               "kind": "port", "cpu_model": cpu_model(), "nproc": os.cpu_count(),
+                               "sample": f"2^20-sample slice ({nb} blocks), same kernel and block size, {threads} threads",
+                               "note": CALIBRATION_NOTE}
     return res
 
 

@@ -69,6 +69,25 @@ int     pcx_plan_order(pcx_plan_t plan);
 int     pcx_plan_set_precision(pcx_plan_t plan, int precision);
 int     pcx_plan_precision(pcx_plan_t plan);
 
+/* Launch-shape tuning of four-step plans (fft_size > 16384; FP64 > 8192).  The reference's
+ * sub_size / NodeSizeRec / Strategy template parameters (fft.hpp:565-598) pick its cache
+ * blocking; these pick the GPU's.  None changes results.
+ * pcx_plan_set_shape: one of the instantiated N1 x N2 splits and tiles: column length
+ *   2^n2_log2, `cols` columns per column tile, `rows` rows per row workgroup (0 = any; the first
+ *   listed match wins, the default first).  Rebuilds the split's twiddle tables after a
+ *   device-wide synchronisation (every enqueued use of the old tables ends first).  Split-
+ *   precision plans keep the default shape.
+ * pcx_plan_shape: the current split and tiles (0s for a single-pass plan).
+ * pcx_plan_set_grid: workgroup caps of the column and row passes (0 = one per tile; the kernels
+ *   loop over the rest).  pcx_plan_set_chunk: bytes of the intermediate of one chunk of a batch
+ *   (0 = the default 2 GiB).  pcx_plan_set_one_pass: 2^15 runs as one decimation-in-frequency
+ *   pass (default on) or as the four-step. */
+int pcx_plan_set_shape(pcx_plan_t plan, int n2_log2, int cols, int rows);
+int pcx_plan_shape(pcx_plan_t plan, int* n2_log2, int* cols, int* rows);
+int pcx_plan_set_grid(pcx_plan_t plan, int64_t grid_col, int64_t grid_row);
+int pcx_plan_set_chunk(pcx_plan_t plan, int64_t bytes);
+int pcx_plan_set_one_pass(pcx_plan_t plan, int enable);
+
 /* Forward FFT.  Replaces fft_raw<PD,PS>(dst, src[, src_size]) and operator()(dst, src)
  * (fft.hpp:624-687, 728-820).  In place when dst == src.  src_size <= fft_size values are read
  * and zero-extended.  The spectrum is stored in the plan's order; an unordered spectrum uses pack
@@ -84,7 +103,10 @@ int pcx_ifft(pcx_plan_t plan, int normalized, float* dst, int64_t dst_pack, cons
 
 /* Batched FFT over `batch` vectors: vector b at src + b*src_dist / dst + b*dst_dist floats.
  * Batched form of the two calls above (the reference runs one fft_unit call per vector; its
- * fft_unit_par, fft.hpp:2168-2572, is not followed: its output order is wrong). */
+ * fft_unit_par, fft.hpp:2168-2572, is not followed: its output order is wrong).
+ * Four-step plans keep one chunk's intermediate in the plan (allocated on the first call that
+ * needs it).  Calls on another stream than the plan's previous four-step call wait for that
+ * call first, so copies of one plan may run on several streams. */
 int pcx_fft_batch(pcx_plan_t plan, int inverse, int normalized, float* dst, int64_t dst_pack,
                   int64_t dst_dist, const float* src, int64_t src_pack, int64_t src_dist,
                   int64_t src_size, int64_t batch, void* stream);
@@ -112,15 +134,24 @@ int pcx_fft_batch_f64(pcx_plan_t plan, int inverse, int normalized, double* dst,
 /* FFT across a range of vectors: the transform of fft_unit_par (fft.hpp:2168-2572), whose
  * output order is wrong in the reference (its test is disabled, test_fft.cpp:789); here it is
  * natural.  For every k < length: dst_j[k] = sum_i src_i[k] * W^(i*j) (inverse: conjugate
- * twiddles, scaled by 1/fft_size if normalized), i, j < fft_size <= 1024.  src_vecs / dst_vecs
+ * twiddles, scaled by 1/fft_size if normalized), i, j < fft_size <= 8192.  src_vecs / dst_vecs
  * are device arrays of fft_size pointers to packed vectors of `length` elements; dst may equal
- * src element for element (in place).  Normal-order plans only. */
+ * src element for element (in place) when the packs are equal; with different packs no
+ * destination vector may overlap a source vector (the caller checks: the tables are device
+ * memory).  Normal-order plans only. */
 int pcx_fft_across(pcx_plan_t plan, int inverse, int normalized, float* const* dst_vecs, int64_t dst_pack,
                    const float* const* src_vecs, int64_t src_pack, int64_t length, void* stream);
+/* The same for a double plan (fft_unit_par<double>, fft.hpp:2168-2172 with T = double):
+ * fft_size <= 2048. */
+int pcx_fft_across_f64(pcx_plan_t plan, int inverse, int normalized, double* const* dst_vecs, int64_t dst_pack,
+                       const double* const* src_vecs, int64_t src_pack, int64_t length, void* stream);
 
 /* ------------------------------------------------------------------ cross-correlation
  * Replaces fxcorr_unit<float>(g, fft_size) (fxcorr.hpp:62-69): spectrum of g (device, pack
- * g_pack, g_size taps) zero-extended to fft_size, scaled by 1/fft_size. */
+ * g_pack, g_size taps) zero-extended to fft_size, scaled by 1/fft_size.  fft_size is any power
+ * of two from 2 to 2^24: up to 16384 every block runs in one fused kernel (forward FFT,
+ * conj-multiply, inverse FFT on chip); above, the blocks go through HBM and the batched
+ * four-step FFT. */
 int pcx_fxcorr_create(const float* g, int64_t g_pack, int64_t g_size, int64_t fft_size, void* stream,
                       pcx_fxcorr_t* xc);
 int pcx_fxcorr_destroy(pcx_fxcorr_t xc);
@@ -136,8 +167,19 @@ int64_t pcx_fxcorr_step(pcx_fxcorr_t xc, int64_t src_pack);
 int pcx_fxcorr_exec_batch(pcx_fxcorr_t xc, float* dst, int64_t dst_pack, int64_t dst_dist, const float* src,
                           int64_t src_pack, int64_t src_dist, int64_t len, int64_t channels, void* stream);
 /* PCX_PREC_FP32 (default), PCX_PREC_BF16X3 or PCX_PREC_FP16X2 (matrix-core radix-16 butterflies
- * in both transforms; no reference counterpart). */
+ * in both transforms, fft_size 16 .. 16384; no reference counterpart). */
 int pcx_fxcorr_set_precision(pcx_fxcorr_t xc, int precision);
+/* The reference's TmpFactory_ slot (fxcorr.hpp:10-32, 47, 82-91: scratch supplied by the caller,
+ * one fft_size vector).  pcx_fxcorr_exec_batch_ws is pcx_fxcorr_exec_batch with the unit's
+ * scratch taken from `workspace` (device memory of workspace_bytes >= pcx_fxcorr_workspace_min,
+ * which is 8 * fft_size bytes): in place, the overlap of up to workspace_bytes / (8 * (fft_size -
+ * step)) blocks is saved there at a time; above fft_size 16384 it holds the blocks in flight
+ * (the four-step FFT keeps its intermediate in its plan).  The unit allocates no buffer of its
+ * own.  Calls that share one workspace must not overlap. */
+int64_t pcx_fxcorr_workspace_min(pcx_fxcorr_t xc);
+int     pcx_fxcorr_exec_batch_ws(pcx_fxcorr_t xc, float* dst, int64_t dst_pack, int64_t dst_dist, const float* src,
+                                 int64_t src_pack, int64_t src_dist, int64_t len, int64_t channels, void* workspace,
+                                 int64_t workspace_bytes, void* stream);
 
 /* ------------------------------------------------------------------ uneven batches
  * A work queue of transforms of uneven power-of-two sizes and mixed directions, natural order,
@@ -159,6 +201,9 @@ typedef struct pcx_queue_s* pcx_queue_t;
 int pcx_queue_create(const pcx_job* jobs, int64_t njobs, int normalized, pcx_queue_t* queue);
 int pcx_queue_destroy(pcx_queue_t queue);
 int pcx_queue_exec(pcx_queue_t queue, float* dst, const float* src, void* stream);
+/* Times the launch shapes on (dst, src) and keeps the fastest.  dst is overwritten many times
+ * and must not alias src.  A graph captured before is dropped (pcx_queue_launch then fails until
+ * pcx_queue_capture runs again); the call synchronises the device. */
 int pcx_queue_autotune(pcx_queue_t queue, float* dst, const float* src, void* stream, const char* cache_path);
 int pcx_queue_capture(pcx_queue_t queue, float* dst, const float* src, void* stream);
 int pcx_queue_launch(pcx_queue_t queue, void* stream);

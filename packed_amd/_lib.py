@@ -76,6 +76,11 @@ SIGNATURES = {
     "pcx_plan_order": (_int, [_vp]),
     "pcx_plan_set_precision": (_int, [_vp, _int]),
     "pcx_plan_precision": (_int, [_vp]),
+    "pcx_plan_set_shape": (_int, [_vp, _int, _int, _int]),
+    "pcx_plan_shape": (_int, [_vp, C.POINTER(_int), C.POINTER(_int), C.POINTER(_int)]),
+    "pcx_plan_set_grid": (_int, [_vp, _i64, _i64]),
+    "pcx_plan_set_chunk": (_int, [_vp, _i64]),
+    "pcx_plan_set_one_pass": (_int, [_vp, _int]),
     "pcx_fft": (_int, [_vp, _vp, _i64, _vp, _i64, _i64, _vp]),
     "pcx_ifft": (_int, [_vp, _int, _vp, _i64, _vp, _i64, _vp]),
     "pcx_fft_batch": (_int, [_vp, _int, _int, _vp, _i64, _i64, _vp, _i64, _i64, _i64, _i64, _vp]),
@@ -90,6 +95,8 @@ SIGNATURES = {
     "pcx_fxcorr_step": (_i64, [_vp, _i64]),
     "pcx_fxcorr_exec_batch": (_int, [_vp, _vp, _i64, _i64, _vp, _i64, _i64, _i64, _i64, _vp]),
     "pcx_fxcorr_set_precision": (_int, [_vp, _int]),
+    "pcx_fxcorr_workspace_min": (_i64, [_vp]),
+    "pcx_fxcorr_exec_batch_ws": (_int, [_vp, _vp, _i64, _i64, _vp, _i64, _i64, _i64, _i64, _vp, _i64, _vp]),
     "pcx_queue_create": (_int, [C.POINTER(Job), _i64, _int, C.POINTER(_vp)]),
     "pcx_queue_destroy": (_int, [_vp]),
     "pcx_queue_exec": (_int, [_vp, _vp, _vp, _vp]),
@@ -112,6 +119,7 @@ SIGNATURES = {
                                   _vp]),
     "pcx_repack_f64": (_int, [_vp, _i64, _vp, _i64, _i64, _vp]),
     "pcx_fft_across": (_int, [_vp, _int, _int, _vp, _i64, _vp, _i64, _i64, _vp]),
+    "pcx_fft_across_f64": (_int, [_vp, _int, _int, _vp, _i64, _vp, _i64, _i64, _vp]),
 }
 
 _lib = None

@@ -917,67 +917,69 @@ void k_col(ColArgsT<T> p) {
 // natural order in j.  Vector i is an arbitrary packed vector (pointer table, pack 2^src_lp).  A
 // workgroup takes C consecutive elements of all A vectors: a wavefront reads C consecutive floats
 // of one vector at a time, and the A-point transforms run down the columns of an LDS tile.
-struct AcrossArgs {
-    const float* const* src; // A device pointers
-    float* const*       dst;
-    int64_t             len;
-    int                 src_lp, dst_lp;
-    int                 conj_in, conj_out;
-    float               scale;
-    const float2*       twA;
+template<class T = float>
+struct AcrossArgsT {
+    const T* const* src; // A device pointers
+    T* const*       dst;
+    int64_t         len;
+    int             src_lp, dst_lp;
+    int             conj_in, conj_out;
+    T               scale;
+    const vec2<T>*  twA;
 };
-template<int LA, int C>
+using AcrossArgs = AcrossArgsT<float>;
+template<int LA, int C, class T = float>
 constexpr int across_lds_floats() {
-    return 2 * (1 << LA) * C + TwTable<LA>::FLOATS;
+    return 2 * (1 << LA) * C + TwTable<LA, T>::FLOATS;
 }
-template<int LA, int C>
-__global__ __launch_bounds__(C * Plan<LA>::TA) void k_across(AcrossArgs p) {
-    using P           = Plan<LA>;
-    constexpr int PPT = P::PPT;
-    constexpr int R0  = P::first_radix;
-    constexpr int RL  = P::last_radix;
-    constexpr int NT  = C * P::TA;
+template<int LA, int C, class T = float>
+__global__ __launch_bounds__((C * Plan<LA, false, T>::TA)) void k_across(AcrossArgsT<T> p) {
+    using P            = Plan<LA, false, T>;
+    constexpr int PPT  = P::PPT;
+    constexpr int R0   = P::first_radix;
+    constexpr int RL   = P::last_radix;
+    constexpr int NT   = C * P::TA;
     constexpr int DATA = 2 * (1 << LA) * C;
     static_assert(!P::HALF, "across tiles keep separate re / im arrays");
-    __shared__ float lds[across_lds_floats<LA, C>()];
-    float*           lre = lds;
-    float*           lim = lds + DATA / 2;
-    float*           ltw = lds + DATA;
-    TwTable<LA>::fill(ltw, p.twA, threadIdx.x, NT);
+    __shared__ T lds[across_lds_floats<LA, C, T>()];
+    T*           lre = lds;
+    T*           lim = lds + DATA / 2;
+    T*           ltw = lds + DATA;
+    TwTable<LA, T>::fill(ltw, p.twA, threadIdx.x, NT);
     __syncthreads();
     const int64_t ntiles = (p.len + C - 1) / C;
     const int     c      = threadIdx.x % C;
     const int     jt     = threadIdx.x / C;
     auto          addr   = [&](int b, int o) { return (b + o) * C + c; };
-    const float   ci     = p.conj_in ? -1.f : 1.f;
-    const float   co     = p.conj_out ? -p.scale : p.scale;
+    const T       ci     = p.conj_in ? T(-1) : T(1);
+    const T       co     = p.conj_out ? -p.scale : p.scale;
     for (int64_t base = 0; base < ntiles; base += gridDim.x) {
         const int64_t tile = xcd_item(base);
         if (tile >= ntiles)
             break; // uniform over the workgroup
         const int64_t k  = tile * C + c;
         const bool    ok = k < p.len;
-        cf            v[PPT];
+        cx<T>         v[PPT];
 #pragma unroll
         for (int m = 0; m < PPT / R0; ++m)
 #pragma unroll
             for (int r = 0; r < R0; ++r) {
-                const float* vec = p.src[in_elem<LA>(jt, m, r)];
-                const int64_t o  = pidx(ok ? k : 0, p.src_lp);
-                cf x             = {vec[o], vec[o + ((int64_t)1 << p.src_lp)]};
-                v[m * R0 + r]    = ok ? cf{x.x, ci * x.y} : cf{0.f, 0.f};
+                const T*      vec = p.src[in_elem<LA, false, T>(jt, m, r)];
+                const int64_t o   = pidx(ok ? k : 0, p.src_lp);
+                cx<T>         x   = {vec[o], vec[o + ((int64_t)1 << p.src_lp)]};
+                v[m * R0 + r]     = ok ? cx<T>{x.x, ci * x.y} : cx<T>{T(0), T(0)};
             }
-        stockham<LA, false, decltype(addr)>(v, lre, lim, jt, addr, ltw);
+        stockham<LA, false, decltype(addr), PREC_FP32, T>(v, lre, lim, jt, addr, ltw);
         if (ok) {
             const int64_t o = pidx(k, p.dst_lp);
 #pragma unroll
             for (int m = 0; m < PPT / RL; ++m)
 #pragma unroll
                 for (int r = 0; r < RL; ++r) {
-                    float* vec                           = p.dst[out_elem<LA>(jt, m, r)];
-                    const cf y                           = v[m * RL + r];
-                    vec[o]                               = y.x * p.scale;
-                    vec[o + ((int64_t)1 << p.dst_lp)]    = y.y * co;
+                    T*          vec                    = p.dst[out_elem<LA, false, T>(jt, m, r)];
+                    const cx<T> y                      = v[m * RL + r];
+                    vec[o]                             = y.x * p.scale;
+                    vec[o + ((int64_t)1 << p.dst_lp)]  = y.y * co;
                 }
         }
         __syncthreads(); // LDS is reused by the next tile

@@ -41,6 +41,8 @@ struct XcArgs {
     int64_t      src_dist, dst_dist; // floats between channels
     int64_t      channels;           // blocks of channel c are nblocks * c .. nblocks * (c + 1) - 1
     int64_t      blk0;               // first block of this launch (a launch covers < 2^32 work-items)
+    int64_t      blk1;               // end of this launch's blocks
+    int64_t      hblk0;              // block whose saved overlap is first in hre / him
 };
 
 template<int LA, int G>
@@ -98,7 +100,7 @@ void k_fxcorr(XcArgs p) {
     const int     g    = TA >= 64 ? __builtin_amdgcn_readfirstlane(t / TA) : t / TA;
     const int     jt   = t % TA;
     const int64_t bg   = p.blk0 + xcd_item(0) * G + g; // block over all channels; neighbours share one L2
-    const bool    ok   = bg < p.nblocks * p.channels;
+    const bool    ok   = bg < p.blk1;
     const int64_t ch   = ok ? bg / p.nblocks : 0;
     const int64_t b    = ok ? bg - ch * p.nblocks : 0;
     const int64_t off  = b * p.step;
@@ -135,7 +137,7 @@ void k_fxcorr(XcArgs p) {
                 cf      x = {0.f, 0.f};
                 if (ok && i < p.len) {
                     if (p.hre && a >= p.step) {
-                        int64_t h = bg * p.hlen + (a - p.step);
+                        int64_t h = (bg - p.hblk0) * p.hlen + (a - p.step);
                         x         = {p.hre[h], p.him[h]};
                     } else if constexpr (B32) {
                         x = BufR<>(src, p.src_lp).ld(poff((uint32_t)i, p.src_lp), 0);
@@ -196,14 +198,73 @@ void k_fxcorr(XcArgs p) {
     }
 }
 
-// Saves, for every block b, the samples [b*step + step, b*step + N) that later blocks overwrite
-// when the correlation runs in place.
-__global__ void k_fxcorr_halo(const float* src, int src_lp, int64_t src_dist, int64_t len, int64_t step, int64_t hlen,
-                              int64_t nblocks, int64_t channels, float* hre, float* him) {
-    const int64_t total = nblocks * channels * hlen;
+// ---- block path for FFT sizes above 16384: the blocks go through HBM in a buffer of nb blocks
+// (block j: re[N] then im[N] at 2*N*j), transformed by the batched four-step FFT.
+struct XcBlockArgs {
+    const float* src;  // signal, pack src_lp
+    float*       dst;  // output, pack dst_lp
+    float*       buf;  // nb blocks of 2N floats
+    const float* kre;  // kernel spectrum FFT(g)/N, natural order, split
+    const float* kim;
+    int64_t      len, step, nblocks;
+    int64_t      src_dist, dst_dist; // floats between channels
+    int64_t      bg0, nb;            // global blocks bg0 .. bg0 + nb - 1 (channel-major)
+    int          src_lp, dst_lp, lg;
+    float        scale;              // 1/N (the kernel spectrum is FFT(g))
+};
+
+// buf[j] = the N samples from block bg0 + j's offset, zero past the signal.
+static __global__ __launch_bounds__(256) void k_xc_gather(XcBlockArgs a) {
+    const int64_t n     = (int64_t)1 << a.lg;
+    const int64_t total = a.nb << a.lg;
     for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
          idx += (int64_t)gridDim.x * blockDim.x) {
-        int64_t bg = idx / hlen, h = idx % hlen;
+        const int64_t j = idx >> a.lg, i = idx & (n - 1);
+        const int64_t bg = a.bg0 + j;
+        const int64_t ch = bg / a.nblocks, b = bg - ch * a.nblocks;
+        const int64_t s  = b * a.step + i;
+        const cf      x  = s < a.len ? ld_cx64(a.src + ch * a.src_dist, s, a.src_lp) : cf{0.f, 0.f};
+        a.buf[2 * n * j + i]     = x.x;
+        a.buf[2 * n * j + n + i] = x.y;
+    }
+}
+// buf[j] = buf[j] * conj(K) / N: the spectrum of the correlation block (fxcorr.hpp:67-68, 115).
+static __global__ __launch_bounds__(256) void k_xc_mul(XcBlockArgs a) {
+    const int64_t n     = (int64_t)1 << a.lg;
+    const int64_t total = a.nb << a.lg;
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t j = idx >> a.lg, i = idx & (n - 1);
+        float*        re = a.buf + 2 * n * j + i;
+        const cf      x  = {re[0], re[n]};
+        const cf      y  = cmulc(x, cf{a.kre[i], a.kim[i]});
+        re[0]            = y.x * a.scale;
+        re[n]            = y.y * a.scale;
+    }
+}
+// The first `step` outputs of block bg0 + j (fewer at the signal's end) to the destination.
+static __global__ __launch_bounds__(256) void k_xc_scatter(XcBlockArgs a) {
+    const int64_t n     = (int64_t)1 << a.lg;
+    const int64_t total = a.nb * a.step;
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t j = idx / a.step, i = idx - j * a.step;
+        const int64_t bg = a.bg0 + j;
+        const int64_t ch = bg / a.nblocks, b = bg - ch * a.nblocks;
+        const int64_t s  = b * a.step + i;
+        if (s < a.len)
+            st_cx64(a.dst + ch * a.dst_dist, s, a.dst_lp, cf{a.buf[2 * n * j + i], a.buf[2 * n * j + n + i]});
+    }
+}
+
+// Saves, for the blocks bg0 .. bg0 + count - 1 (over all channels), the samples
+// [b*step + step, b*step + N) that later blocks overwrite when the correlation runs in place.
+static __global__ void k_fxcorr_halo(const float* src, int src_lp, int64_t src_dist, int64_t len, int64_t step, int64_t hlen,
+                              int64_t nblocks, int64_t bg0, int64_t count, float* hre, float* him) {
+    const int64_t total = count * hlen;
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (int64_t)gridDim.x * blockDim.x) {
+        int64_t bg = bg0 + idx / hlen, h = idx % hlen;
         int64_t ch = bg / nblocks, b = bg % nblocks;
         int64_t i  = b * step + step + h;
         cf      x  = i < len ? ld_cx64(src + ch * src_dist, i, src_lp) : cf{0.f, 0.f};

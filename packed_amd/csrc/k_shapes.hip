@@ -1,0 +1,22 @@
+// k_shapes.hip -- default FP32 four-step shapes, 2^15 .. 2^24 (first match per size).
+#include "shapes.hpp"
+
+namespace pcxd {
+namespace {
+constexpr FourStep kDefault[] = {
+    // measured best on MI355X for 2^16 is N1 = 512, N2 = 128 with 128-column tiles (512-byte load
+    // and store segments in the column pass; 128 KiB of LDS, one workgroup of 1024 threads per
+    // CU): 0.778 ms per 2 GiB against 0.855 ms with 64 columns; 2^15 (four-step forward, 64 rows
+    // per row workgroup) 1.59 against 1.70 ms per 2 GiB
+    shape<15, 8, 7, 64, 128>(), shape<16, 9, 7, 32, 128>(), shape<17, 9, 8, 32, 64>(), shape<18, 9, 9, 32, 32>(),
+    // 2^19: 512-point rows, 32 per workgroup (128-byte store segments), 1.91 against 2.03 ms per
+    // 2 GiB with 1024-point rows (an alternative)
+    shape<19, 9, 10, 32, 16>(), shape<20, 10, 10, 16, 16>(), shape<21, 11, 10, 8, 16>(), shape<22, 12, 10, 4, 16>(),
+    // 2^23, 2^24: more rows per workgroup than the LDS of two workgroups per CU allows (they
+    // spill, but store 4 / 2 floats per segment instead of 2 / 1): 5.03 / 5.87 ms per 2 GiB
+    // forward against 7.47 / 8.94
+    shape<23, 12, 11, 4, 8>(), shape<24, 14, 10, 2, 16>(),
+};
+} // namespace
+ShapeList<float> shapes_default() { return {kDefault, (int)(sizeof(kDefault) / sizeof(kDefault[0]))}; }
+} // namespace pcxd

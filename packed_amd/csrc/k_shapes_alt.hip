@@ -1,0 +1,19 @@
+// k_shapes_alt.hip -- alternative FP32 four-step shapes explored by the tuning sweeps and the
+// uneven-batch autotuner.
+#include "shapes.hpp"
+
+namespace pcxd {
+namespace {
+constexpr FourStep kAlt[] = {
+    // N = 2^16
+    shape<16, 8, 8, 32, 64>(), shape<16, 8, 8, 16, 64>(), shape<16, 8, 8, 64, 64>(), shape<16, 8, 8, 32, 32>(),
+    shape<16, 8, 8, 16, 32>(), shape<16, 10, 6, 16, 64>(), shape<16, 7, 9, 64, 32>(), shape<16, 6, 10, 64, 16>(),
+    shape<16, 9, 7, 32, 32>(), shape<16, 10, 6, 16, 32>(), shape<16, 9, 7, 32, 64>(),
+    // N = 2^15, 2^19
+    shape<15, 8, 7, 32, 64>(), shape<15, 8, 7, 32, 128>(), shape<19, 10, 9, 16, 32>(),
+    // the largest sizes
+    shape<23, 13, 10, 2, 16>(), shape<24, 14, 10, 1, 16>(), shape<24, 13, 11, 2, 8>(),
+};
+} // namespace
+ShapeList<float> shapes_alt() { return {kAlt, (int)(sizeof(kAlt) / sizeof(kAlt[0]))}; }
+} // namespace pcxd

@@ -62,7 +62,8 @@ __device__ __forceinline__ void split3_pk(float a, float b, uint32_t& t0, uint32
 typedef _Float16 halfx8 __attribute__((ext_vector_type(8)));
 
 // 2^e (and 2^-e) for the butterfly held in lanes l and l^32 (its column of the B operand):
-// m = the largest |x| over both lanes, e = 15 - ceil-exponent(m), clamped to [-100, 100].
+// m = the largest |x| over both lanes, e = 15 - ceil-exponent(m), clamped to [-126, 126] so
+// that 2^e and 2^-e are both normal FP32 numbers.
 __device__ __forceinline__ void col_scale(const float* r, const float* i, float& sc, float& inv) {
     float m = 0.f;
 #pragma unroll
@@ -72,7 +73,7 @@ __device__ __forceinline__ void col_scale(const float* r, const float* i, float&
     m       = fmaxf(__uint_as_float(w[0]), __uint_as_float(w[1]));
     int ex  = (int)((__float_as_uint(m) >> 23) & 0xFFu) - 126; // m = f * 2^ex, f in [0.5, 1)
     int e   = 15 - ex;
-    e       = e < -100 ? -100 : (e > 100 ? 100 : e);
+    e       = e < -126 ? -126 : (e > 126 ? 126 : e);
     sc      = __uint_as_float((uint32_t)(127 + e) << 23);
     inv     = __uint_as_float((uint32_t)(127 - e) << 23);
 }

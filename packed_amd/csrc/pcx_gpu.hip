@@ -1,13 +1,9 @@
 // pcx_gpu.hip -- C ABI of the gfx950 FFT / overlap-save cross-correlation path (include/pcx_gpu.h).
 //
-// Plans own device twiddle tables and, for N > 16384, a scratch buffer for the four-step
-// intermediate.  Everything is launched asynchronously on the caller's stream.
+// Plans own device twiddle tables and, for N > 16384, a buffer for the four-step intermediate.  Everything is launched asynchronously on the caller's stream.
 #include "pcx_gpu.h"
 
-#include "fft_kernels.hpp"
-#include "fxcorr_kernels.hpp"
-#include "vexpr_kernels.hpp"
-#include "queue_kernels.hpp"
+#include "dispatch.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -22,6 +18,7 @@
 #include <algorithm>
 
 using namespace pcxk;
+using namespace pcxd;
 
 namespace {
 
@@ -39,13 +36,7 @@ int fail(int code, std::string msg) {
             return fail(PCX_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_));              \
     } while (0)
 
-constexpr int kMaxSinglePass = 14; // N <= 16384 runs in one LDS-resident pass
-constexpr int kMaxLog2       = 24; // four-step limit: N2 <= 2048 columns, N1 <= 16384
-constexpr int kMaxSinglePass64 = 13; // FP64: N <= 8192 in one pass
-constexpr int kMaxLog2_64      = 23; // FP64 four-step limit
-
-bool    is_pow2(int64_t v) { return v > 0 && (v & (v - 1)) == 0; }
-int64_t pidx64(int64_t i, int lp) { return i + ((i >> lp) << lp); }
+bool is_pow2(int64_t v) { return v > 0 && (v & (v - 1)) == 0; }
 int  ilog2(int64_t v) {
     int l = 0;
     while (((int64_t)1 << l) < v)
@@ -76,301 +67,64 @@ int check_pack(int64_t p, const char* what) {
     return fail(PCX_ERR_INVALID_PACK, std::string(what) + " pack size (which is " + s(p) + ") is not a power of two");
 }
 
+// Plan-time work (twiddle tables, kernel spectra) runs on one non-blocking stream per device, and
+// set-up waits for that stream only: other streams of the process are never serialised by it.
+constexpr int kMaxDevices = 64;
+std::mutex    g_setup_mu;
+hipStream_t   g_setup[kMaxDevices] = {};
+
+hipError_t setup_stream(hipStream_t* out) {
+    int        dev = 0;
+    hipError_t e   = hipGetDevice(&dev);
+    if (e != hipSuccess)
+        return e;
+    if (dev < 0 || dev >= kMaxDevices)
+        return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_setup_mu);
+    if (!g_setup[dev]) {
+        e = hipStreamCreateWithFlags(&g_setup[dev], hipStreamNonBlocking);
+        if (e != hipSuccess)
+            return e;
+    }
+    *out = g_setup[dev];
+    return hipSuccess;
+}
+
 template<class V>
-hipError_t gen_twiddles(V* t, int64_t M, int64_t stride, int64_t count) {
+hipError_t gen_twiddles(V* t, int64_t M, int64_t stride, int64_t count, hipStream_t st) {
     using T    = decltype(V{}.x);
     int64_t nb = (count + 255) / 256;
-    hipLaunchKernelGGL(k_twiddles<T>, dim3((unsigned)nb), dim3(256), 0, 0, t, M, stride, count);
+    hipLaunchKernelGGL(k_twiddles<T>, dim3((unsigned)nb), dim3(256), 0, st, t, M, stride, count);
     return hipGetLastError();
 }
 
-// ---------------------------------------------------------------- kernel dispatch tables
+
+// ---------------------------------------------------------------- shape lookup
 template<class T>
-using RowLaunchT = void (*)(const RowArgsT<T>&, int64_t nrows, int64_t grid_cap, hipStream_t);
-template<class T>
-using ColLaunchT = void (*)(const ColArgsT<T>&, int64_t grid_cap, hipStream_t);
-using RowLaunch  = RowLaunchT<float>;
-using ColLaunch  = ColLaunchT<float>;
-using XcLaunch  = void (*)(const XcArgs&, hipStream_t);
-
-// grid_cap > 0 bounds the grid; the kernels then loop over their work (grid-stride).  Grids are
-// rounded up to a multiple of 8 for the XCD-aware item order (spare workgroups exit at once).
-int64_t grid8(int64_t n) { return (n + 7) & ~(int64_t)7; }
-// One dispatch covers at most 2^32 - 1 work-items (the 32-bit grid size of an AQL packet).
-// Grid-stride kernels take a grid capped to that (a multiple of 8) and loop over the rest.
-int64_t fit_grid(int64_t nwg, int64_t threads) {
-    const int64_t cap = ((int64_t)0xFFFFFFFF / threads) & ~(int64_t)7;
-    return nwg < cap ? nwg : cap;
-}
-template<int LA, int G, int LD, int ST, int TW, bool UNI = (Plan<LA>::TA >= 64), int ORD = 0, int PREC = PREC_FP32,
-         class T = float>
-void launch_row(const RowArgsT<T>& a, int64_t nrows, int64_t grid_cap, hipStream_t st) {
-    int64_t nwg = (nrows + G - 1) / G;
-    if (grid_cap > 0 && nwg > grid_cap)
-        nwg = grid_cap;
-    nwg = fit_grid(grid8(nwg), G * Plan<LA, false, T>::TA);
-    hipLaunchKernelGGL((k_row<LA, G, LD, ST, TW, UNI, ORD, PREC, T>), dim3((unsigned)nwg),
-                       dim3(G * Plan<LA, false, T>::TA), 0, st, a);
-}
-template<int LA, int C, int TW, int PREC = PREC_FP32, class T = float>
-void launch_col(const ColArgsT<T>& a, int64_t grid_cap, hipStream_t st) {
-    int64_t nwg = a.ntiles;
-    if (grid_cap > 0 && nwg > grid_cap)
-        nwg = grid_cap;
-    nwg = fit_grid(grid8(nwg), C * Plan<LA, false, T>::TA);
-    hipLaunchKernelGGL((k_col<LA, C, TW, PREC, T>), dim3((unsigned)nwg), dim3(C * Plan<LA, false, T>::TA), 0, st, a);
-}
-template<int LA, int PREC = PREC_FP32>
-void launch_xc(const XcArgs& a, hipStream_t st) {
-    constexpr int G     = xc_rows<LA>();
-    constexpr int NT    = G * xc_threads<LA>();
-    const int64_t total = a.nblocks * a.channels;
-    // 32-bit buffer offsets cover a channel when its packed footprint stays below 2^29 floats
-    int  lp    = a.src_lp > a.dst_lp ? a.src_lp : a.dst_lp;
-    bool small = pidx64(a.len + (int64_t)(1 << LA), lp) < ((int64_t)1 << 29) && a.src_lp < 29 && a.dst_lp < 29;
-    // one workgroup per G blocks; launches of at most 2^32 - 1 work-items, in block order
-    const int64_t per = fit_grid(grid8((total + G - 1) / G), NT) * G;
-    XcArgs        s   = a;
-    for (s.blk0 = 0; s.blk0 < total; s.blk0 += per) {
-        const int64_t nwg = grid8((std::min(per, total - s.blk0) + G - 1) / G);
-        if (small)
-            hipLaunchKernelGGL((k_fxcorr<LA, G, true, PREC>), dim3((unsigned)nwg), dim3(NT), 0, st, s);
-        else
-            hipLaunchKernelGGL((k_fxcorr<LA, G, false, PREC>), dim3((unsigned)nwg), dim3(NT), 0, st, s);
-    }
-}
-
-// Single-pass variants: 0 natural in/out; 1, 2 permuted store (forward, bit_reversed /
-// unordered); 3, 4 permuted load (inverse, bit_reversed / unordered).
-using RowSet = std::array<RowLaunch, 5>;
-template<int LA>
-constexpr RowSet single_set() {
-    constexpr int  G   = default_rows<LA>();
-    constexpr bool UNI = Plan<LA>::TA >= 64;
-    return {&launch_row<LA, G, LD_CONTIG, ST_CONTIG, 0, UNI, 0>, &launch_row<LA, G, LD_CONTIG, ST_PERM, 0, UNI, ORD_BITREV>,
-            &launch_row<LA, G, LD_CONTIG, ST_PERM, 0, UNI, ORD_UNORD>, &launch_row<LA, G, LD_PERM, ST_CONTIG, 0, UNI, ORD_BITREV>,
-            &launch_row<LA, G, LD_PERM, ST_CONTIG, 0, UNI, ORD_UNORD>};
-}
-int single_variant(int inverse, int order) { return order == 0 ? 0 : (inverse ? 2 : 0) + order; }
-template<int... L>
-constexpr auto single_table(std::integer_sequence<int, L...>) {
-    return std::array<RowSet, sizeof...(L)>{single_set<L + 1>()...};
-}
-template<int... L>
-constexpr auto xc_table(std::integer_sequence<int, L...>) {
-    return std::array<XcLaunch, sizeof...(L)>{&launch_xc<L + 1>...};
-}
-// index LA-1 for LA in [1, 14]
-const auto kSingle = single_table(std::make_integer_sequence<int, kMaxSinglePass>{});
-// split-precision matrix-core variants (natural order): index [precision - 1][LA - 4] for
-// precision BF16X3 / FP16X2 and LA in [4, 14]
-template<int PREC, int... L>
-constexpr auto single_mfma_table(std::integer_sequence<int, L...>) {
-    return std::array<RowLaunch, sizeof...(L)>{
-        &launch_row<L + 4, default_rows<L + 4>(), LD_CONTIG, ST_CONTIG, 0, (Plan<L + 4>::TA >= 64), 0, PREC>...};
-}
-const std::array<std::array<RowLaunch, kMaxSinglePass - 3>, 2> kSingleMfma = {
-    single_mfma_table<PREC_BF16X3>(std::make_integer_sequence<int, kMaxSinglePass - 3>{}),
-    single_mfma_table<PREC_FP16X2>(std::make_integer_sequence<int, kMaxSinglePass - 3>{})};
-const auto kXc = xc_table(std::make_integer_sequence<int, kMaxSinglePass>{});
-template<int PREC, int... L>
-constexpr auto xc_mfma_table(std::integer_sequence<int, L...>) {
-    return std::array<XcLaunch, sizeof...(L)>{&launch_xc<L + 4, PREC>...};
-}
-const std::array<std::array<XcLaunch, kMaxSinglePass - 3>, 2> kXcMfma = {
-    xc_mfma_table<PREC_BF16X3>(std::make_integer_sequence<int, kMaxSinglePass - 3>{}),
-    xc_mfma_table<PREC_FP16X2>(std::make_integer_sequence<int, kMaxSinglePass - 3>{})};
-
-using PassSet = std::array<RowLaunch, 6>;
-// Pass-twiddle table capacity in LDS for N = 2^LG (2: too large, read from global memory).
-template<int LG>
-constexpr int tw_cap() {
-    return pass_tw_entries(LG) <= 2048 ? pass_tw_entries(LG) : 2;
-}
-// ---- FP64 (fft_unit<double, Order>): natural and bit_reversed orders (the reference has no FP64
-// unordered kernel: avx2_fft.hpp:114-217 is written for float only)
-template<int LA>
-constexpr std::array<RowLaunchT<double>, 5> single_set64() {
-    using D             = double;
-    constexpr int  G    = default_rows<LA, D>();
-    constexpr bool UNI  = Plan<LA, false, D>::TA >= 64;
-    return {&launch_row<LA, G, LD_CONTIG, ST_CONTIG, 0, UNI, 0, PREC_FP32, D>,
-            &launch_row<LA, G, LD_CONTIG, ST_PERM, 0, UNI, ORD_BITREV, PREC_FP32, D>, nullptr,
-            &launch_row<LA, G, LD_PERM, ST_CONTIG, 0, UNI, ORD_BITREV, PREC_FP32, D>, nullptr};
-}
-template<int... L>
-constexpr auto single_table64(std::integer_sequence<int, L...>) {
-    return std::array<std::array<RowLaunchT<double>, 5>, sizeof...(L)>{single_set64<L + 1>()...};
-}
-const auto kSingle64 = single_table64(std::make_integer_sequence<int, kMaxSinglePass64>{}); // index LA-1
-
-// Four-step row-pass variants: forward with strided store (natural order) or permuted store
-// (bit_reversed, unordered); inverse with strided or permuted load and the pass twiddle.
-// The G rows of a workgroup lie in one transform (G <= N2), so buffer addressing applies.
-template<int LA, int G, int LG>
-constexpr PassSet pass_set() {
-    constexpr int TW = tw_cap<LG>();
-    return {&launch_row<LA, G, LD_CONTIG, ST_STRIDED, 0, true, 0>,
-            &launch_row<LA, G, LD_CONTIG, ST_PERM, 0, true, ORD_BITREV>,
-            &launch_row<LA, G, LD_CONTIG, ST_PERM, 0, true, ORD_UNORD>,
-            &launch_row<LA, G, LD_STRIDED, ST_CONTIG, TW, true, 0>,
-            &launch_row<LA, G, LD_PERM, ST_CONTIG, TW, true, ORD_BITREV>,
-            &launch_row<LA, G, LD_PERM, ST_CONTIG, TW, true, ORD_UNORD>};
-}
-template<int LA, int C, int LG>
-constexpr std::array<ColLaunch, 2> col_set() {
-    return {&launch_col<LA, C, 0>, &launch_col<LA, C, tw_cap<LG>()>};
-}
-
-// The four-step shapes this build instantiates: N = 2^lg = N1 * N2 with N1 = 2^n1 (row length,
-// G rows per workgroup, G * N1 / PPT <= 1024 threads) and N2 = 2^n2 (column length, C columns
-// per tile, C * N2 <= 16384 for LDS).
-template<class T>
-struct FourStepT {
-    int                              lg, n1, n2, g, c;
-    std::array<RowLaunchT<T>, 6>     rows;
-    std::array<ColLaunchT<T>, 2>     cols;
-};
-using FourStep = FourStepT<float>;
-template<int LG, int N1, int N2, int G, int C>
-FourStep shape() {
-    static_assert(N1 + N2 == LG, "shape");
-    return {LG, N1, N2, G, C, pass_set<N1, G, LG>(), col_set<N2, C, LG>()};
-}
-const std::array<FourStep, 27> kShapes = {
-    // defaults (first match per lg): measured best on MI355X for 2^16 is N1 = 512, N2 = 128 with
-    // 128-column tiles (512-byte load and store segments in the column pass; 128 KiB of LDS, one
-    // workgroup of 1024 threads per CU): 0.778 ms per 2 GiB against 0.855 ms with 64 columns;
-    // 2^15 (four-step forward, 64 rows per row workgroup) 1.59 against 1.70 ms per 2 GiB
-    shape<15, 8, 7, 64, 128>(), shape<16, 9, 7, 32, 128>(), shape<16, 8, 8, 32, 64>(), shape<17, 9, 8, 32, 64>(), shape<18, 9, 9, 32, 32>(),
-    // 2^19: 512-point rows, 32 per workgroup (128-byte store segments), 1.91 against 2.03 ms per
-    // 2 GiB with 1024-point rows (the alternative below)
-    shape<19, 9, 10, 32, 16>(), shape<20, 10, 10, 16, 16>(), shape<21, 11, 10, 8, 16>(), shape<22, 12, 10, 4, 16>(),
-    // 2^23, 2^24: more rows per workgroup than the LDS of two workgroups per CU allows (they
-    // spill, but store 4 / 2 floats per segment instead of 2 / 1): 5.03 / 5.87 ms per 2 GiB
-    // forward against 7.47 / 8.94
-    shape<23, 12, 11, 4, 8>(), shape<24, 14, 10, 2, 16>(),
-    // alternatives for N = 2^16 explored by the tuning sweeps
-    shape<16, 8, 8, 16, 64>(), shape<16, 8, 8, 64, 64>(), shape<16, 8, 8, 32, 32>(), shape<16, 8, 8, 16, 32>(),
-    shape<16, 10, 6, 16, 64>(), shape<16, 7, 9, 64, 32>(), shape<16, 6, 10, 64, 16>(),
-    shape<16, 9, 7, 32, 32>(), shape<16, 10, 6, 16, 32>(), shape<16, 9, 7, 32, 64>(), shape<15, 8, 7, 32, 64>(), shape<15, 8, 7, 32, 128>(),
-    shape<19, 10, 9, 16, 32>(),
-    // alternatives for the largest sizes explored by the sweeps
-    shape<23, 13, 10, 2, 16>(), shape<24, 14, 10, 1, 16>(), shape<24, 13, 11, 2, 8>(),
-};
-// FP64 shapes, 2^14 .. 2^23, 256-512 threads per workgroup.  C and G are the doubles per column
-// segment and per row store segment.  The defaults are the measured best of the tuning sweeps on
-// MI355X (2 GiB batches, forward): wider segments win over occupancy, e.g. 2^20 with C = G = 8
-// and 129-133 KiB per workgroup runs in 2.43 ms against 3.23 ms for C = 8, G = 2.  Pass-twiddle
-// tables above 512 entries are read from global memory.
-template<int LG>
-constexpr int tw_cap64() {
-    return pass_tw_entries(LG) <= 512 ? pass_tw_entries(LG) : 2;
-}
-template<int LG, int N1, int N2, int G, int C>
-FourStepT<double> shape64() {
-    static_assert(N1 + N2 == LG && G <= (1 << N2), "shape");
-    using D          = double;
-    constexpr int TW = tw_cap64<LG>();
-    return {LG, N1, N2, G, C,
-            {&launch_row<N1, G, LD_CONTIG, ST_STRIDED, 0, true, 0, PREC_FP32, D>,
-             &launch_row<N1, G, LD_CONTIG, ST_PERM, 0, true, ORD_BITREV, PREC_FP32, D>, nullptr,
-             &launch_row<N1, G, LD_STRIDED, ST_CONTIG, TW, true, 0, PREC_FP32, D>,
-             &launch_row<N1, G, LD_PERM, ST_CONTIG, TW, true, ORD_BITREV, PREC_FP32, D>, nullptr},
-            {&launch_col<N2, C, 0, PREC_FP32, D>, &launch_col<N2, C, TW, PREC_FP32, D>}};
-}
-const std::array<FourStepT<double>, 29> kShapes64 = {
-    // defaults (first match per lg); forward ms for a 2 GiB batch measured on MI355X
-    // 2^14 .. 2^16: one workgroup per CU with 512-byte column segments (2^14, 2^15: 64 columns)
-    // and 256-byte segments in both passes at 2^16 (32 columns, 32 rows): 1.63 / 1.58 / 1.62 ms
-    // against 1.68 / 1.68 / 1.73 ms with the two-workgroups-per-CU tiles listed below
-    shape64<14, 7, 7, 32, 64>(),  shape64<15, 8, 7, 32, 64>(), shape64<16, 8, 8, 32, 32>(),
-    shape64<17, 8, 9, 32, 16>(),  // 1.84 ms
-    shape64<18, 10, 8, 8, 32>(),  // 1.89
-    shape64<19, 10, 9, 8, 16>(),  // 2.41
-    shape64<20, 10, 10, 8, 8>(),  // 2.43
-    shape64<21, 11, 10, 4, 8>(),  // 3.11
-    shape64<22, 12, 10, 4, 8>(),  // 3.12
-    shape64<23, 13, 10, 2, 4>(),  // 4.91
-    // alternatives explored by the sweeps
-    shape64<17, 9, 8, 8, 16>(),   // 2.08
-    shape64<17, 9, 8, 16, 16>(),  // 1.90 (inverse 2.05)
-    shape64<18, 10, 8, 4, 16>(),  // 2.71
-    shape64<18, 9, 9, 8, 8>(),    // 2.22
-    shape64<18, 8, 10, 16, 4>(),  // 2.15
-    shape64<19, 10, 9, 4, 8>(),   // 3.25
-    shape64<19, 9, 10, 8, 4>(),   // 2.81
-    shape64<20, 11, 9, 2, 8>(),   // 3.23
-    shape64<20, 10, 10, 4, 4>(),  // 3.15
-    shape64<20, 11, 9, 4, 8>(),   // 2.63
-    shape64<21, 12, 9, 2, 8>(),   // 3.59
-    shape64<22, 13, 9, 1, 8>(),   // 4.49
-    shape64<23, 13, 10, 1, 4>(),  // 5.91
-    shape64<14, 7, 7, 32, 32>(), shape64<15, 8, 7, 16, 32>(), shape64<16, 8, 8, 16, 16>(),
-    shape64<17, 8, 9, 16, 8>(),   // 1.88
-    shape64<18, 10, 8, 8, 16>(),  // 2.02
-    shape64<19, 10, 9, 8, 8>()};  // 2.44
-const FourStepT<double>* find_shape64(int lg, int n2, int g, int c) {
-    for (const auto& f : kShapes64)
+const FourStepT<T>* find_in(const ShapeList<T>& l, int lg, int n2, int g, int c) {
+    for (int i = 0; i < l.count; ++i) {
+        const FourStepT<T>& f = l.data[i];
         if (f.lg == lg && (n2 == 0 || f.n2 == n2) && (g == 0 || f.g == g) && (c == 0 || f.c == c))
             return &f;
+    }
     return nullptr;
 }
-
-// split-precision matrix-core variants of the default shapes for 2^15 .. 2^20 (natural order):
-// index [precision - 1], rows {forward, inverse}, cols {inverse, forward}
-struct FourStepMfma {
-    int                                     lg;
-    int                                     n1, n2; // the split its kernels are built for
-    int                                     c;      // columns per tile of its column kernels
-    std::array<std::array<RowLaunch, 2>, 2> rows;
-    std::array<std::array<ColLaunch, 2>, 2> cols;
-};
-template<int LG, int N1, int N2, int G, int C>
-FourStepMfma shape_mfma() {
-    static_assert(N1 + N2 == LG, "shape");
-    return {LG, N1, N2, C,
-            {{{&launch_row<N1, G, LD_CONTIG, ST_STRIDED, 0, true, 0, PREC_BF16X3>,
-               &launch_row<N1, G, LD_STRIDED, ST_CONTIG, tw_cap<LG>(), true, 0, PREC_BF16X3>},
-              {&launch_row<N1, G, LD_CONTIG, ST_STRIDED, 0, true, 0, PREC_FP16X2>,
-               &launch_row<N1, G, LD_STRIDED, ST_CONTIG, tw_cap<LG>(), true, 0, PREC_FP16X2>}}},
-            {{{&launch_col<N2, C, 0, PREC_BF16X3>, &launch_col<N2, C, tw_cap<LG>(), PREC_BF16X3>},
-              {&launch_col<N2, C, 0, PREC_FP16X2>, &launch_col<N2, C, tw_cap<LG>(), PREC_FP16X2>}}}};
+// The first listed shape that matches (0 = any): the defaults first, then the alternatives.
+const FourStep* find_shape(int lg, int n2, int g, int c) {
+    if (const FourStep* f = find_in(shapes_default(), lg, n2, g, c))
+        return f;
+    return find_in(shapes_alt(), lg, n2, g, c);
 }
-const std::array<FourStepMfma, 6> kShapesMfma = {shape_mfma<15, 8, 7, 32, 64>(),  shape_mfma<16, 9, 7, 32, 64>(),
-                                                 shape_mfma<17, 9, 8, 32, 64>(),  shape_mfma<18, 9, 9, 32, 32>(),
-                                                 shape_mfma<19, 9, 10, 32, 16>(), shape_mfma<20, 10, 10, 16, 16>()};
-// One-pass decimation-in-frequency kernel (k_dif) for N = 2^15 = M * R: natural order with
-// M = 2^14, R = 2; bit_reversed / unordered with M = 2^13, R = 4 (two workgroups per CU; the
-// stores are contiguous there).  Measured on MI355X against the four-step for a 2 GiB batch:
-// 1.55 vs 1.72 ms (natural), 1.50 vs 1.70 ms (bit_reversed).  For 2^16 and 2^17 the R-fold
-// rereads cost more than the second pass saves (2.27 vs 1.71 ms at 2^16), so those sizes stay
-// four-step.  Index [order].
-using DifLaunch = void (*)(const RowArgs&, int64_t batch, hipStream_t);
-template<int LM, int LR, int ORD>
-void launch_dif(const RowArgs& a, int64_t batch, hipStream_t st) {
-    int64_t nwg = ((batch + 7) / 8) * 8 * ((int64_t)1 << LR);
-    hipLaunchKernelGGL((k_dif<LM, LR, ORD>), dim3((unsigned)nwg), dim3(Plan<LM, false, float, dif_ppt<LM>()>::TA), 0, st,
-                       a);
+const FourStepT<double>* find_shape64(int lg, int n2, int g, int c) {
+    if (const FourStepT<double>* f = find_in(shapes64_default(), lg, n2, g, c))
+        return f;
+    return find_in(shapes64_alt(), lg, n2, g, c);
 }
-constexpr int kDifLog2 = 15;
-const std::array<DifLaunch, 3> kDif = {&launch_dif<14, 1, ORD_NORMAL>, &launch_dif<13, 2, ORD_BITREV>,
-                                       &launch_dif<13, 2, ORD_UNORD>};
-constexpr int dif_core_log2(int order) { return order == 0 ? 14 : 13; }
-
-const FourStep* find_shape(int lg, int n2, int g, int c);
 // The split kernels for the plan's shape: same size and the same N1 x N2 split (the plan's
 // scratch layout and row / column counts follow its shape); only the default shapes have them.
 const FourStepMfma* find_mfma(int lg, const FourStep* sh) {
-    for (const auto& f : kShapesMfma)
+    for (const auto& f : shapes_mfma())
         if (f.lg == lg && sh == find_shape(lg, 0, 0, 0) && f.n1 == sh->n1 && f.n2 == sh->n2)
-            return &f;
-    return nullptr;
-}
-
-const FourStep* find_shape(int lg, int n2, int g, int c) {
-    for (const auto& f : kShapes)
-        if (f.lg == lg && (n2 == 0 || f.n2 == n2) && (g == 0 || f.g == g) && (c == 0 || f.c == c))
             return &f;
     return nullptr;
 }
@@ -389,14 +143,19 @@ struct pcx_plan_s {
     float2* twl    = nullptr;     // W_N^e, e < 2^tw_ls
     float2* twh    = nullptr;     // W_N^(e << tw_ls)
     int     tw_ls  = 0;
-    float*  scratch      = nullptr;
-    int64_t scratch_cap  = 0; // transforms the scratch holds
+    // four-step intermediate of one chunk (allocated on the first large call); calls on different
+    // streams are ordered on it: tmp_ev is recorded after the last use, on tmp_stream
+    float*      tmp        = nullptr;
+    int64_t     tmp_cap    = 0; // transforms the intermediate holds
+    hipEvent_t  tmp_ev     = nullptr;
+    hipStream_t tmp_stream = nullptr;
+    bool        tmp_used   = false;
     int64_t chunk        = 0; // transforms per four-step chunk
     const FourStep* shape = nullptr; // four-step kernels and tile sizes
     int     precision    = PCX_PREC_FP32;
     int64_t grid_col     = 2048; // grid caps of the two passes (persistent loop; 0: one workgroup per tile)
     int64_t grid_row     = 2048;
-    int     nt_flags     = 0; // bit 0/1: streaming load/store of src/dst, bit 2/3: of the scratch
+    int     nt_flags     = 0; // bit 0/1: streaming load/store of src/dst, bit 2/3: of the intermediate
     float2* tw_dif       = nullptr; // W_M for the one-pass k_dif kernel
     bool    use_dif      = false;   // out-of-place transforms of 2^15 run as one k_dif pass
     // FP64 plans (pcx_plan_create_f64): the same roles in double precision
@@ -409,8 +168,39 @@ struct pcx_plan_s {
     std::mutex mu;
 };
 
-extern "C" int pcx_version(void) { return 1; }
+extern "C" int pcx_version(void) { return 2; }
 extern "C" const char* pcx_last_error(void) { return g_err.c_str(); }
+
+namespace {
+
+// Default bytes of one four-step chunk's intermediate (2 GiB: the whole 2^28-point headline batch
+// in one chunk; a 288 GB part holds it several times over).
+constexpr int64_t kChunkBytes = (int64_t)2048 << 20;
+
+// Twiddle tables of a four-step FP32 plan for its shape's N1 x N2 split (W_N1, W_N2) and the
+// pass tables W_N = lo[e mod 2^ls] * hi[e >> ls].
+hipError_t fourstep_tables(pcx_plan_t p, hipStream_t st) {
+    const int64_t n1 = (int64_t)1 << p->n1_lg, n2 = (int64_t)1 << p->n2_lg;
+    const int64_t nl = (int64_t)1 << p->tw_ls, nh = p->n >> p->tw_ls;
+    hipError_t    he = hipMalloc(&p->tw_row, sizeof(float2) * n1);
+    if (he == hipSuccess)
+        he = hipMalloc(&p->tw_col, sizeof(float2) * n2);
+    if (he == hipSuccess && !p->twl)
+        he = hipMalloc(&p->twl, sizeof(float2) * nl);
+    if (he == hipSuccess && !p->twh)
+        he = hipMalloc(&p->twh, sizeof(float2) * nh);
+    if (he == hipSuccess)
+        he = gen_twiddles(p->tw_row, n1, 1, n1, st);
+    if (he == hipSuccess)
+        he = gen_twiddles(p->tw_col, n2, 1, n2, st);
+    if (he == hipSuccess)
+        he = gen_twiddles(p->twl, p->n, 1, nl, st);
+    if (he == hipSuccess)
+        he = gen_twiddles(p->twh, p->n, nl, nh, st);
+    return he;
+}
+
+} // namespace
 
 extern "C" int pcx_plan_create(int64_t fft_size, int64_t sub_size, int order, pcx_plan_t* out) {
     if (!out)
@@ -428,6 +218,8 @@ extern "C" int pcx_plan_create(int64_t fft_size, int64_t sub_size, int order, pc
     if (order == PCX_ORDER_UNORDERED && lg < 5)
         return fail(PCX_ERR_UNSUPPORTED,
                     "unordered fft_size (which is " + s(fft_size) + ") is below the 32-point finisher size");
+    hipStream_t st = nullptr;
+    HIPCHK(setup_stream(&st));
     auto* p  = new pcx_plan_s;
     p->n     = fft_size;
     p->lg    = lg;
@@ -436,22 +228,10 @@ extern "C" int pcx_plan_create(int64_t fft_size, int64_t sub_size, int order, pc
     if (lg <= kMaxSinglePass) {
         he = hipMalloc(&p->tw_row, sizeof(float2) * fft_size);
         if (he == hipSuccess)
-            he = gen_twiddles(p->tw_row, fft_size, 1, fft_size);
+            he = gen_twiddles(p->tw_row, fft_size, 1, fft_size, st);
     } else {
         p->two_pass = true;
-        // shape: N2 = 2^min(lg/2, 10) columns-length by default; PCX_N2_LOG2 and PCX_TILE ("C,G")
-        // select one of the other instantiated shapes (tuning sweeps)
-        int n2s = 0, c = 0, g = 0; // 0: the default (first listed) shape for this size
-        if (const char* env = std::getenv("PCX_N2_LOG2"))
-            n2s = std::atoi(env);
-        if (const char* env = std::getenv("PCX_TILE"))
-            if (std::sscanf(env, "%d,%d", &c, &g) != 2)
-                c = g = 0;
-        p->shape = find_shape(lg, n2s, g, c);
-        if (!p->shape)
-            p->shape = find_shape(lg, n2s, 0, 0);
-        if (!p->shape)
-            p->shape = find_shape(lg, 0, 0, 0);
+        p->shape    = find_shape(lg, 0, 0, 0); // the default (first listed) shape of this size
         if (!p->shape) {
             delete p;
             return fail(PCX_ERR_UNSUPPORTED, "no four-step kernels for fft_size " + s(fft_size));
@@ -459,50 +239,22 @@ extern "C" int pcx_plan_create(int64_t fft_size, int64_t sub_size, int order, pc
         p->n2_lg = p->shape->n2;
         p->n1_lg = p->shape->n1;
         p->tw_ls = (lg + 1) / 2;
-        int64_t n1 = (int64_t)1 << p->n1_lg, n2 = (int64_t)1 << p->n2_lg;
-        int64_t nl = (int64_t)1 << p->tw_ls, nh = fft_size >> p->tw_ls;
-        he = hipMalloc(&p->tw_row, sizeof(float2) * n1);
-        if (he == hipSuccess)
-            he = hipMalloc(&p->tw_col, sizeof(float2) * n2);
-        if (he == hipSuccess)
-            he = hipMalloc(&p->twl, sizeof(float2) * nl);
-        if (he == hipSuccess)
-            he = hipMalloc(&p->twh, sizeof(float2) * nh);
-        if (he == hipSuccess)
-            he = gen_twiddles(p->tw_row, n1, 1, n1);
-        if (he == hipSuccess)
-            he = gen_twiddles(p->tw_col, n2, 1, n2);
-        if (he == hipSuccess)
-            he = gen_twiddles(p->twl, fft_size, 1, nl);
-        if (he == hipSuccess)
-            he = gen_twiddles(p->twh, fft_size, nl, nh);
-        // transforms per four-step chunk: the scratch holds one chunk's intermediate
-        int64_t chunk_bytes = (int64_t)2048 << 20;
-        if (const char* env = std::getenv("PCX_CHUNK_MB"))
-            chunk_bytes = (int64_t)std::atoll(env) << 20;
-        p->chunk = chunk_bytes / (fft_size * 8);
-        if (p->chunk < 1)
-            p->chunk = 1;
-        if (const char* env = std::getenv("PCX_GRID")) { // "col,row" grid caps for tuning sweeps
-            long long ga = 0, gb = 0;
-            if (std::sscanf(env, "%lld,%lld", &ga, &gb) == 2)
-                p->grid_col = ga, p->grid_row = gb;
-        }
-        if (const char* env = std::getenv("PCX_NT"))
-            p->nt_flags = std::atoi(env);
+        he       = fourstep_tables(p, st);
+        // transforms per four-step chunk: the intermediate holds one chunk
+        p->chunk = std::max<int64_t>(1, kChunkBytes / (fft_size * 8));
         if (lg == kDifLog2) {
-            p->use_dif = true;
-            if (const char* env = std::getenv("PCX_DIF"))
-                p->use_dif = std::atoi(env) != 0;
+            p->use_dif     = true;
             const int64_t m = (int64_t)1 << dif_core_log2(order);
             if (he == hipSuccess)
                 he = hipMalloc(&p->tw_dif, sizeof(float2) * m);
             if (he == hipSuccess)
-                he = gen_twiddles(p->tw_dif, m, 1, m);
+                he = gen_twiddles(p->tw_dif, m, 1, m, st);
         }
+        if (he == hipSuccess)
+            he = hipEventCreateWithFlags(&p->tmp_ev, hipEventDisableTiming);
     }
     if (he == hipSuccess)
-        he = hipDeviceSynchronize();
+        he = hipStreamSynchronize(st);
     if (he != hipSuccess) {
         pcx_plan_destroy(p);
         return fail(PCX_ERR_HIP, std::string("plan setup: ") + hipGetErrorString(he));
@@ -514,16 +266,21 @@ extern "C" int pcx_plan_create(int64_t fft_size, int64_t sub_size, int order, pc
 extern "C" int pcx_plan_destroy(pcx_plan_t p) {
     if (!p)
         return PCX_SUCCESS;
+    // the intermediate may still be in use by enqueued work
+    if (p->tmp_ev && p->tmp_used)
+        (void)hipEventSynchronize(p->tmp_ev);
     (void)hipFree(p->tw_row);
     (void)hipFree(p->tw_col);
     (void)hipFree(p->twl);
     (void)hipFree(p->twh);
-    (void)hipFree(p->scratch);
+    (void)hipFree(p->tmp);
     (void)hipFree(p->tw_dif);
     (void)hipFree(p->tw64_row);
     (void)hipFree(p->tw64_col);
     (void)hipFree(p->twl64);
     (void)hipFree(p->twh64);
+    if (p->tmp_ev)
+        (void)hipEventDestroy(p->tmp_ev);
     delete p;
     return PCX_SUCCESS;
 }
@@ -545,6 +302,8 @@ extern "C" int pcx_plan_create_f64(int64_t fft_size, int64_t sub_size, int order
     if (lg > kMaxLog2_64)
         return fail(PCX_ERR_UNSUPPORTED,
                     "double fft_size (which is " + s(fft_size) + ") exceeds the supported maximum 2^23");
+    hipStream_t st = nullptr;
+    HIPCHK(setup_stream(&st));
     auto* p  = new pcx_plan_s;
     p->n     = fft_size;
     p->lg    = lg;
@@ -554,22 +313,13 @@ extern "C" int pcx_plan_create_f64(int64_t fft_size, int64_t sub_size, int order
     if (lg <= kMaxSinglePass64) {
         he = hipMalloc(&p->tw64_row, sizeof(double2) * fft_size);
         if (he == hipSuccess)
-            he = gen_twiddles(p->tw64_row, fft_size, 1, fft_size);
+            he = gen_twiddles(p->tw64_row, fft_size, 1, fft_size, st);
     } else {
         p->two_pass = true;
-        // PCX_N2_LOG2 and PCX_TILE ("C,G") select another instantiated shape (tuning sweeps)
-        int n2s = 0, c = 0, g = 0;
-        if (const char* env = std::getenv("PCX_N2_LOG2"))
-            n2s = std::atoi(env);
-        if (const char* env = std::getenv("PCX_TILE"))
-            if (std::sscanf(env, "%d,%d", &c, &g) != 2)
-                c = g = 0;
-        p->shape64 = find_shape64(lg, n2s, g, c);
-        if (!p->shape64)
-            p->shape64 = find_shape64(lg, 0, 0, 0);
-        p->n1_lg = p->shape64->n1;
-        p->n2_lg = p->shape64->n2;
-        p->tw_ls = (lg + 1) / 2;
+        p->shape64  = find_shape64(lg, 0, 0, 0);
+        p->n1_lg    = p->shape64->n1;
+        p->n2_lg    = p->shape64->n2;
+        p->tw_ls    = (lg + 1) / 2;
         int64_t n1 = (int64_t)1 << p->n1_lg, n2 = (int64_t)1 << p->n2_lg;
         int64_t nl = (int64_t)1 << p->tw_ls, nh = fft_size >> p->tw_ls;
         he = hipMalloc(&p->tw64_row, sizeof(double2) * n1);
@@ -580,22 +330,19 @@ extern "C" int pcx_plan_create_f64(int64_t fft_size, int64_t sub_size, int order
         if (he == hipSuccess)
             he = hipMalloc(&p->twh64, sizeof(double2) * nh);
         if (he == hipSuccess)
-            he = gen_twiddles(p->tw64_row, n1, 1, n1);
+            he = gen_twiddles(p->tw64_row, n1, 1, n1, st);
         if (he == hipSuccess)
-            he = gen_twiddles(p->tw64_col, n2, 1, n2);
+            he = gen_twiddles(p->tw64_col, n2, 1, n2, st);
         if (he == hipSuccess)
-            he = gen_twiddles(p->twl64, fft_size, 1, nl);
+            he = gen_twiddles(p->twl64, fft_size, 1, nl, st);
         if (he == hipSuccess)
-            he = gen_twiddles(p->twh64, fft_size, nl, nh);
-        int64_t chunk_bytes = (int64_t)2048 << 20;
-        if (const char* env = std::getenv("PCX_CHUNK_MB"))
-            chunk_bytes = (int64_t)std::atoll(env) << 20;
-        p->chunk = chunk_bytes / (fft_size * 16);
-        if (p->chunk < 1)
-            p->chunk = 1;
+            he = gen_twiddles(p->twh64, fft_size, nl, nh, st);
+        p->chunk = std::max<int64_t>(1, kChunkBytes / (fft_size * 16));
+        if (he == hipSuccess)
+            he = hipEventCreateWithFlags(&p->tmp_ev, hipEventDisableTiming);
     }
     if (he == hipSuccess)
-        he = hipDeviceSynchronize();
+        he = hipStreamSynchronize(st);
     if (he != hipSuccess) {
         pcx_plan_destroy(p);
         return fail(PCX_ERR_HIP, std::string("plan setup: ") + hipGetErrorString(he));
@@ -634,21 +381,42 @@ namespace {
 // Switches a four-step plan to another instantiated shape, rebuilding the W_N1 / W_N2 tables
 // when the split changes (the pass tables W_N do not depend on the split).
 int plan_apply_shape(pcx_plan_t p, const FourStep* sh) {
-    if (!p->two_pass || !sh || sh->lg != p->lg)
+    if (!p->two_pass || p->f64 || !sh || sh->lg != p->lg)
         return fail(PCX_ERR_INVALID_ARGUMENT, "shape does not match the plan");
     if (sh->n1 != p->n1_lg || sh->n2 != p->n2_lg) {
+        // the W_N1 / W_N2 tables follow the split; every enqueued use of the old ones must end
         HIPCHK(hipDeviceSynchronize());
         HIPCHK(hipFree(p->tw_row));
         HIPCHK(hipFree(p->tw_col));
         p->tw_row = p->tw_col = nullptr;
-        int64_t n1 = (int64_t)1 << sh->n1, n2 = (int64_t)1 << sh->n2;
-        HIPCHK(hipMalloc(&p->tw_row, sizeof(float2) * n1));
-        HIPCHK(hipMalloc(&p->tw_col, sizeof(float2) * n2));
-        HIPCHK(gen_twiddles(p->tw_row, n1, 1, n1));
-        HIPCHK(gen_twiddles(p->tw_col, n2, 1, n2));
-        HIPCHK(hipDeviceSynchronize());
+        p->n1_lg = sh->n1, p->n2_lg = sh->n2;
+        hipStream_t st = nullptr;
+        HIPCHK(setup_stream(&st));
+        HIPCHK(fourstep_tables(p, st));
+        HIPCHK(hipStreamSynchronize(st));
     }
     p->shape = sh, p->n1_lg = sh->n1, p->n2_lg = sh->n2;
+    return PCX_SUCCESS;
+}
+
+int plan_apply_shape64(pcx_plan_t p, const FourStepT<double>* sh) {
+    if (!p->two_pass || !p->f64 || !sh || sh->lg != p->lg)
+        return fail(PCX_ERR_INVALID_ARGUMENT, "shape does not match the plan");
+    if (sh->n1 != p->n1_lg || sh->n2 != p->n2_lg) {
+        HIPCHK(hipDeviceSynchronize());
+        HIPCHK(hipFree(p->tw64_row));
+        HIPCHK(hipFree(p->tw64_col));
+        p->tw64_row = p->tw64_col = nullptr;
+        const int64_t n1 = (int64_t)1 << sh->n1, n2 = (int64_t)1 << sh->n2;
+        hipStream_t   st = nullptr;
+        HIPCHK(setup_stream(&st));
+        HIPCHK(hipMalloc(&p->tw64_row, sizeof(double2) * n1));
+        HIPCHK(hipMalloc(&p->tw64_col, sizeof(double2) * n2));
+        HIPCHK(gen_twiddles(p->tw64_row, n1, 1, n1, st));
+        HIPCHK(gen_twiddles(p->tw64_col, n2, 1, n2, st));
+        HIPCHK(hipStreamSynchronize(st));
+    }
+    p->shape64 = sh, p->n1_lg = sh->n1, p->n2_lg = sh->n2;
     return PCX_SUCCESS;
 }
 
@@ -656,12 +424,20 @@ int plan_apply_shape(pcx_plan_t p, const FourStep* sh) {
 
 namespace {
 
+// A four-step intermediate the caller provides (the allocation-free entry point): `bytes` of
+// device memory at `buf`, used instead of the plan's own.
+struct ExtTmp {
+    void*   buf;
+    int64_t bytes;
+};
+
 // src_offs / dst_offs (optional, device arrays of `batch` element offsets) replace b * dist.
 // T is the element type: float, or double for a plan made by pcx_plan_create_f64.
 template<class T>
 int run_batch(pcx_plan_t p, int inverse, int normalized, T* dst, int64_t dst_pack, int64_t dst_dist,
               const T* src, int64_t src_pack, int64_t src_dist, int64_t src_size, int64_t batch,
-              hipStream_t st, const int64_t* src_offs = nullptr, const int64_t* dst_offs = nullptr) {
+              hipStream_t st, const int64_t* src_offs = nullptr, const int64_t* dst_offs = nullptr,
+              const ExtTmp* ext = nullptr) {
     constexpr bool F64 = std::is_same_v<T, double>;
     if (!p)
         return fail(PCX_ERR_INVALID_ARGUMENT, "plan is null");
@@ -689,7 +465,10 @@ int run_batch(pcx_plan_t p, int inverse, int normalized, T* dst, int64_t dst_pac
         if (!inverse && dst_pack < 8)
             dst_pack = 8;
     }
-    if ((const T*)dst == src && dst_pack != src_pack && (dst_pack > 8 || src_pack > 8))
+    // fft.hpp:783, 889: a pack change wider than one SIMD register (8 floats, 4 doubles) cannot run
+    // in place
+    constexpr int64_t kReg = F64 ? 4 : 8;
+    if ((const T*)dst == src && dst_pack != src_pack && (dst_pack > kReg || src_pack > kReg))
         return fail(PCX_ERR_INPLACE_REPACK, "cannot perform in-place repack from source pack size (which is " +
                                                 s(src_pack) + ") to destination pack size (which is " + s(dst_pack) + ")");
     if (batch == 0)
@@ -712,11 +491,11 @@ int run_batch(pcx_plan_t p, int inverse, int normalized, T* dst, int64_t dst_pac
         a.twA = tw_row, a.twl = nullptr, a.twh = nullptr, a.tw_ls = 0;
         a.src_offs = src_offs, a.dst_offs = dst_offs;
         if constexpr (F64)
-            kSingle64[p->lg - 1][single_variant(inverse, p->order)](a, batch, 0, st);
+            single64_tab()[p->lg - 1][single_variant(inverse, p->order)](a, batch, 0, st);
         else if (p->precision != PCX_PREC_FP32)
-            kSingleMfma[p->precision - 1][p->lg - 4](a, batch, 0, st);
+            single_mfma_tab()[p->precision - 1][p->lg - 4](a, batch, 0, st);
         else
-            kSingle[p->lg - 1][single_variant(inverse, p->order)](a, batch, 0, st);
+            single_tab()[p->lg - 1][single_variant(inverse, p->order)](a, batch, 0, st);
         HIPCHK(hipGetLastError());
         return PCX_SUCCESS;
     }
@@ -732,26 +511,49 @@ int run_batch(pcx_plan_t p, int inverse, int normalized, T* dst, int64_t dst_pac
         a.twA = p->tw_dif, a.twl = p->twl, a.twh = p->twh, a.tw_ls = p->tw_ls, a.tw_lg = p->lg;
         a.nt_store = (p->nt_flags >> 1) & 1;
         a.src_offs = src_offs, a.dst_offs = dst_offs;
-        kDif[p->order](a, batch, st);
+        dif_tab()[p->order](a, batch, st);
         HIPCHK(hipGetLastError());
         return PCX_SUCCESS;
     }
 
-    // ---- four-step: N = N1 * N2 through a plan-owned scratch, `chunk` transforms at a time
+    // ---- four-step: N = N1 * N2 through an intermediate, `chunk` transforms at a time
     std::lock_guard<std::mutex> lock(p->mu);
     int64_t chunk = batch < p->chunk ? batch : p->chunk;
-    if (p->scratch_cap < chunk) {
-        if (p->scratch) {
-            HIPCHK(hipStreamSynchronize(st));
-            HIPCHK(hipFree(p->scratch));
-            p->scratch = nullptr;
+    T*      tmp   = nullptr;
+    if (ext) {
+        const int64_t cap = ext->bytes / (int64_t)(2 * sizeof(T) * n);
+        if (cap < 1)
+            return fail(PCX_ERR_UNSUPPORTED, "fft_size (which is " + s(n) + ") needs " + s(2 * sizeof(T) * n) +
+                                                 " bytes of intermediate; the fixed workspace holds " + s(ext->bytes));
+        chunk = std::min(chunk, cap);
+        tmp   = reinterpret_cast<T*>(ext->buf);
+    } else {
+        // Calls on different streams share the plan's intermediate: a call on another stream than
+        // the last user waits for that use (copies of a C++ fft_unit share one plan).  Not while
+        // the stream is being captured into a graph (the event belongs to no capture).
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        HIPCHK(hipStreamIsCapturing(st, &cs));
+        const bool capturing = cs != hipStreamCaptureStatusNone;
+        if (p->tmp_cap < chunk) {
+            if (capturing)
+                return fail(PCX_ERR_INVALID_ARGUMENT, "the plan's intermediate must be allocated before graph capture "
+                                                      "(run one call of this batch size first)");
+            if (p->tmp) {
+                HIPCHK(hipEventSynchronize(p->tmp_ev)); // the last use, on whichever stream, has ended
+                HIPCHK(hipFree(p->tmp));
+                p->tmp     = nullptr;
+                p->tmp_cap = 0;
+            }
+            hipError_t he = hipMalloc(&p->tmp, sizeof(T) * 2 * n * chunk);
+            if (he != hipSuccess)
+                return fail(PCX_ERR_OUT_OF_MEMORY,
+                            "intermediate allocation of " + s(2 * sizeof(T) * n * chunk) + " bytes failed");
+            p->tmp_cap = chunk;
+        } else if (p->tmp_used && p->tmp_stream != st && !capturing) {
+            HIPCHK(hipStreamWaitEvent(st, p->tmp_ev, 0));
         }
-        hipError_t he = hipMalloc(&p->scratch, sizeof(T) * 2 * n * chunk);
-        if (he != hipSuccess)
-            return fail(PCX_ERR_OUT_OF_MEMORY, "scratch allocation of " + s(2 * sizeof(T) * n * chunk) + " bytes failed");
-        p->scratch_cap = chunk;
+        tmp = reinterpret_cast<T*>(p->tmp);
     }
-    T* const scratch = reinterpret_cast<T*>(p->scratch);
     const FourStepT<T>* shape;
     if constexpr (F64)
         shape = p->shape64;
@@ -782,30 +584,30 @@ int run_batch(pcx_plan_t p, int inverse, int normalized, T* dst, int64_t dst_pac
         ra.twA = tw_row, ra.rows_lg = p->n2_lg, ra.nrows = cb * n2, ra.order = p->order;
         ca.tw_lg = p->lg, ra.tw_lg = p->lg;
         if (!inverse) {
-            // pass A: columns of src -> scratch, times W_N^(n1*k2)
+            // pass A: columns of src -> intermediate, times W_N^(n1*k2)
             ca.src = cs, ca.src_dist = src_dist, ca.src_lp = src_lp, ca.src_size = src_size, ca.src_offs = cso;
-            ca.dst = scratch, ca.dst_dist = 2 * n, ca.dst_lp = p->lg;
+            ca.dst = tmp, ca.dst_dist = 2 * n, ca.dst_lp = p->lg;
             ca.conj_in = 0, ca.conj_out = 0, ca.scale = T(1);
             ca.twl = twl, ca.twh = twh, ca.tw_ls = p->tw_ls;
             ca.nt_load = p->nt_flags & 1, ca.nt_store = (p->nt_flags >> 2) & 1;
             colf(ca, p->grid_col, st);
-            // pass B: rows of scratch -> dst in the plan's order
-            ra.src = scratch, ra.src_dist = 2 * n, ra.src_lp = p->lg, ra.src_size = n;
+            // pass B: rows of the intermediate -> dst in the plan's order
+            ra.src = tmp, ra.src_dist = 2 * n, ra.src_lp = p->lg, ra.src_size = n;
             ra.dst = cd, ra.dst_dist = dst_dist, ra.dst_lp = dst_lp, ra.dst_offs = cdo;
             ra.conj_in = 0, ra.conj_out = 0, ra.scale = T(1);
             ra.twl = nullptr, ra.twh = nullptr;
             ra.nt_load = (p->nt_flags >> 3) & 1, ra.nt_store = (p->nt_flags >> 1) & 1;
             rowf(ra, ra.nrows, p->grid_row, st);
         } else {
-            // pass 1: rows (over k1) of the spectrum -> scratch, times W_N^(n1*k2)
+            // pass 1: rows (over k1) of the spectrum -> intermediate, times W_N^(n1*k2)
             ra.src = cs, ra.src_dist = src_dist, ra.src_lp = src_lp, ra.src_size = n, ra.src_offs = cso;
-            ra.dst = scratch, ra.dst_dist = 2 * n, ra.dst_lp = p->lg;
+            ra.dst = tmp, ra.dst_dist = 2 * n, ra.dst_lp = p->lg;
             ra.conj_in = 1, ra.conj_out = 0, ra.scale = T(1);
             ra.twl = twl, ra.twh = twh, ra.tw_ls = p->tw_ls;
             ra.nt_load = p->nt_flags & 1, ra.nt_store = (p->nt_flags >> 2) & 1;
             rowf(ra, ra.nrows, p->grid_row, st);
-            // pass 2: columns (over k2) of scratch -> dst, natural order
-            ca.src = scratch, ca.src_dist = 2 * n, ca.src_lp = p->lg, ca.src_size = n;
+            // pass 2: columns (over k2) of the intermediate -> dst, natural order
+            ca.src = tmp, ca.src_dist = 2 * n, ca.src_lp = p->lg, ca.src_size = n;
             ca.dst = cd, ca.dst_dist = dst_dist, ca.dst_lp = dst_lp, ca.dst_offs = cdo;
             ca.conj_in = 0, ca.conj_out = 1, ca.scale = scale;
             ca.twl = nullptr, ca.twh = nullptr;
@@ -814,12 +616,92 @@ int run_batch(pcx_plan_t p, int inverse, int normalized, T* dst, int64_t dst_pac
         }
         HIPCHK(hipGetLastError());
     }
+    if (!ext) {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        HIPCHK(hipStreamIsCapturing(st, &cs));
+        if (cs == hipStreamCaptureStatusNone) {
+            HIPCHK(hipEventRecord(p->tmp_ev, st));
+            p->tmp_stream = st;
+            p->tmp_used   = true;
+        }
+    }
     return PCX_SUCCESS;
 }
 
 int64_t footprint(int64_t n, int64_t pack) { return ((n + pack - 1) / pack) * pack * 2; }
 
 } // namespace
+
+// ---------------------------------------------------------------- tuning of four-step plans
+extern "C" int pcx_plan_set_shape(pcx_plan_t p, int n2_log2, int cols, int rows) {
+    if (!p)
+        return fail(PCX_ERR_INVALID_ARGUMENT, "plan is null");
+    if (!p->two_pass)
+        return fail(PCX_ERR_UNSUPPORTED, "a single-pass plan (fft_size " + s(p->n) + ") has no four-step shape");
+    if (p->precision != PCX_PREC_FP32)
+        return fail(PCX_ERR_UNSUPPORTED, "split-precision plans keep the default four-step shape");
+    if (n2_log2 < 0 || cols < 0 || rows < 0)
+        return fail(PCX_ERR_INVALID_ARGUMENT, "shape selectors are negative");
+    std::lock_guard<std::mutex> lock(p->mu);
+    if (p->f64) {
+        const FourStepT<double>* sh = find_shape64(p->lg, n2_log2, rows, cols);
+        if (!sh)
+            return fail(PCX_ERR_UNSUPPORTED, "no double four-step shape of fft_size " + s(p->n) + " matches");
+        return plan_apply_shape64(p, sh);
+    }
+    const FourStep* sh = find_shape(p->lg, n2_log2, rows, cols);
+    if (!sh)
+        return fail(PCX_ERR_UNSUPPORTED, "no four-step shape of fft_size " + s(p->n) + " matches");
+    return plan_apply_shape(p, sh);
+}
+
+extern "C" int pcx_plan_shape(pcx_plan_t p, int* n2_log2, int* cols, int* rows) {
+    if (!p)
+        return fail(PCX_ERR_INVALID_ARGUMENT, "plan is null");
+    int a = 0, c = 0, g = 0;
+    if (p->two_pass) {
+        a = p->n2_lg;
+        c = p->f64 ? p->shape64->c : p->shape->c;
+        g = p->f64 ? p->shape64->g : p->shape->g;
+    }
+    if (n2_log2)
+        *n2_log2 = a;
+    if (cols)
+        *cols = c;
+    if (rows)
+        *rows = g;
+    return PCX_SUCCESS;
+}
+
+extern "C" int pcx_plan_set_grid(pcx_plan_t p, int64_t grid_col, int64_t grid_row) {
+    if (!p)
+        return fail(PCX_ERR_INVALID_ARGUMENT, "plan is null");
+    if (grid_col < 0 || grid_row < 0 || grid_col > ((int64_t)1 << 31) || grid_row > ((int64_t)1 << 31))
+        return fail(PCX_ERR_INVALID_ARGUMENT, "grid caps are not in [0, 2^31]");
+    std::lock_guard<std::mutex> lock(p->mu);
+    p->grid_col = grid_col, p->grid_row = grid_row;
+    return PCX_SUCCESS;
+}
+
+extern "C" int pcx_plan_set_chunk(pcx_plan_t p, int64_t bytes) {
+    if (!p)
+        return fail(PCX_ERR_INVALID_ARGUMENT, "plan is null");
+    if (bytes < 0)
+        return fail(PCX_ERR_INVALID_ARGUMENT, "chunk size (which is " + s(bytes) + ") is negative");
+    std::lock_guard<std::mutex> lock(p->mu);
+    const int64_t per = p->n * (p->f64 ? 16 : 8);
+    p->chunk          = std::max<int64_t>(1, (bytes ? bytes : kChunkBytes) / per);
+    return PCX_SUCCESS;
+}
+
+extern "C" int pcx_plan_set_one_pass(pcx_plan_t p, int enable) {
+    if (!p)
+        return fail(PCX_ERR_INVALID_ARGUMENT, "plan is null");
+    if (!p->tw_dif)
+        return enable ? fail(PCX_ERR_UNSUPPORTED, "no one-pass kernel for fft_size " + s(p->n)) : PCX_SUCCESS;
+    p->use_dif = enable != 0;
+    return PCX_SUCCESS;
+}
 
 extern "C" int pcx_fft(pcx_plan_t p, float* dst, int64_t dst_pack, const float* src, int64_t src_pack,
                        int64_t src_size, void* stream) {
@@ -893,15 +775,65 @@ extern "C" int pcx_fft_batch_f64(pcx_plan_t p, int inverse, int normalized, doub
 
 // ---------------------------------------------------------------- cross-correlation
 struct pcx_fxcorr_s {
-    int64_t n = 0, m = 0;
-    int     lg = 0;
-    int     precision = PCX_PREC_FP32;
-    float2* tw = nullptr;
-    float*  kspec = nullptr; // FFT(g)/N, split re[N] im[N]
-    float*  halo  = nullptr;
-    int64_t halo_cap = 0;
-    std::mutex mu;
+    int64_t    n = 0, m = 0;
+    int        lg        = 0;
+    int        precision = PCX_PREC_FP32;
+    float2*    tw        = nullptr; // W_N of the fused kernel (N <= 16384)
+    float*     kspec     = nullptr; // FFT(g)/N (fused kernel) or FFT(g) (block path), split re[N] im[N]
+    pcx_plan_t plan      = nullptr; // block path (N > 16384): the batched four-step FFT of size N
+    // the unit's own buffers: the in-place overlap of every block (fused kernel) or the blocks in
+    // flight (block path); calls on another stream than the last user wait for that use (ev)
+    float*      own      = nullptr;
+    int64_t     own_cap  = 0; // floats
+    hipEvent_t  ev       = nullptr;
+    hipStream_t ev_stream = nullptr;
+    bool        ev_used  = false;
+    std::mutex  mu;
 };
+
+namespace {
+
+// Block path: at most this many bytes of blocks in flight in the unit's own buffer.
+constexpr int64_t kXcBlockBytes = (int64_t)512 << 20;
+
+// The unit's own buffer with at least `floats` floats, ordered after its last use on any stream.
+int xc_own(pcx_fxcorr_t x, int64_t floats, hipStream_t st) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    HIPCHK(hipStreamIsCapturing(st, &cs));
+    const bool capturing = cs != hipStreamCaptureStatusNone;
+    if (x->own_cap < floats) {
+        if (capturing)
+            return fail(PCX_ERR_INVALID_ARGUMENT, "the correlation's buffers must be allocated before graph capture "
+                                                  "(run one call of this size first)");
+        if (x->own) {
+            if (x->ev_used)
+                HIPCHK(hipEventSynchronize(x->ev));
+            HIPCHK(hipFree(x->own));
+            x->own     = nullptr;
+            x->own_cap = 0;
+        }
+        if (hipMalloc(&x->own, sizeof(float) * floats) != hipSuccess)
+            return fail(PCX_ERR_OUT_OF_MEMORY, "correlation buffer allocation of " + s(4 * floats) + " bytes failed");
+        x->own_cap = floats;
+    } else if (x->ev_used && x->ev_stream != st && !capturing) {
+        HIPCHK(hipStreamWaitEvent(st, x->ev, 0));
+    }
+    return PCX_SUCCESS;
+}
+int xc_own_done(pcx_fxcorr_t x, hipStream_t st) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    HIPCHK(hipStreamIsCapturing(st, &cs));
+    if (cs == hipStreamCaptureStatusNone) {
+        HIPCHK(hipEventRecord(x->ev, st));
+        x->ev_stream = st;
+        x->ev_used   = true;
+    }
+    return PCX_SUCCESS;
+}
+
+int64_t grid_of(int64_t work) { return std::max<int64_t>(1, std::min<int64_t>((work + 255) / 256, (int64_t)1 << 20)); }
+
+} // namespace
 
 extern "C" int pcx_fxcorr_create(const float* g, int64_t g_pack, int64_t g_size, int64_t fft_size, void* stream,
                                  pcx_fxcorr_t* out) {
@@ -918,30 +850,47 @@ extern "C" int pcx_fxcorr_create(const float* g, int64_t g_pack, int64_t g_size,
         return fail(PCX_ERR_SIZE_MISMATCH,
                     "source size (which is " + s(g_size) + ") is bigger than fft size (which is " + s(fft_size) + ")");
     int lg = ilog2(fft_size);
-    if (lg > kMaxSinglePass)
-        return fail(PCX_ERR_UNSUPPORTED, "fxcorr fft_size (which is " + s(fft_size) + ") exceeds 16384");
-    hipStream_t st = (hipStream_t)stream;
-    auto*       x  = new pcx_fxcorr_s;
+    if (lg > kMaxLog2)
+        return fail(PCX_ERR_UNSUPPORTED, "fxcorr fft_size (which is " + s(fft_size) + ") exceeds 2^24");
+    hipStream_t st    = (hipStream_t)stream;
+    hipStream_t setup = nullptr;
+    HIPCHK(setup_stream(&setup));
+    auto* x = new pcx_fxcorr_s;
     x->n = fft_size, x->m = g_size, x->lg = lg;
-    hipError_t he = hipMalloc(&x->tw, sizeof(float2) * fft_size);
+    hipError_t he = hipMalloc(&x->kspec, sizeof(float) * 2 * fft_size);
     if (he == hipSuccess)
-        he = hipMalloc(&x->kspec, sizeof(float) * 2 * fft_size);
-    if (he == hipSuccess)
-        he = gen_twiddles(x->tw, fft_size, 1, fft_size);
-    if (he == hipSuccess)
-        he = hipDeviceSynchronize();
+        he = hipEventCreateWithFlags(&x->ev, hipEventDisableTiming);
+    if (he == hipSuccess && lg <= kMaxSinglePass) {
+        he = hipMalloc(&x->tw, sizeof(float2) * fft_size);
+        if (he == hipSuccess)
+            he = gen_twiddles(x->tw, fft_size, 1, fft_size, setup);
+        if (he == hipSuccess)
+            he = hipStreamSynchronize(setup);
+    }
     if (he != hipSuccess) {
         pcx_fxcorr_destroy(x);
         return fail(PCX_ERR_HIP, std::string("fxcorr setup: ") + hipGetErrorString(he));
     }
-    // kernel spectrum: FFT of g zero-extended to N, scaled by 1/N (fxcorr.hpp:67-68)
-    RowArgs a{};
-    a.src = g, a.dst = x->kspec, a.src_dist = 0, a.dst_dist = 0, a.src_size = g_size, a.nrows = 1;
-    a.src_lp = ilog2(g_pack), a.dst_lp = lg, a.rows_lg = 0;
-    a.order = 0, a.conj_in = 0, a.conj_out = 0, a.scale = (float)(1.0 / (double)fft_size);
-    a.twA = x->tw;
-    kSingle[lg - 1][0](a, 1, 0, st);
-    he = hipGetLastError();
+    if (lg <= kMaxSinglePass) {
+        // kernel spectrum: FFT of g zero-extended to N, scaled by 1/N (fxcorr.hpp:67-68)
+        RowArgs a{};
+        a.src = g, a.dst = x->kspec, a.src_dist = 0, a.dst_dist = 0, a.src_size = g_size, a.nrows = 1;
+        a.src_lp = ilog2(g_pack), a.dst_lp = lg, a.rows_lg = 0;
+        a.order = 0, a.conj_in = 0, a.conj_out = 0, a.scale = (float)(1.0 / (double)fft_size);
+        a.twA = x->tw;
+        single_tab()[lg - 1][0](a, 1, 0, st);
+        he = hipGetLastError();
+    } else {
+        // block path: FFT(g) through the four-step plan; the 1/N is applied with the product
+        if (int e = pcx_plan_create(fft_size, 2048, PCX_ORDER_NORMAL, &x->plan)) {
+            pcx_fxcorr_destroy(x);
+            return e;
+        }
+        if (int e = run_batch(x->plan, 0, 0, x->kspec, fft_size, 0, g, g_pack, 0, g_size, 1, st)) {
+            pcx_fxcorr_destroy(x);
+            return e;
+        }
+    }
     if (he == hipSuccess)
         he = hipStreamSynchronize(st);
     if (he != hipSuccess) {
@@ -955,9 +904,14 @@ extern "C" int pcx_fxcorr_create(const float* g, int64_t g_pack, int64_t g_size,
 extern "C" int pcx_fxcorr_destroy(pcx_fxcorr_t x) {
     if (!x)
         return PCX_SUCCESS;
+    if (x->ev && x->ev_used)
+        (void)hipEventSynchronize(x->ev);
     (void)hipFree(x->tw);
     (void)hipFree(x->kspec);
-    (void)hipFree(x->halo);
+    (void)hipFree(x->own);
+    if (x->ev)
+        (void)hipEventDestroy(x->ev);
+    pcx_plan_destroy(x->plan);
     delete x;
     return PCX_SUCCESS;
 }
@@ -972,19 +926,23 @@ extern "C" int64_t pcx_fxcorr_step(pcx_fxcorr_t x, int64_t src_pack) {
     return (x->n - overlap) / src_pack * src_pack;
 }
 
+extern "C" int64_t pcx_fxcorr_workspace_min(pcx_fxcorr_t x) { return x ? 8 * x->n : -1; }
+
 extern "C" int pcx_fxcorr_set_precision(pcx_fxcorr_t x, int precision) {
     if (!x)
         return fail(PCX_ERR_INVALID_ARGUMENT, "fxcorr handle is null");
     if (precision != PCX_PREC_FP32 && precision != PCX_PREC_BF16X3 && precision != PCX_PREC_FP16X2)
         return fail(PCX_ERR_UNSUPPORTED, "precision " + s(precision) + " is not implemented");
-    if (precision != PCX_PREC_FP32 && x->lg < 4)
-        return fail(PCX_ERR_UNSUPPORTED, "split-precision correlation needs fft_size >= 16");
+    if (precision != PCX_PREC_FP32 && (x->lg < 4 || x->lg > kMaxSinglePass))
+        return fail(PCX_ERR_UNSUPPORTED, "split-precision correlation covers fft_size 16 .. 16384");
     x->precision = precision;
     return PCX_SUCCESS;
 }
 
-extern "C" int pcx_fxcorr_exec_batch(pcx_fxcorr_t x, float* dst, int64_t dst_pack, int64_t dst_dist, const float* src,
-                                     int64_t src_pack, int64_t src_dist, int64_t len, int64_t channels, void* stream) {
+namespace {
+
+int fxcorr_run(pcx_fxcorr_t x, float* dst, int64_t dst_pack, int64_t dst_dist, const float* src, int64_t src_pack,
+               int64_t src_dist, int64_t len, int64_t channels, const ExtTmp* ws, hipStream_t st) {
     if (!x)
         return fail(PCX_ERR_INVALID_ARGUMENT, "fxcorr handle is null");
     if (!dst || !src)
@@ -1005,47 +963,108 @@ extern "C" int pcx_fxcorr_exec_batch(pcx_fxcorr_t x, float* dst, int64_t dst_pac
     if (inplace && (dst_pack != src_pack || dst_dist != src_dist))
         return fail(PCX_ERR_INPLACE_REPACK, "cannot perform in-place repack from source pack size (which is " +
                                                 s(src_pack) + ") to destination pack size (which is " + s(dst_pack) + ")");
+    if (ws && (!ws->buf || ws->bytes < 8 * x->n))
+        return fail(PCX_ERR_INVALID_ARGUMENT, "workspace (which is " + s(ws ? ws->bytes : 0) +
+                                                  " bytes) is smaller than one fft_size vector (which is " + s(8 * x->n) +
+                                                  " bytes)");
     if (len == 0 || channels == 0)
         return PCX_SUCCESS;
-    hipStream_t st      = (hipStream_t)stream;
-    int64_t     nblocks = (len + step - 1) / step;
-    XcArgs      a{};
-    a.src = src, a.dst = dst, a.kre = x->kspec, a.kim = x->kspec + x->n, a.twA = x->tw;
-    a.len = len, a.step = step, a.nblocks = nblocks, a.hlen = x->n - step;
+    const int64_t nblocks = (len + step - 1) / step;
+    const int64_t total   = nblocks * channels;
+    const int64_t n       = x->n;
+    std::lock_guard<std::mutex> lock(x->mu);
+
+    if (x->lg > kMaxSinglePass) {
+        // ---- block path: gather nb blocks, forward FFT, times conj(FFT(g)) / N, unnormalised
+        // inverse, keep `step` outputs per block.  Blocks run in increasing order, and a block's
+        // outputs lie below every later block's samples, so in place needs no saved overlap.
+        float*  buf = nullptr;
+        int64_t nb  = 0;
+        if (ws) {
+            buf = reinterpret_cast<float*>(ws->buf);
+            nb  = std::min(total, ws->bytes / (8 * n));
+        } else {
+            nb = std::min(total, std::max<int64_t>(1, kXcBlockBytes / (8 * n)));
+            if (int e = xc_own(x, 2 * n * nb, st))
+                return e;
+            buf = x->own;
+        }
+        XcBlockArgs a{};
+        a.src = src, a.dst = dst, a.buf = buf, a.kre = x->kspec, a.kim = x->kspec + n;
+        a.len = len, a.step = step, a.nblocks = nblocks, a.src_dist = src_dist, a.dst_dist = dst_dist;
+        a.src_lp = ilog2(src_pack), a.dst_lp = ilog2(dst_pack), a.lg = x->lg;
+        a.scale = (float)(1.0 / (double)n);
+        for (a.bg0 = 0; a.bg0 < total; a.bg0 += nb) {
+            a.nb = std::min(nb, total - a.bg0);
+            hipLaunchKernelGGL(k_xc_gather, dim3((unsigned)grid_of(a.nb * n)), dim3(256), 0, st, a);
+            HIPCHK(hipGetLastError());
+            if (int e = run_batch(x->plan, 0, 0, buf, n, 2 * n, (const float*)buf, n, 2 * n, n, a.nb, st))
+                return e;
+            hipLaunchKernelGGL(k_xc_mul, dim3((unsigned)grid_of(a.nb * n)), dim3(256), 0, st, a);
+            HIPCHK(hipGetLastError());
+            if (int e = run_batch(x->plan, 1, 0, buf, n, 2 * n, (const float*)buf, n, 2 * n, n, a.nb, st))
+                return e;
+            hipLaunchKernelGGL(k_xc_scatter, dim3((unsigned)grid_of(a.nb * step)), dim3(256), 0, st, a);
+            HIPCHK(hipGetLastError());
+        }
+        return ws ? PCX_SUCCESS : xc_own_done(x, st);
+    }
+
+    // ---- fused kernel: one workgroup per block
+    XcArgs a{};
+    a.src = src, a.dst = dst, a.kre = x->kspec, a.kim = x->kspec + n, a.twA = x->tw;
+    a.len = len, a.step = step, a.nblocks = nblocks, a.hlen = n - step;
     a.src_lp = ilog2(src_pack), a.dst_lp = ilog2(dst_pack);
     a.src_dist = src_dist, a.dst_dist = dst_dist, a.channels = channels;
-    std::lock_guard<std::mutex> lock(x->mu);
-    if (inplace && nblocks > 1 && a.hlen > 0) {
-        // blocks overwrite the overlap that the next block still reads: save it first
-        int64_t need = nblocks * channels * a.hlen;
-        if (x->halo_cap < need) {
-            if (x->halo) {
-                HIPCHK(hipStreamSynchronize(st));
-                HIPCHK(hipFree(x->halo));
-                x->halo = nullptr;
-            }
-            hipError_t he = hipMalloc(&x->halo, sizeof(float) * 2 * need);
-            if (he != hipSuccess)
-                return fail(PCX_ERR_OUT_OF_MEMORY, "halo allocation failed");
-            x->halo_cap = need;
-        }
-        a.hre = x->halo, a.him = x->halo + need;
-        int64_t nb = std::min<int64_t>((need + 255) / 256, (int64_t)1 << 20);
-        hipLaunchKernelGGL(k_fxcorr_halo, dim3((unsigned)nb), dim3(256), 0, st, src, a.src_lp, src_dist, len, step, a.hlen,
-                           nblocks, channels, x->halo, x->halo + need);
+    const XcLaunch launch = x->precision != PCX_PREC_FP32 ? xc_mfma_tab()[x->precision - 1][x->lg - 4] : xc_tab()[x->lg - 1];
+    if (!(inplace && nblocks > 1 && a.hlen > 0)) {
+        a.blk0 = 0, a.blk1 = total;
+        launch(a, st);
+        HIPCHK(hipGetLastError());
+        return PCX_SUCCESS;
+    }
+    // In place: blocks overwrite the overlap that the next block still reads, so it is saved
+    // first.  With a workspace of W bytes the blocks run in groups of W / (8 * (N - step)), one
+    // group after another (a group only overwrites samples of its own blocks).
+    const int64_t group = ws ? std::min(total, ws->bytes / (8 * a.hlen)) : total;
+    float*        hbuf  = nullptr;
+    if (ws) {
+        hbuf = reinterpret_cast<float*>(ws->buf);
+    } else {
+        if (int e = xc_own(x, 2 * total * a.hlen, st))
+            return e;
+        hbuf = x->own;
+    }
+    for (int64_t g0 = 0; g0 < total; g0 += group) {
+        const int64_t cnt = std::min(group, total - g0);
+        a.hre = hbuf, a.him = hbuf + cnt * a.hlen;
+        a.blk0 = g0, a.blk1 = g0 + cnt, a.hblk0 = g0;
+        hipLaunchKernelGGL(k_fxcorr_halo, dim3((unsigned)grid_of(cnt * a.hlen)), dim3(256), 0, st, src, a.src_lp,
+                           src_dist, len, step, a.hlen, nblocks, g0, cnt, hbuf, hbuf + cnt * a.hlen);
+        HIPCHK(hipGetLastError());
+        launch(a, st);
         HIPCHK(hipGetLastError());
     }
-    if (x->precision != PCX_PREC_FP32)
-        kXcMfma[x->precision - 1][x->lg - 4](a, st);
-    else
-        kXc[x->lg - 1](a, st);
-    HIPCHK(hipGetLastError());
-    return PCX_SUCCESS;
+    return ws ? PCX_SUCCESS : xc_own_done(x, st);
+}
+
+} // namespace
+
+extern "C" int pcx_fxcorr_exec_batch(pcx_fxcorr_t x, float* dst, int64_t dst_pack, int64_t dst_dist, const float* src,
+                                     int64_t src_pack, int64_t src_dist, int64_t len, int64_t channels, void* stream) {
+    return fxcorr_run(x, dst, dst_pack, dst_dist, src, src_pack, src_dist, len, channels, nullptr, (hipStream_t)stream);
 }
 
 extern "C" int pcx_fxcorr_exec(pcx_fxcorr_t x, float* dst, int64_t dst_pack, const float* src, int64_t src_pack,
                                int64_t len, void* stream) {
     return pcx_fxcorr_exec_batch(x, dst, dst_pack, 0, src, src_pack, 0, len, 1, stream);
+}
+
+extern "C" int pcx_fxcorr_exec_batch_ws(pcx_fxcorr_t x, float* dst, int64_t dst_pack, int64_t dst_dist,
+                                        const float* src, int64_t src_pack, int64_t src_dist, int64_t len,
+                                        int64_t channels, void* workspace, int64_t workspace_bytes, void* stream) {
+    const ExtTmp ws{workspace, workspace_bytes};
+    return fxcorr_run(x, dst, dst_pack, dst_dist, src, src_pack, src_dist, len, channels, &ws, (hipStream_t)stream);
 }
 
 // ---------------------------------------------------------------- memory and utilities
@@ -1146,12 +1165,12 @@ namespace {
 
 int queue_run(pcx_queue_t q, float* dst, const float* src, hipStream_t st) {
     if (q->nitems > 0) {
-        hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, st, q->counter);
+        launch_reset_counter(q->counter, st);
         QueueArgs a{};
         a.src = src, a.dst = dst, a.src_offs = q->q_src_offs, a.dst_offs = q->q_dst_offs;
         a.items = q->items, a.nitems = q->nitems, a.normalized = q->normalized, a.counter = q->counter, a.tw = q->tw;
         int grid = q->grid < q->nitems ? q->grid : q->nitems;
-        hipLaunchKernelGGL(k_queue, dim3(grid), dim3(kQueueThreads), 0, st, a);
+        launch_queue(a, grid, st);
         HIPCHK(hipGetLastError());
     }
     for (auto& g : q->groups) {
@@ -1210,7 +1229,8 @@ extern "C" int pcx_queue_create(const pcx_job* jobs, int64_t njobs, int normaliz
     }
     std::vector<QItem>   items;
     std::vector<int64_t> qs, qd;
-    hipError_t           he = hipSuccess;
+    hipStream_t          setup = nullptr;
+    hipError_t           he    = setup_stream(&setup);
     // persistent items, largest size first (longest-processing-time order)
     for (int lg = kQueueMaxLog2; lg >= 1; --lg) {
         int per = kQueueThreads / (lg >= 4 ? (1 << lg) / 16 : 1);
@@ -1244,7 +1264,7 @@ extern "C" int pcx_queue_create(const pcx_job* jobs, int64_t njobs, int normaliz
         if (he == hipSuccess)
             he = hipMemcpy(q->q_dst_offs, qd.data(), sizeof(int64_t) * qd.size(), hipMemcpyHostToDevice);
         for (int lg = 1; lg <= kQueueMaxLog2 && he == hipSuccess; ++lg)
-            he = gen_twiddles(q->tw + ((1 << lg) - 2), (int64_t)1 << lg, 1, (int64_t)1 << lg);
+            he = gen_twiddles(q->tw + ((1 << lg) - 2), (int64_t)1 << lg, 1, (int64_t)1 << lg, setup);
     }
     // grouped launches for the larger sizes, largest first
     for (int lg = kMaxLog2; lg > kQueueMaxLog2 && he == hipSuccess; --lg) {
@@ -1275,7 +1295,7 @@ extern "C" int pcx_queue_create(const pcx_job* jobs, int64_t njobs, int normaliz
         }
     }
     if (he == hipSuccess)
-        he = hipDeviceSynchronize();
+        he = hipStreamSynchronize(setup);
     if (he != hipSuccess) {
         queue_free(q);
         return fail(PCX_ERR_HIP, std::string("queue setup: ") + hipGetErrorString(he));
@@ -1306,10 +1326,10 @@ extern "C" int pcx_queue_capture(pcx_queue_t q, float* dst, const float* src, vo
     hipStream_t st = (hipStream_t)stream;
     if (!st)
         return fail(PCX_ERR_INVALID_ARGUMENT, "graph capture needs a non-default stream");
-    // make sure every group's scratch exists before capture (allocation is not capturable)
+    // make sure every group's intermediate exists before capture (allocation is not capturable)
     for (auto& g : q->groups) {
         int64_t n = (int64_t)1 << g.lg;
-        if (g.plan->two_pass && g.plan->scratch_cap < std::min(g.count, g.plan->chunk)) {
+        if (g.plan->two_pass && g.plan->tmp_cap < std::min(g.count, g.plan->chunk)) {
             if (int e = run_batch(g.plan, g.inv, q->normalized, dst, n, 2 * n, src, n, 2 * n, n, g.count, st,
                                   g.src_offs, g.dst_offs))
                 return e;
@@ -1395,7 +1415,20 @@ extern "C" int pcx_queue_autotune(pcx_queue_t q, float* dst, const float* src, v
         return fail(PCX_ERR_INVALID_ARGUMENT, "queue is null");
     if (!dst || !src)
         return fail(PCX_ERR_INVALID_ARGUMENT, "data pointer is null");
+    if ((const float*)dst == src)
+        return fail(PCX_ERR_INVALID_ARGUMENT, "autotuning overwrites dst many times: it must not alias src");
     hipStream_t st = (hipStream_t)stream;
+    // a captured graph holds the kernels, grids and twiddle tables of the shapes it was captured
+    // with; the tuning below may change all of them, so the graph is dropped (capture again after)
+    HIPCHK(hipDeviceSynchronize());
+    if (q->exec) {
+        HIPCHK(hipGraphExecDestroy(q->exec));
+        q->exec = nullptr;
+    }
+    if (q->graph) {
+        HIPCHK(hipGraphDestroy(q->graph));
+        q->graph = nullptr;
+    }
     if (cache_path) { // reuse earlier decisions
         if (FILE* f = std::fopen(cache_path, "r")) {
             char key[32];
@@ -1447,9 +1480,13 @@ extern "C" int pcx_queue_autotune(pcx_queue_t q, float* dst, const float* src, v
         const FourStep* best = g.plan->shape;
         int64_t         bgc = g.plan->grid_col, bgr = g.plan->grid_row;
         float           bms = 1e30f;
-        for (const auto& sh : kShapes) {
-            if (sh.lg != g.lg)
-                continue;
+        std::vector<const FourStep*> cands;
+        for (const ShapeList<float>& l : {shapes_default(), shapes_alt()})
+            for (int i = 0; i < l.count; ++i)
+                if (l.data[i].lg == g.lg)
+                    cands.push_back(&l.data[i]);
+        for (const FourStep* shp : cands) {
+            const FourStep& sh = *shp;
             if (int err = plan_apply_shape(g.plan, &sh))
                 return err;
             for (int64_t gc : {(int64_t)0, (int64_t)1024, (int64_t)2048}) {
@@ -1529,7 +1566,7 @@ int vexpr_eval(const S* srcs, int32_t nsrc, const I* prog, int32_t nprog, T* dst
     int64_t nwg = (n + 255) / 256;
     if (nwg > 16384)
         nwg = 16384;
-    hipLaunchKernelGGL(k_vexpr<T>, dim3((unsigned)nwg), dim3(256), 0, (hipStream_t)stream, a);
+    launch_vexpr(a, nwg, (hipStream_t)stream);
     HIPCHK(hipGetLastError());
     return PCX_SUCCESS;
 }
@@ -1560,30 +1597,15 @@ extern "C" int pcx_repack_f64(double* dst, int64_t dst_pack, const double* src, 
 
 // ---------------------------------------------------------------- FFT across vectors
 namespace {
-using AcrossLaunch = void (*)(const AcrossArgs&, int64_t, hipStream_t);
-template<int LA>
-void launch_across(const AcrossArgs& a, int64_t grid, hipStream_t st) {
-    constexpr int C = LA <= 8 ? 64 : (LA == 9 ? 32 : 16);
-    int64_t       n = (a.len + C - 1) / C;
-    if (n > grid)
-        n = grid;
-    hipLaunchKernelGGL((k_across<LA, C>), dim3((unsigned)fit_grid(grid8(n), C * Plan<LA>::TA)), dim3(C * Plan<LA>::TA), 0,
-                       st, a);
-}
-template<int... L>
-constexpr auto across_table(std::integer_sequence<int, L...>) {
-    return std::array<AcrossLaunch, sizeof...(L)>{&launch_across<L + 1>...};
-}
-constexpr int kMaxAcross = 10;
-const auto    kAcross    = across_table(std::make_integer_sequence<int, kMaxAcross>{});
-} // namespace
-
-extern "C" int pcx_fft_across(pcx_plan_t p, int inverse, int normalized, float* const* dst_vecs, int64_t dst_pack,
-                              const float* const* src_vecs, int64_t src_pack, int64_t length, void* stream) {
+template<class T, class Tab>
+int fft_across(pcx_plan_t p, int inverse, int normalized, T* const* dst_vecs, int64_t dst_pack, const T* const* src_vecs,
+               int64_t src_pack, int64_t length, hipStream_t st, const Tab& tab) {
+    constexpr bool F64 = std::is_same_v<T, double>;
     if (!p)
         return fail(PCX_ERR_INVALID_ARGUMENT, "plan is null");
-    if (p->f64)
-        return fail(PCX_ERR_UNSUPPORTED, "transforms across vectors are built for float plans");
+    if (p->f64 != F64)
+        return fail(PCX_ERR_INVALID_ARGUMENT, p->f64 ? "the plan is for double data (use pcx_fft_across_f64)"
+                                                     : "the plan is for float data (use pcx_fft_across)");
     if (!dst_vecs || !src_vecs)
         return fail(PCX_ERR_INVALID_ARGUMENT, "vector table is null");
     if (length < 0)
@@ -1594,17 +1616,35 @@ extern "C" int pcx_fft_across(pcx_plan_t p, int inverse, int normalized, float* 
         return e;
     if (p->order != PCX_ORDER_NORMAL)
         return fail(PCX_ERR_UNSUPPORTED, "transforms across vectors are built for the normal order");
-    if (p->lg > kMaxAcross)
-        return fail(PCX_ERR_UNSUPPORTED, "transforms across vectors cover at most 1024 vectors (fft_size " + s(p->n) + ")");
+    if (p->lg > (int)tab.size())
+        return fail(PCX_ERR_UNSUPPORTED, "transforms across vectors cover at most " + s((int64_t)1 << tab.size()) +
+                                             " vectors (fft_size " + s(p->n) + ")");
     if (length == 0)
         return PCX_SUCCESS;
-    AcrossArgs a{};
+    AcrossArgsT<T> a{};
     a.src = src_vecs, a.dst = dst_vecs, a.len = length;
     a.src_lp = ilog2(src_pack), a.dst_lp = ilog2(dst_pack);
     a.conj_in = inverse, a.conj_out = inverse;
-    a.scale   = (inverse && normalized) ? (float)(1.0 / (double)p->n) : 1.0f;
-    a.twA     = p->tw_row;
-    kAcross[p->lg - 1](a, 2048, (hipStream_t)stream);
+    a.scale   = (inverse && normalized) ? (T)(1.0 / (double)p->n) : T(1);
+    if constexpr (F64)
+        a.twA = p->tw64_row;
+    else
+        a.twA = p->tw_row;
+    tab[p->lg - 1](a, 2048, st);
     HIPCHK(hipGetLastError());
     return PCX_SUCCESS;
+}
+} // namespace
+
+extern "C" int pcx_fft_across(pcx_plan_t p, int inverse, int normalized, float* const* dst_vecs, int64_t dst_pack,
+                              const float* const* src_vecs, int64_t src_pack, int64_t length, void* stream) {
+    return fft_across(p, inverse, normalized, dst_vecs, dst_pack, src_vecs, src_pack, length, (hipStream_t)stream,
+                      across_tab());
+}
+
+extern "C" int pcx_fft_across_f64(pcx_plan_t p, int inverse, int normalized, double* const* dst_vecs,
+                                  int64_t dst_pack, const double* const* src_vecs, int64_t src_pack, int64_t length,
+                                  void* stream) {
+    return fft_across(p, inverse, normalized, dst_vecs, dst_pack, src_vecs, src_pack, length, (hipStream_t)stream,
+                      across64_tab());
 }

@@ -221,6 +221,28 @@ class fft_unit:
     def size(self) -> int:
         return int(lib().pcx_plan_size(self._h))
 
+    # ---------------------------------------------------------------- launch-shape tuning
+    def set_shape(self, n2_log2: int = 0, cols: int = 0, rows: int = 0) -> None:
+        """Four-step split 2^n2_log2 column length, `cols` columns per column tile, `rows` rows per
+        row workgroup (0 = any; pcx_plan_set_shape)."""
+        check(lib().pcx_plan_set_shape(self._h, int(n2_log2), int(cols), int(rows)))
+
+    def shape(self):
+        import ctypes as C
+
+        a, c, g = C.c_int(), C.c_int(), C.c_int()
+        check(lib().pcx_plan_shape(self._h, C.byref(a), C.byref(c), C.byref(g)))
+        return a.value, c.value, g.value
+
+    def set_grid(self, grid_col: int, grid_row: int) -> None:
+        check(lib().pcx_plan_set_grid(self._h, int(grid_col), int(grid_row)))
+
+    def set_chunk(self, nbytes: int) -> None:
+        check(lib().pcx_plan_set_chunk(self._h, int(nbytes)))
+
+    def set_one_pass(self, enable: bool) -> None:
+        check(lib().pcx_plan_set_one_pass(self._h, int(bool(enable))))
+
     # ---------------------------------------------------------------- raw pointer forms
     def fft_raw(self, dst: int, src: int | None = None, dst_pack: int = 1, src_pack: int | None = None,
                 src_size: int | None = None, stream=None) -> None:
@@ -269,10 +291,25 @@ class fft_unit:
             raise PcxError(_lib.ERR_SIZE_MISMATCH, "vectors differ in size")
         if len({v.pack for v in src_vecs}) != 1 or len({v.pack for v in dst_vecs}) != 1:
             raise PcxError(_lib.ERR_INVALID_PACK, "vectors of one range share a pack size")
+        spk, dpk = src_vecs[0].pack, dst_vecs[0].pack
+        if spk != dpk:
+            # the kernel reads element k of every source while other workgroups write element k of
+            # the destinations at other packed offsets: no destination may overlap a source
+            # (fft.hpp:783 refuses an in-place repack)
+            esz = 8 if self.dtype == "float64" else 4
+            s0 = np.array([v.data_ptr() for v in src_vecs], np.int64)
+            d0 = np.array([v.data_ptr() for v in dst_vecs], np.int64)
+            s1 = s0 + esz * real_size(length, spk)
+            d1 = d0 + esz * real_size(length, dpk)
+            if np.any((d0[:, None] < s1[None, :]) & (s0[None, :] < d1[:, None])):
+                raise PcxError(_lib.ERR_INPLACE_REPACK,
+                               f"cannot perform in-place repack from source pack size (which is {spk}) to "
+                               f"destination pack size (which is {dpk})")
         sp = torch.tensor([v.data_ptr() for v in src_vecs], dtype=torch.int64, device="cuda")
         dp = torch.tensor([v.data_ptr() for v in dst_vecs], dtype=torch.int64, device="cuda")
-        check(lib().pcx_fft_across(self._h, int(inverse), int(normalized), dp.data_ptr(), dst_vecs[0].pack,
-                                   sp.data_ptr(), src_vecs[0].pack, length, _stream_ptr(stream)))
+        fn = lib().pcx_fft_across_f64 if self.dtype == "float64" else lib().pcx_fft_across
+        check(fn(self._h, int(inverse), int(normalized), dp.data_ptr(), dst_vecs[0].pack, sp.data_ptr(),
+                 src_vecs[0].pack, length, _stream_ptr(stream)))
         self._across_tables = (sp, dp)  # kept alive until the next call (asynchronous launch)
 
     def _storage_call(self, dst: md_storage, src: md_storage | None, inverse: bool, normalized: bool,
@@ -378,10 +415,20 @@ class fxcorr_unit:
         check(lib().pcx_fxcorr_exec(self._h, dst.data_ptr(), dst.pack, f.data_ptr(), f.pack, f.size(),
                                     _stream_ptr(stream)))
 
-    def batch(self, dst: int, src: int, length: int, channels: int, pack: int, dist: int, stream=None) -> None:
-        """Multi-channel correlation on raw device pointers: channel c at c*dist floats."""
-        check(lib().pcx_fxcorr_exec_batch(self._h, dst, pack, dist, src, pack, dist, length, channels,
-                                          _stream_ptr(stream)))
+    def batch(self, dst: int, src: int, length: int, channels: int, pack: int, dist: int, stream=None,
+              workspace: int | None = None, workspace_bytes: int = 0) -> None:
+        """Multi-channel correlation on raw device pointers: channel c at c*dist floats.  With a
+        workspace (device pointer, >= workspace_min() bytes) the unit's scratch comes from it (the
+        reference's TmpFactory_ slot) and the unit allocates nothing of its own."""
+        if workspace is None:
+            check(lib().pcx_fxcorr_exec_batch(self._h, dst, pack, dist, src, pack, dist, length, channels,
+                                              _stream_ptr(stream)))
+        else:
+            check(lib().pcx_fxcorr_exec_batch_ws(self._h, dst, pack, dist, src, pack, dist, length, channels,
+                                                 workspace, int(workspace_bytes), _stream_ptr(stream)))
+
+    def workspace_min(self) -> int:
+        return int(lib().pcx_fxcorr_workspace_min(self._h))
 
 
 class fft_queue:

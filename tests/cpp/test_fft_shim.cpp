@@ -524,6 +524,140 @@ static void test_md_storage() {
     }
 }
 
+// The reference's template surface (fft.hpp:565-605, 2168-2172; fxcorr.hpp:44-91): a unit with a
+// fixed SubSize and its one-argument constructor, non-default node-size strategies, fft_unit_par
+// with all four parameters and with double, and fxcorr_unit with a driven FFT_ and with a
+// TmpFactory_.  Every form is checked against the default unit or the FP64 oracle.
+// a pack-4 view of device memory (any type with real_type, pack_size, size() and data() is a vector)
+struct pack4_view {
+    float* p;
+    size_t n;
+    using real_type                   = float;
+    static constexpr size_t pack_size = 4;
+    size_t       size() const { return n; }
+    float*       data() { return p; }
+    const float* data() const { return p; }
+};
+
+struct driven_fft : pcx_gpu::fft_unit<float, pcx_gpu::fft_order::bit_reversed> {
+    using pcx_gpu::fft_unit<float, pcx_gpu::fft_order::bit_reversed>::fft_unit;
+};
+
+static void test_template_surface() {
+    using namespace pcx_gpu;
+    const size_t n = 4096;
+    std::vector<cxf> x(n);
+    for (size_t i = 0; i < n; ++i)
+        x[i] = cxf(std::sin(0.3f * (float)i), std::cos(0.7f * (float)i + 1.f));
+    vector<float> a(x), b(x), c(x);
+    fft_unit<float, fft_order::normal> dyn(n, 256);
+    fft_unit<float, fft_order::normal, device_allocator<float>, 2048> fixed(n);
+    fft_unit<float, fft_order::normal, device_allocator<float>, dynamic_size, 8, strategy8, strategy48> strat(n, 64);
+    dyn(a);
+    fixed(b);
+    strat(c);
+    auto ha = a.to_host(), hb = b.to_host(), hc = c.to_host();
+    bool same = true;
+    for (size_t i = 0; i < n; ++i)
+        same = same && ha[i] == hb[i] && ha[i] == hc[i];
+    CHECK(same, "fixed SubSize and strategy units equal the default unit");
+    CHECK(rel_l2(ha, oracle_fft(0, false, true, x)) < 2 * 12 * std::numeric_limits<float>::epsilon(),
+          "template surface unit vs oracle");
+
+    // fft_unit_par<double, Order, Allocator, 4>, across 32 vectors of 50 elements (pack 4)
+    const size_t                    m = 32, len = 50;
+    std::vector<std::vector<cxd>>   host(m, std::vector<cxd>(len));
+    std::vector<vector<double>>     src, dst;
+    std::vector<vector<double>*>    dptr;
+    for (size_t i = 0; i < m; ++i) {
+        for (size_t k = 0; k < len; ++k)
+            host[i][k] = cxd(std::sin(0.37 * (double)(i * len + k)), std::cos(0.11 * (double)(i + 3 * k))) *
+                         (1.0 + std::ldexp(1.0, -40));
+        src.emplace_back(host[i]);
+        dst.emplace_back(len);
+    }
+    for (auto& v : dst)
+        dptr.push_back(&v); // a range of pointers to vectors (fft.hpp:2221-2228)
+    fft_unit_par<double, fft_order::normal, device_allocator<double>, 4> par(m);
+    par(dptr, src);
+    pcx_synchronize(nullptr);
+    double num = 0, den = 0;
+    for (size_t j = 0; j < m; ++j) {
+        auto got = dst[j].to_host();
+        for (size_t k = 0; k < len; ++k) {
+            long double wr = 0, wi = 0;
+            for (size_t i = 0; i < m; ++i) {
+                long double ang = -2.0L * 3.141592653589793238462643383279L * (long double)((i * j) % m) / (long double)m;
+                wr += (long double)host[i][k].real() * std::cos(ang) - (long double)host[i][k].imag() * std::sin(ang);
+                wi += (long double)host[i][k].real() * std::sin(ang) + (long double)host[i][k].imag() * std::cos(ang);
+            }
+            num += std::norm(got[k] - cxd((double)wr, (double)wi));
+            den += (double)(wr * wr + wi * wi);
+        }
+    }
+    CHECK(std::sqrt(num / den) < 4 * 5 * std::numeric_limits<double>::epsilon(), "fft_unit_par<double> rel L2 %g",
+          std::sqrt(num / den));
+    // differing packs with a destination on a source: the reference's in-place repack refusal
+    std::vector<vector<float, 8>> s8;
+    for (size_t i = 0; i < 8; ++i)
+        s8.emplace_back(16);
+    bool threw = false;
+    try {
+        std::vector<pack4_view> dv;
+        for (auto& v : s8)
+            dv.push_back({v.data(), 16});
+        fft_unit_par<float> p8(8);
+        p8(dv, s8);
+    } catch (const std::invalid_argument& e) {
+        threw = std::string(e.what()).rfind("cannot perform in-place repack", 0) == 0;
+    }
+    CHECK(threw, "fft_unit_par refuses a destination that overlaps a source at another pack");
+
+    // fxcorr_unit: fused (default FFT_), driven FFT_ (a user type with fft_raw / ifft_raw), and a
+    // TmpFactory_ supplying the fused path's scratch; in place and out of place; block sizes
+    // 1024 and 2^15 (above the fused kernel's 16384)
+    for (size_t N : {size_t(1024), size_t(1) << 15}) {
+        const size_t L = 70000, M = N == 1024 ? 300 : 20000;
+        std::vector<float> sr(L), si(L), kr(M), ki(M);
+        orc_gen_normal(0x5EED0D00 + N, 0.7071067811865476, sr.data(), si.data(), (int64_t)L);
+        orc_gen_uniform(0x5EED0D01 + N, kr.data(), ki.data(), (int64_t)M);
+        std::vector<cxf> sig(L), ker(M);
+        for (size_t i = 0; i < L; ++i)
+            sig[i] = {sr[i], si[i]};
+        for (size_t i = 0; i < M; ++i)
+            ker[i] = std::polar(1.0f, 3.14159265f * kr[i]);
+        std::vector<double> fa(L), fb(L), ga(M), gb(M), o1(L), o2(L);
+        for (size_t i = 0; i < L; ++i)
+            fa[i] = sig[i].real(), fb[i] = sig[i].imag();
+        for (size_t i = 0; i < M; ++i)
+            ga[i] = ker[i].real(), gb[i] = ker[i].imag();
+        orc_xcorr_naive(fa.data(), fb.data(), (int64_t)L, ga.data(), gb.data(), (int64_t)M, o1.data(), o2.data());
+        std::vector<cxd> want(L);
+        for (size_t i = 0; i < L; ++i)
+            want[i] = {o1[i], o2[i]};
+        vector<float> kv(ker);
+        using factory = detail_::pseudo_vector_factory<float>;
+        fxcorr_unit<float> fused(kv, N);
+        fxcorr_unit<float, device_allocator<float>, driven_fft> driven(kv, N);
+        fxcorr_unit<float, device_allocator<float>, fft_unit<float, fft_order::unordered>, factory> with_factory(kv, N);
+        vector<float> v1(sig), v2(sig), v3(sig), o(L);
+        fused(v1);
+        driven(v2);
+        with_factory(v3);
+        with_factory(o, vector<float>(sig));
+        pcx_synchronize(nullptr);
+        auto h1 = v1.to_host(), h2 = v2.to_host(), h3 = v3.to_host(), h4 = o.to_host();
+        CHECK(rel_l2(h1, want) < 4e-6, "fxcorr fused N=%zu err %g", N, rel_l2(h1, want));
+        CHECK(rel_l2(h2, want) < 4e-6, "fxcorr driven FFT_ N=%zu err %g", N, rel_l2(h2, want));
+        CHECK(rel_l2(h3, want) < 4e-6, "fxcorr TmpFactory_ in place N=%zu err %g", N, rel_l2(h3, want));
+        CHECK(rel_l2(h4, want) < 4e-6, "fxcorr TmpFactory_ out of place N=%zu err %g", N, rel_l2(h4, want));
+        bool eq = true;
+        for (size_t i = 0; i < L; ++i)
+            eq = eq && h1[i] == h3[i];
+        CHECK(eq, "fxcorr TmpFactory_ scratch gives the own-buffer result bit for bit (N=%zu)", N);
+    }
+}
+
 int main() {
     for (size_t n = 64; n <= 16384; n *= 2) { // test_fft.cpp:779-798 runs 2^6 .. 2^14
         test_fft_float<8>(n);
@@ -541,6 +675,7 @@ int main() {
     test_errors();
     test_fxcorr();
     test_fft_par();
+    test_template_surface();
     for (size_t n = 1; n < 32; ++n) { // test_op.cpp:241-252
         test_bin_ops<float, 1>(n);
         test_bin_ops<float, 8>(n);

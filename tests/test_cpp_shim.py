@@ -46,6 +46,37 @@ def test_mixed_element_types_are_a_compile_error(tmp_path):
     assert res.returncode != 0
 
 
+@pytest.mark.parametrize("body", [
+    # tests/test_fft.cpp:307, 463-464, 719-720 instantiation forms
+    "auto u = pcx_gpu::fft_unit<float, pcx_gpu::fft_order::normal>(1024, 256); (void)u;",
+    "pcx_gpu::fft_unit_par<float, pcx_gpu::fft_order::normal, pcx_gpu::device_allocator<float>, 4>* p = nullptr; (void)p;",
+    # all seven parameters (fft.hpp:565-571) and the fixed-SubSize constructor (fft.hpp:594-598)
+    "using U = pcx_gpu::fft_unit<float, pcx_gpu::fft_order::bit_reversed, std::allocator<float>, 2048, 8, "
+    "pcx_gpu::strategy8, pcx_gpu::strategy48>; U* u = nullptr; if (u) { U v(4096); U w(4096, std::allocator<float>{}); }",
+    "using U = pcx_gpu::fft_unit<double, pcx_gpu::fft_order::normal, std::allocator<double>, pcx_gpu::dynamic_size, 4, "
+    "pcx_gpu::strategy42, pcx_gpu::strategy42>; U* u = nullptr; if (u) { U v(4096, 512); }",
+    "pcx_gpu::fft_unit_par<double>* p = nullptr; (void)p;",
+    # fxcorr_unit's policy slots (fxcorr.hpp:44-48)
+    "using X = pcx_gpu::fxcorr_unit<float, pcx_gpu::device_allocator<float>, "
+    "pcx_gpu::fft_unit<float, pcx_gpu::fft_order::normal>, pcx_gpu::detail_::pseudo_vector_factory<float>>; "
+    "X* x = nullptr; (void)x;",
+])
+def test_reference_template_forms_compile(tmp_path, body):
+    res = _syntax(tmp_path, body)
+    assert res.returncode == 0, res.stderr
+
+
+@pytest.mark.parametrize("body", [
+    # SubSize must be a power of two >= the default pack size (fft.hpp:572-574)
+    "pcx_gpu::fft_unit<float, pcx_gpu::fft_order::normal, std::allocator<float>, 12>* u = nullptr; (void)u;",
+    "pcx_gpu::fft_unit<float, pcx_gpu::fft_order::normal, std::allocator<float>, 4>* u = nullptr; (void)u;",
+    # the one-argument constructor belongs to a fixed SubSize, the sub_size argument to a dynamic one
+    "pcx_gpu::fft_unit<float, pcx_gpu::fft_order::normal, std::allocator<float>, 2048> u(1024, 256);",
+])
+def test_reference_template_constraints(tmp_path, body):
+    assert _syntax(tmp_path, body).returncode != 0
+
+
 @pytest.mark.gpu
 def test_cpp_drop_in():
     assert os.path.exists(BIN), "build the C++ test first: python __graft_entry__.py"

@@ -225,3 +225,24 @@ def test_f64_type_checks(pa):
     assert rc == _lib.ERR_INVALID_ARGUMENT
     assert _lib.lib().pcx_plan_element_size(u64.handle) == 8
     assert _lib.lib().pcx_plan_element_size(u32.handle) == 4
+
+
+def test_f64_in_place_repack_message(pa):
+    """fft.hpp:783, 889: a pack change wider than one SIMD register cannot run in place; a double
+    register holds 4 values, so packs 4 -> 8 are refused for double (and allowed for float)."""
+    from packed_amd import PcxError, _lib
+
+    L = _lib.lib()
+    u64 = pa.fft_unit(1024, dtype="float64")
+    v64 = pa.vector(1024, pack=8, dtype="float64")
+    rc = L.pcx_fft_f64(u64.handle, v64.data_ptr(), 8, v64.data_ptr(), 4, 1024, None)
+    assert rc == _lib.ERR_INPLACE_REPACK
+    assert L.pcx_last_error().decode() == ("cannot perform in-place repack from source pack size (which is 4) "
+                                            "to destination pack size (which is 8)")
+    rc = L.pcx_ifft_f64(u64.handle, 1, v64.data_ptr(), 2, v64.data_ptr(), 8, None)
+    assert rc == _lib.ERR_INPLACE_REPACK
+    u32 = pa.fft_unit(1024)
+    v32 = pa.vector(1024, pack=8)
+    assert L.pcx_fft(u32.handle, v32.data_ptr(), 8, v32.data_ptr(), 4, 1024, None) == _lib.SUCCESS
+    with pytest.raises(PcxError):
+        _lib.check(L.pcx_fft(u32.handle, v32.data_ptr(), 16, v32.data_ptr(), 8, 1024, None))

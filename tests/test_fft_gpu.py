@@ -192,7 +192,7 @@ def test_one_pass_dif(orc, pa, lg, order):
 
 
 @pytest.mark.parametrize("order", ORDERS)
-def test_one_pass_dif_agrees_with_four_step(orc, pa, monkeypatch, order):
+def test_one_pass_dif_agrees_with_four_step(orc, pa, order):
     """The same batch through the one-pass kernel and through the two-pass four-step split."""
     import torch
 
@@ -200,12 +200,36 @@ def test_one_pass_dif_agrees_with_four_step(orc, pa, monkeypatch, order):
     xs = [orc.gen_normal(0x5EED0040 + b, n) for b in range(batch)]
     src = to_device(batch_buffer(xs, n, 2 * n))
     outs = []
-    for flag in ("1", "0"):
-        monkeypatch.setenv("PCX_DIF", flag)
+    for flag in (True, False):
         unit = pa.fft_unit(n, order=order)
+        unit.set_one_pass(flag)
         d = torch.zeros_like(src)
         unit.batch(d.data_ptr(), src.data_ptr(), batch, n, 2 * n, n, 2 * n)
         torch.cuda.synchronize()
         outs.append(d.cpu().numpy().astype(np.float64))
     diff = np.linalg.norm(outs[0] - outs[1]) / np.linalg.norm(outs[1])
     assert diff < fft_tol(n), diff
+
+
+def test_shared_plan_on_two_streams(orc, pa):
+    """One four-step plan (as copies of a C++ fft_unit share it) called on two streams without a
+    synchronisation between the calls: the second call waits for the first one's use of the
+    plan's intermediate, so both batches come out right."""
+    import torch
+
+    n, batch = 1 << 16, 64
+    unit = pa.fft_unit(n)
+    xs = [[orc.gen_normal(0x5EED0070 + 100 * k + b, n) for b in range(batch)] for k in range(2)]
+    srcs = [to_device(batch_buffer(x, n, 2 * n)) for x in xs]
+    dsts = [torch.zeros_like(s) for s in srcs]
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    torch.cuda.synchronize()
+    for rep in range(3):
+        for k in range(2):
+            unit.batch(dsts[k].data_ptr(), srcs[k].data_ptr(), batch, n, 2 * n, n, 2 * n, stream=streams[k])
+    torch.cuda.synchronize()
+    for k in range(2):
+        out = dsts[k].cpu().numpy()
+        for b in (0, batch - 1):
+            got = orc.from_packed(out[b * 2 * n:], n, n)
+            assert orc.rel_l2(got, orc.fft64(xs[k][b].astype(np.complex128))) < fft_tol(n), (k, b)

@@ -174,3 +174,62 @@ def test_multichannel_distance_is_checked(orc, pa):
             xc.batch(buf.data_ptr(), buf.data_ptr(), 1000, 2, 8, dist)
     xc.batch(buf.data_ptr(), buf.data_ptr(), 1000, 2, 8, 2000)  # exactly one footprint apart: fine
     torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("L,M,N", [(200000, 20000, 1 << 15), (300000, 5000, 1 << 16), (70000, 30000, 1 << 17)])
+@pytest.mark.parametrize("inplace", [True, False])
+def test_block_path_above_16384(orc, pa, L, M, N, inplace):
+    """FFT blocks above 16384 (fxcorr.hpp:62-69 takes any power of two) run through HBM and the
+    batched four-step FFT; kernels longer than 16384 taps included."""
+    f = orc.gen_normal(0x5EED0061, L, np.sqrt(0.5)).astype(np.complex128)
+    g = np.exp(1j * np.pi * np.real(orc.gen_uniform(0x5EED0062, M)))
+    got, xc = _run(pa, f.astype(np.complex64), g.astype(np.complex64), N, inplace=inplace)
+    assert xc.step(8) == (N - (M - 1)) // 8 * 8
+    assert orc.rel_l2(got, orc.xcorr_naive(f, g)) < 4e-6
+
+
+@pytest.mark.parametrize("L,M,N,C", [(20000, 300, 1024, 3), (3000, 57, 128, 4), (150000, 20000, 1 << 15, 2)])
+@pytest.mark.parametrize("inplace", [True, False])
+def test_workspace_matches_own_buffers(orc, pa, L, M, N, C, inplace):
+    """pcx_fxcorr_exec_batch_ws (the TmpFactory_ slot, fxcorr.hpp:10-32): with a workspace of one
+    fft_size vector, or three, the outputs equal the unit's own-buffer path bit for bit."""
+    import torch
+
+    pack = 8
+    dist = orc.real_size(L, pack)
+    g = np.exp(1j * np.pi * np.real(orc.gen_uniform(0x5EED0063, M)))
+    buf = np.zeros(dist * C, np.float32)
+    for c in range(C):
+        buf[c * dist:(c + 1) * dist] = orc.to_packed(orc.gen_normal(0x5EED0064 + c, L, np.sqrt(0.5)), pack)
+    xc = pa.fxcorr_unit(pa.vector.from_complex(g.astype(np.complex64), pack), N)
+    assert xc.workspace_min() == 8 * N
+    src = torch.from_numpy(buf).cuda()
+    ref = src.clone() if inplace else torch.zeros_like(src)
+    xc.batch(ref.data_ptr(), (ref if inplace else src).data_ptr(), L, C, pack, dist)
+    for mult in (1, 3):
+        ws = torch.empty(2 * N * mult, dtype=torch.float32, device="cuda")
+        got = src.clone() if inplace else torch.zeros_like(src)
+        xc.batch(got.data_ptr(), (got if inplace else src).data_ptr(), L, C, pack, dist, workspace=ws.data_ptr(),
+                 workspace_bytes=ws.numel() * 4)
+        torch.cuda.synchronize()
+        assert torch.equal(got, ref), mult
+    fs = [orc.from_packed(buf[c * dist:], L, pack).astype(np.complex128) for c in range(C)]
+    out = ref.cpu().numpy()
+    for c in range(C):
+        assert orc.rel_l2(orc.from_packed(out[c * dist:], L, pack), orc.xcorr_naive(fs[c], g)) < 4e-6
+
+
+def test_workspace_too_small_is_refused(orc, pa):
+    import torch
+
+    from packed_amd import PcxError
+
+    g = pa.vector.from_complex(orc.gen_uniform(0x5EED0F01, 64))
+    xc = pa.fxcorr_unit(g, 512)
+    buf = torch.zeros(2 * 1000, dtype=torch.float32, device="cuda")
+    ws = torch.empty(2 * 512, dtype=torch.float32, device="cuda")
+    with pytest.raises(PcxError):
+        xc.batch(buf.data_ptr(), buf.data_ptr(), 1000, 1, 8, 2000, workspace=ws.data_ptr(), workspace_bytes=4 * 1023)
+    pa.fxcorr_unit(g, 1024, precision="bf16x3")  # split precision covers fft_size 16 .. 16384
+    with pytest.raises(PcxError):
+        pa.fxcorr_unit(g, 1 << 15, precision="bf16x3")

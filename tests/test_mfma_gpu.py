@@ -79,3 +79,22 @@ def test_split_rejects_unsupported(pa, precision):
         pa.fft_unit(1024, order="bit_reversed", precision=precision)
     with pytest.raises(PcxError):
         pa.fft_unit(8, precision=precision)
+
+
+@pytest.mark.parametrize("lg,scale", [(4, 1e36), (6, 1e35), (4, 1e-37), (8, 1e-37)])
+def test_fp16x2_exponent_extremes(orc, pa, lg, scale):
+    """Inputs near the ends of the FP32 exponent range: the per-butterfly scale 2^e reaches
+    e = +-126 (both 2^e and 2^-e stay normal), so neither FP16 overflow nor FP16 underflow sets in."""
+    import torch
+
+    n = 1 << lg
+    x = orc.gen_normal(0x5EED0910 + lg, n).astype(np.complex128) * scale
+    x32 = x.astype(np.complex64)
+    s = to_device(batch_buffer([x32], n, 2 * n))
+    d = torch.zeros_like(s)
+    pa.fft_unit(n, precision="fp16x2").batch(d.data_ptr(), s.data_ptr(), 1, n, 2 * n, n, 2 * n)
+    torch.cuda.synchronize()
+    got = unpack_batch(d.cpu().numpy(), n, n, 2 * n, 1)[0]
+    assert np.all(np.isfinite(got))
+    err = orc.rel_l2(got, orc.fft64(x32.astype(np.complex128)))
+    assert err < 1e-6, err

@@ -129,3 +129,30 @@ def test_queue_autotune_and_cache(orc, pa, tmp_path):
     q2.exec(dst.data_ptr(), src.data_ptr())
     torch.cuda.synchronize()
     _check(orc, jobs, data, dst.cpu().numpy())
+
+
+def test_capture_then_autotune_then_capture(orc, pa):
+    """Autotuning after a capture drops the captured graph (its kernels, grids and twiddle tables
+    may all change); launching it fails until the queue is captured again, and the new graph
+    gives the oracle's results.  Autotuning refuses dst == src (it overwrites dst many times)."""
+    import torch
+
+    from packed_amd import PcxError
+
+    jobs, total, buf, data = _case(orc, 5)
+    src = torch.from_numpy(buf).cuda()
+    dst = torch.zeros_like(src)
+    q = pa.fft_queue(jobs)
+    s = torch.cuda.Stream()
+    q.capture(dst.data_ptr(), src.data_ptr(), s)
+    with pytest.raises(PcxError):
+        q.autotune(src.data_ptr(), src.data_ptr())
+    q.autotune(dst.data_ptr(), src.data_ptr())
+    with pytest.raises(PcxError):
+        q.launch(s)
+    q.capture(dst.data_ptr(), src.data_ptr(), s)
+    dst.zero_()
+    torch.cuda.synchronize()
+    q.launch(s)
+    s.synchronize()
+    _check(orc, jobs, data, dst.cpu().numpy())
