@@ -111,6 +111,20 @@ int pcx_fft_batch(pcx_plan_t plan, int inverse, int normalized, float* dst, int6
                   int64_t dst_dist, const float* src, int64_t src_pack, int64_t src_dist,
                   int64_t src_size, int64_t batch, void* stream);
 
+/* pcx_fft_batch without any allocation, for callers that cannot allocate device memory: code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc.  No counterpart
+ * in the reference (its units allocate nothing per call on the CPU).  The four-step intermediate
+ * (fft_size > 16384) lives in a fixed 128 MiB device array per GPU, held until the process ends,
+ * instead of the plan's own buffer; batches of any size run through it in chunks of
+ * 128 MiB / (8 * fft_size) transforms.  Every FP32 plan fits (2^24 points need exactly 128 MiB);
+ * a larger need fails with PCX_ERR_UNSUPPORTED.  The first call on a device looks up the array's
+ * address; later calls make no allocation, free or allocating library call.  All calls on one GPU
+ * share the array: calls that overlap in time on one GPU (other streams, other threads) race and
+ * give wrong results.  Results equal pcx_fft_batch's bit for bit. */
+int pcx_fft_batch_noalloc(pcx_plan_t plan, int inverse, int normalized, float* dst, int64_t dst_pack,
+                          int64_t dst_dist, const float* src, int64_t src_pack, int64_t src_dist,
+                          int64_t src_size, int64_t batch, void* stream);
+
 /* ------------------------------------------------------------------ FFT unit, double
  * Replaces fft_unit<double, Order, Allocator>::fft_unit(fft_size, sub_size, allocator)
  * (fft.hpp:600-605 with T = double; default pack 4, types.hpp:70-72).  Same checks and messages

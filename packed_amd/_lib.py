@@ -84,6 +84,7 @@ SIGNATURES = {
     "pcx_fft": (_int, [_vp, _vp, _i64, _vp, _i64, _i64, _vp]),
     "pcx_ifft": (_int, [_vp, _int, _vp, _i64, _vp, _i64, _vp]),
     "pcx_fft_batch": (_int, [_vp, _int, _int, _vp, _i64, _i64, _vp, _i64, _i64, _i64, _i64, _vp]),
+    "pcx_fft_batch_noalloc": (_int, [_vp, _int, _int, _vp, _i64, _i64, _vp, _i64, _i64, _i64, _i64, _vp]),
     "pcx_plan_create_f64": (_int, [_i64, _i64, _int, C.POINTER(_vp)]),
     "pcx_plan_element_size": (_int, [_vp]),
     "pcx_fft_f64": (_int, [_vp, _vp, _i64, _vp, _i64, _i64, _vp]),

@@ -720,7 +720,8 @@ extern "C" int pcx_ifft(pcx_plan_t p, int normalized, float* dst, int64_t dst_pa
 namespace {
 template<class T>
 int fft_batch(pcx_plan_t p, int inverse, int normalized, T* dst, int64_t dst_pack, int64_t dst_dist, const T* src,
-              int64_t src_pack, int64_t src_dist, int64_t src_size, int64_t batch, void* stream) {
+              int64_t src_pack, int64_t src_dist, int64_t src_size, int64_t batch, void* stream,
+              const ExtTmp* ext = nullptr) {
     if (!p)
         return fail(PCX_ERR_INVALID_ARGUMENT, "plan is null");
     if (batch > 1) {
@@ -740,7 +741,7 @@ int fft_batch(pcx_plan_t p, int inverse, int normalized, T* dst, int64_t dst_pac
             return fail(PCX_ERR_INVALID_ARGUMENT, "in-place batch needs equal source and destination distances");
     }
     return run_batch(p, inverse, normalized, dst, dst_pack, dst_dist, src, src_pack, src_dist,
-                     inverse ? p->n : src_size, batch, (hipStream_t)stream);
+                     inverse ? p->n : src_size, batch, (hipStream_t)stream, nullptr, nullptr, ext);
 }
 } // namespace
 
@@ -749,6 +750,53 @@ extern "C" int pcx_fft_batch(pcx_plan_t p, int inverse, int normalized, float* d
                              int64_t src_size, int64_t batch, void* stream) {
     return fft_batch(p, inverse, normalized, dst, dst_pack, dst_dist, src, src_pack, src_dist, src_size, batch,
                      stream);
+}
+
+// ---------------------------------------------------------------- allocation-free batched FFT
+// The four-step intermediate of pcx_fft_batch_noalloc: a fixed 128 MiB array per GPU (one
+// 2^24-point FP32 transform, the largest plan), held until the process ends.  Host code uses the
+// address hipGetSymbolAddress returns for the current device, never the array's name (a host-side
+// stand-in).  The array needs no clearing: pass A writes every element pass B reads.
+constexpr int64_t kNoAllocBytes = (int64_t)128 << 20;
+static __device__ __align__(256) float scratch[kNoAllocBytes / (int64_t)sizeof(float)];
+
+namespace {
+std::mutex g_scratch_mu;
+void*      g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device, looked up once per device.
+int noalloc_scratch(void** out) {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices)
+        return fail(PCX_ERR_UNSUPPORTED, "device " + s(dev) + " is beyond the supported " + s(kMaxDevices));
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr)
+            return fail(PCX_ERR_HIP, "hipGetSymbolAddress returned no address for the fixed intermediate");
+        g_scratch_ptr[dev] = ptr;
+    }
+    *out = g_scratch_ptr[dev];
+    return PCX_SUCCESS;
+}
+} // namespace
+
+extern "C" int pcx_fft_batch_noalloc(pcx_plan_t p, int inverse, int normalized, float* dst, int64_t dst_pack,
+                                     int64_t dst_dist, const float* src, int64_t src_pack, int64_t src_dist,
+                                     int64_t src_size, int64_t batch, void* stream) {
+    if (!p)
+        return fail(PCX_ERR_INVALID_ARGUMENT, "plan is null");
+    if (p->two_pass && 8 * p->n > kNoAllocBytes)
+        return fail(PCX_ERR_UNSUPPORTED, "fft_size (which is " + s(p->n) + ") needs " + s(8 * p->n) +
+                                             " bytes of intermediate; the fixed intermediate holds " + s(kNoAllocBytes));
+    void* buf = nullptr;
+    if (int e = noalloc_scratch(&buf))
+        return e;
+    const ExtTmp ext{buf, kNoAllocBytes};
+    return fft_batch(p, inverse, normalized, dst, dst_pack, dst_dist, src, src_pack, src_dist, src_size, batch, stream,
+                     &ext);
 }
 
 // ---------------------------------------------------------------- FP64 transforms
