@@ -306,6 +306,17 @@ __device__ __forceinline__ void row_group(const RowArgsT<T>& p, int64_t wg, T* l
     // multiple of 2^PP, or smaller than the room left below the next multiple.
     constexpr int  PP     = LA - 6 > 5 ? LA - 6 : 5;
     auto           paddr  = [&](int b, int o) { return g * STR + b + (b >> PP) + o + (o >> PP); };
+    // Row-fastest assignment of the strided passes: thread t takes row ga = t mod G and butterfly
+    // index ja = t / G, so the lanes of a wave hold consecutive rows (k2) and a strided load or
+    // store of element k1 of rows k2 .. k2 + G - 1 touches G consecutive elements.  The first
+    // stage (strided load) or the last stage (strided store) runs in this assignment; the LDS
+    // exchange next to it does the transposition.
+    constexpr bool ALT_LD = LD == LD_STRIDED && !P::HALF && P::NSTAGES >= 2;
+    constexpr bool ALT_ST = ST == ST_STRIDED && !P::HALF && P::NSTAGES >= 2;
+    const int      ga     = t % G;
+    const int      ja     = t / G;
+    auto           addrA  = [&](int b, int o) { return ga * STR + b + (b >> 5) + o + (o >> 5); };
+    const StageMap<decltype(addrA)> mapA{ja, addrA};
 
     cx<T> v[PPT];
     // ------------------------------------------------------------------ load
@@ -419,6 +430,22 @@ __device__ __forceinline__ void row_group(const RowArgsT<T>& p, int64_t wg, T* l
                 lre[o] = v[u].x;
                 lim[o] = v[u].y;
             }
+        } else if constexpr (ALT_LD) {
+            // LD_STRIDED, row-fastest: element k1 of row k2 sits at slot k2 + (k1 << rows_lg); the
+            // stage-0 inputs of this thread's butterflies go straight to registers, G consecutive
+            // slots per wave and element (the rows of a workgroup lie in one transform)
+            const RdView<UNI, T> src(tbase(p.src, p.src_offs, p.src_dist, row0 >> p.rows_lg), p.src_lp);
+            const int64_t        rw = row0 + ga;
+            const bool           ok = rw < p.nrows;
+            const uint32_t       kb = (uint32_t)(rw & rmask);
+#pragma unroll
+            for (int m = 0; m < PPT / R0; ++m)
+#pragma unroll
+                for (int r = 0; r < R0; ++r) {
+                    const uint32_t k1 = (uint32_t)in_elem<LA, false, T>(ja, m, r);
+                    cx<T>          x  = src.ld(ok ? poff(kb + (k1 << p.rows_lg), p.src_lp) : 0u, p.nt_load);
+                    v[m * R0 + r]     = ok ? x : cx<T>{T(0), T(0)};
+                }
         } else { // LD_STRIDED: element k1 of row k2 sits at slot k2 + (k1 << rows_lg)
             // (the rows of a workgroup lie in one transform: G divides the rows per transform)
             const RdView<UNI, T> src(tbase(p.src, p.src_offs, p.src_dist, row0 >> p.rows_lg), p.src_lp);
@@ -437,14 +464,16 @@ __device__ __forceinline__ void row_group(const RowArgsT<T>& p, int64_t wg, T* l
                 lim[o] = x.y;
             }
         }
-        __syncthreads();
+        if constexpr (!ALT_LD) {
+            __syncthreads();
 #pragma unroll
-        for (int m = 0; m < PPT / R0; ++m)
+            for (int m = 0; m < PPT / R0; ++m)
 #pragma unroll
-            for (int r = 0; r < R0; ++r) {
-                int o         = LD == LD_PERM ? paddr(jt + m * TA, r * (P::A / R0)) : addr(jt + m * TA, r * (P::A / R0));
-                v[m * R0 + r] = cx<T>{lre[o], lim[o]};
-            }
+                for (int r = 0; r < R0; ++r) {
+                    int o = LD == LD_PERM ? paddr(jt + m * TA, r * (P::A / R0)) : addr(jt + m * TA, r * (P::A / R0));
+                    v[m * R0 + r] = cx<T>{lre[o], lim[o]};
+                }
+        }
     }
     if (p.conj_in) {
 #pragma unroll
@@ -453,14 +482,24 @@ __device__ __forceinline__ void row_group(const RowArgsT<T>& p, int64_t wg, T* l
     }
     // ------------------------------------------------------------------ transform
     // (the barrier before the first stage's LDS writes also orders them after the reads above)
-    stockham<LA, false, decltype(addr), PREC, T>(v, lre, lim, jt, addr, ltw, mf);
+    if constexpr (ALT_LD)
+        stockham<LA, false, decltype(addr), PREC, T, 0, true, false, decltype(addrA), decltype(addrA)>(
+            v, lre, lim, jt, addr, ltw, mf, &mapA, &mapA);
+    else if constexpr (ALT_ST)
+        stockham<LA, false, decltype(addr), PREC, T, 0, false, true, decltype(addrA), decltype(addrA)>(
+            v, lre, lim, jt, addr, ltw, mf, &mapA, &mapA);
+    else
+        stockham<LA, false, decltype(addr), PREC, T>(v, lre, lim, jt, addr, ltw, mf);
+    // the thread's outputs: butterfly index and row of the last stage's assignment
+    const int      jo  = ALT_ST ? ja : jt;
+    const uint32_t ko2 = ALT_ST ? (uint32_t)((row0 + ga) & rmask) : k2;
 
     if constexpr (TW > 0) {
 #pragma unroll
         for (int m = 0; m < PPT / RL; ++m)
 #pragma unroll
             for (int r = 0; r < RL; ++r) {
-                uint32_t e    = (uint32_t)out_elem<LA, false, T>(jt, m, r) * k2;
+                uint32_t e    = (uint32_t)out_elem<LA, false, T>(jo, m, r) * ko2;
                 v[m * RL + r] = cmul(v[m * RL + r], PassTw<TW, T>::get(lptw, p.twl, p.twh, p.tw_ls, p.tw_lg, e));
                 __builtin_amdgcn_sched_barrier(0); // one element's table reads in flight at a time
             }
@@ -553,6 +592,20 @@ __device__ __forceinline__ void row_group(const RowArgsT<T>& p, int64_t wg, T* l
                     dst.st(poff((uint32_t)(rw & rmask) + ((uint32_t)(idx / G) << p.rows_lg), p.dst_lp), v[u],
                            p.nt_store);
             }
+        }
+    } else if constexpr (ALT_ST) {
+        // ST_STRIDED, row-fastest: element k1 of row k2 goes to slot k2 + (k1 << rows_lg), G
+        // consecutive slots per wave and element, straight from registers
+        const WrView<UNI, T> dst(tbase(p.dst, p.dst_offs, p.dst_dist, row0 >> p.rows_lg), p.dst_lp);
+        const int64_t        rw = row0 + ga;
+        if (rw < p.nrows) {
+#pragma unroll
+            for (int m = 0; m < PPT / RL; ++m)
+#pragma unroll
+                for (int r = 0; r < RL; ++r) {
+                    const uint32_t k1 = (uint32_t)out_elem<LA, false, T>(ja, m, r);
+                    dst.st(poff(ko2 + (k1 << p.rows_lg), p.dst_lp), v[m * RL + r], p.nt_store);
+                }
         }
     } else {
         __syncthreads(); // the last stage's LDS reads are done

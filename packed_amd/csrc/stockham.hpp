@@ -156,28 +156,56 @@ __device__ __forceinline__ void dft16_mfma(const MfmaDft16* mf, cx<T>* v) {
         mf->template dft16<PREC>(v);
 }
 
+// A thread's position in a stage: its butterfly index within the transform (jt) and the LDS
+// address map of its transform (addr).
+template<class Addr>
+struct StageMap {
+    int         jt;
+    const Addr& addr;
+};
+
 // PREC_BF16X3 / PREC_FP16X2 run the radix-16 butterflies on the matrix cores (mf holds the
 // operands).
 // With Plan::HALF, lre and lim are the same array and each exchange runs in two halves.
 // T = double runs the same schedule in FP64 (PREC is then PREC_FP32, the plain butterflies).
-template<int LA, bool REV, class Addr, int PREC = PREC_FP32, class T = float, int PX = 0>
+// first / last (optional, not with Plan::HALF, at least two stages) give the first and the last
+// stage another assignment of butterflies to threads than the middle stages (jt0, addr): the
+// exchange before the last stage (or after the first) then also transposes, so a strided store
+// (load) needs no LDS round trip of its own.
+template<int LA, bool REV, class Addr, int PREC = PREC_FP32, class T = float, int PX = 0, bool FM = false,
+         bool LM = false, class AddrF = Addr, class AddrL = Addr>
 __device__ __forceinline__ void stockham(cx<T>* v, T* lre, T* lim, int jt0, const Addr& addr, const T* ltw,
-                                         const MfmaDft16* mf = nullptr) {
+                                         const MfmaDft16* mf = nullptr, const StageMap<AddrF>* first = nullptr,
+                                         const StageMap<AddrL>* last = nullptr) {
     using P           = Plan<LA, REV, T, PX>;
     static_assert(PREC == PREC_FP32 || sizeof(T) == 4, "matrix-core butterflies are FP32 only");
+    static_assert(!(FM || LM) || (!P::HALF && P::NSTAGES >= 2), "remapped stages need two stages and two arrays");
     constexpr int TA  = P::TA;
     constexpr int PPT = P::PPT;
     sfor<0, P::NSTAGES>([&](auto ss) {
-        constexpr int s  = decltype(ss)::value;
-        constexpr int R  = P::radix(s);
-        constexpr int Ns = P::ns(s);
-        constexpr int NB = PPT / R; // butterflies per thread
+        constexpr int  s    = decltype(ss)::value;
+        constexpr int  R    = P::radix(s);
+        constexpr int  Ns   = P::ns(s);
+        constexpr int  NB   = PPT / R; // butterflies per thread
+        constexpr bool MAPF = FM && s == 0;
+        constexpr bool MAPL = LM && s + 1 == P::NSTAGES;
         // (the 2^14-point plans need the registers; smaller plans run faster with the addresses
         // precomputed)
-        const int     jt = P::HALF ? opaque(jt0) : jt0;
+        int jt;
+        if constexpr (P::HALF)
+            jt = opaque(jt0);
+        else if constexpr (MAPL)
+            jt = last->jt;
+        else if constexpr (MAPF)
+            jt = first->jt;
+        else
+            jt = jt0;
         if constexpr (s > 0 && !P::HALF) {
             __syncthreads(); // previous stage's LDS writes are visible
-            stage_read<LA, REV, s, 2, PX>(v, lre, lim, jt, addr);
+            if constexpr (MAPL)
+                stage_read<LA, REV, s, 2, PX>(v, lre, lim, jt, last->addr);
+            else
+                stage_read<LA, REV, s, 2, PX>(v, lre, lim, jt, addr);
         }
         // twiddle and butterfly one radix-R group at a time, so only one group's twiddles are live
 #pragma unroll
@@ -199,7 +227,10 @@ __device__ __forceinline__ void stockham(cx<T>* v, T* lre, T* lim, int jt0, cons
         if constexpr (s + 1 < P::NSTAGES) {
             __syncthreads(); // every read of this stage is done before LDS is overwritten
             if constexpr (!P::HALF) {
-                stage_write<LA, REV, s, 2, PX>(v, lre, lim, jt, addr);
+                if constexpr (MAPF)
+                    stage_write<LA, REV, s, 2, PX>(v, lre, lim, jt, first->addr);
+                else
+                    stage_write<LA, REV, s, 2, PX>(v, lre, lim, jt, addr);
             } else {
                 // real parts out and back in the next stage's layout, then imaginary parts
                 stage_write<LA, REV, s, 0, PX>(v, lre, lim, jt, addr);
