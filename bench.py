@@ -457,8 +457,9 @@ def run_config3(args):
 
 def run_config4(args):
     """10,000 FP32 C2C transforms, sizes 2^6..2^20 drawn log-uniform (seeded), complex normal
-    inputs, mixed directions: persistent work queue (eager, hipGraph, autotuned) vs one launch per
-    size class and one launch per transform."""
+    inputs, mixed directions: the work queue (every transform in one persistent launch, eager and
+    as a hipGraph; then autotuned, where the tuner may move size classes to grouped launches after
+    the persistent one) vs one launch per size class and one launch per transform."""
     import numpy as np
     import torch
 
@@ -491,7 +492,12 @@ def run_config4(args):
         return {"ms": round(ms, 3), "gflops": round(flops / (ms * 1e-3) / 1e9, 1),
                 "gbps": round(algo_bytes / (ms * 1e-3) / 1e9, 1)}
 
+    # the default queue: every transform in one persistent launch
     res["queue_eager"] = rate(time_steps(lambda: q.exec(dst.data_ptr(), src.data_ptr()), 5, 1))
+    s1 = torch.cuda.Stream()
+    q.capture(dst.data_ptr(), src.data_ptr(), s1)
+    with torch.cuda.stream(s1):
+        res["queue_graph_one_launch"] = rate(time_steps(lambda: q.launch(s1), 5, 1))
     # one launch per (size, direction) group
     units = {lg: pa.fft_unit(1 << lg) for lg in range(6, 21)}
     groups = []

@@ -199,10 +199,13 @@ int     pcx_fxcorr_exec_batch_ws(pcx_fxcorr_t xc, float* dst, int64_t dst_pack, 
  * A work queue of transforms of uneven power-of-two sizes and mixed directions, natural order,
  * each vector in split layout (pack = its size: re[N] then im[N]) at its own offset.  The
  * reference runs one fft_unit per size and one call per vector; this is its batched form.
- * Sizes up to 4096 run in one persistent launch that pulls work items from a device counter;
- * larger sizes run as grouped single-pass or four-step launches.  pcx_queue_capture records the
- * whole pass into a hipGraph for fixed (dst, src); pcx_queue_autotune times the launch shapes
- * on the caller's data and keeps the fastest (optionally persisted in a cache file). */
+ * Every transform of 2^1 .. 2^20 points runs in one persistent launch whose workgroups pull
+ * items from a device counter, largest first: groups of single-pass transforms up to 2^14, and
+ * whole four-step transforms of 2^15 .. 2^20 (each through an intermediate owned by its
+ * workgroup: the queue holds 2 * 2^L floats per CU, 2^L the largest such size in the list).
+ * Sizes above 2^20 follow as grouped four-step launches.  pcx_queue_capture records the whole
+ * pass into a hipGraph for fixed (dst, src); pcx_queue_autotune times the launch shapes on the
+ * caller's data and keeps the fastest (optionally persisted in a cache file). */
 typedef struct {
     int64_t log2n;      /* transform size 2^log2n, 1 <= log2n <= 24 */
     int32_t inverse;    /* 0 forward, 1 inverse */

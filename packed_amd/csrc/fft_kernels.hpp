@@ -224,16 +224,16 @@ constexpr int default_rows() {
     return Plan<LA, false, T>::TA >= 256 ? 1 : 256 / Plan<LA, false, T>::TA;
 }
 
-template<int LA, int G_, class T = float>
+template<int LA, int G_, class T = float, int PX = 0>
 struct RowShape {
-    static constexpr int TA = Plan<LA, false, T>::TA;
+    static constexpr int TA = Plan<LA, false, T, PX>::TA;
     static constexpr int G  = G_; // rows per workgroup
     static constexpr int NT = G * TA;
     // odd row stride: rows of a strided access land in distinct banks
     static constexpr int STRIDE = (1 << LA) + ((1 << LA) >> 5) + 1;
     // re and im arrays, or one array shared by both halves of each exchange (Plan::HALF)
-    static constexpr int DATA   = (Plan<LA, false, T>::HALF ? 1 : 2) * G * STRIDE;
-    __device__ static T* im(T* lre) { return Plan<LA, false, T>::HALF ? lre : lre + DATA / 2; }
+    static constexpr int DATA   = (Plan<LA, false, T, PX>::HALF ? 1 : 2) * G * STRIDE;
+    __device__ static T* im(T* lre) { return Plan<LA, false, T, PX>::HALF ? lre : lre + DATA / 2; }
 };
 
 // Waves per SIMD that the LDS footprint allows (argument of __launch_bounds__).
@@ -253,28 +253,28 @@ constexpr int waves_per_simd(int threads, int lds_bytes) {
 // Element held by local slot s = jt + u*TA of a permuted (bit_reversed / unordered) row.  When a
 // thread's slots differ only in their high bits (TA >= 32: q touches the low five bits, which
 // belong to jt) this is base(jt) + brev(u), a compile-time offset from one address.
-template<int LA, int ORD, class T = float>
+template<int LA, int ORD, class T = float, int PX = 0>
 __device__ __forceinline__ int perm_base(int jt) {
-    using P           = Plan<LA, false, T>;
+    using P           = Plan<LA, false, T, PX>;
     constexpr int LT  = ilog2c(P::TA);
     constexpr int LPP = ilog2c(P::PPT);
     uint32_t      j   = ORD == ORD_UNORD ? uq((uint32_t)jt) : (uint32_t)jt;
     return (int)(brev_bits(j, LT) << LPP);
 }
-template<int LA, int ORD, class T = float>
+template<int LA, int ORD, class T = float, int PX = 0>
 __device__ __forceinline__ int perm_elem(int jt, int u) {
-    using P = Plan<LA, false, T>;
+    using P = Plan<LA, false, T, PX>;
     if constexpr (P::TA >= 32)
-        return perm_base<LA, ORD, T>(jt) + brevc(u, ilog2c(P::PPT));
+        return perm_base<LA, ORD, T, PX>(jt) + brevc(u, ilog2c(P::PPT));
     else
         return (int)slot_freq(ORD, (uint32_t)(jt + u * P::TA), LA);
 }
 
-template<int LA, int G, int LD, int ST, int TW, bool UNI, int ORD, int PREC = PREC_FP32, class T = float>
+template<int LA, int G, int LD, int ST, int TW, bool UNI, int ORD, int PREC = PREC_FP32, class T = float, int PX = 0>
 __device__ __forceinline__ void row_group(const RowArgsT<T>& p, int64_t wg, T* lre, T* lim, const T* ltw,
                                           const T* lptw, const MfmaDft16* mf = nullptr) {
-    using P           = Plan<LA, false, T>;
-    using S           = RowShape<LA, G, T>;
+    using P           = Plan<LA, false, T, PX>;
+    using S           = RowShape<LA, G, T, PX>;
     constexpr int TA  = P::TA;
     constexpr int PPT = P::PPT;
     constexpr int NT  = S::NT;
@@ -330,7 +330,7 @@ __device__ __forceinline__ void row_group(const RowArgsT<T>& p, int64_t wg, T* l
                 for (int m = 0; m < PPT / R0; ++m)
 #pragma unroll
                     for (int r = 0; r < R0; ++r) {
-                        uint32_t c    = (uint32_t)(in_elem<LA, false, T>(jt, m, r) - jt) + (k2 << LA);
+                        uint32_t c    = (uint32_t)(in_elem<LA, false, T, PX>(jt, m, r) - jt) + (k2 << LA);
                         v[m * R0 + r] = src.ld(vo, poff(c, p.src_lp), p.nt_load);
                     }
             } else {
@@ -338,7 +338,7 @@ __device__ __forceinline__ void row_group(const RowArgsT<T>& p, int64_t wg, T* l
                 for (int m = 0; m < PPT / R0; ++m)
 #pragma unroll
                     for (int r = 0; r < R0; ++r) {
-                        uint32_t i    = (uint32_t)in_elem<LA, false, T>(jt, m, r) + (k2 << LA);
+                        uint32_t i    = (uint32_t)in_elem<LA, false, T, PX>(jt, m, r) + (k2 << LA);
                         v[m * R0 + r] = src.ld(poff(i, p.src_lp), p.nt_load);
                     }
             }
@@ -347,7 +347,7 @@ __device__ __forceinline__ void row_group(const RowArgsT<T>& p, int64_t wg, T* l
             for (int m = 0; m < PPT / R0; ++m)
 #pragma unroll
                 for (int r = 0; r < R0; ++r) {
-                    uint32_t i    = (uint32_t)in_elem<LA, false, T>(jt, m, r) + (k2 << LA);
+                    uint32_t i    = (uint32_t)in_elem<LA, false, T, PX>(jt, m, r) + (k2 << LA);
                     bool     ok   = row_ok && i < p.src_size;
                     cx<T>    x    = src.ld(ok ? poff(i, p.src_lp) : 0u, p.nt_load);
                     v[m * R0 + r] = ok ? x : cx<T>{T(0), T(0)};
@@ -359,9 +359,9 @@ __device__ __forceinline__ void row_group(const RowArgsT<T>& p, int64_t wg, T* l
         auto pos = [&](int u) {
             if constexpr (LD == LD_PERM) {
                 if constexpr (TA >= 32)
-                    return paddr(perm_base<LA, ORD, T>(jt), brevc(u, ilog2c(PPT)));
+                    return paddr(perm_base<LA, ORD, T, PX>(jt), brevc(u, ilog2c(PPT)));
                 else
-                    return paddr(perm_elem<LA, ORD, T>(jt, u), 0);
+                    return paddr(perm_elem<LA, ORD, T, PX>(jt, u), 0);
             } else {
                 int idx = t + u * NT;
                 int k1  = idx / G;
@@ -400,13 +400,13 @@ __device__ __forceinline__ void row_group(const RowArgsT<T>& p, int64_t wg, T* l
         for (int u = 0; u < PPT; ++u)
             lre[pos(u)] = v[u].x;
         __syncthreads();
-        stage_read<LA, false, 0, 0>(v, lre, lim, jt, rd);
+        stage_read<LA, false, 0, 0, PX>(v, lre, lim, jt, rd);
         __syncthreads();
 #pragma unroll
         for (int u = 0; u < PPT; ++u)
             lre[pos(u)] = v[u].y;
         __syncthreads();
-        stage_read<LA, false, 0, 1>(v, lre, lim, jt, rd);
+        stage_read<LA, false, 0, 1, PX>(v, lre, lim, jt, rd);
     } else {
         if constexpr (LD == LD_PERM) {
             // this row's spectrum occupies slots [A*brev(k2), A*brev(k2) + A); local slot s holds
@@ -424,9 +424,9 @@ __device__ __forceinline__ void row_group(const RowArgsT<T>& p, int64_t wg, T* l
             for (int u = 0; u < PPT; ++u) {
                 int o;
                 if constexpr (TA >= 32)
-                    o = paddr(perm_base<LA, ORD, T>(jt), brevc(u, ilog2c(PPT)));
+                    o = paddr(perm_base<LA, ORD, T, PX>(jt), brevc(u, ilog2c(PPT)));
                 else
-                    o = paddr(perm_elem<LA, ORD, T>(jt, u), 0);
+                    o = paddr(perm_elem<LA, ORD, T, PX>(jt, u), 0);
                 lre[o] = v[u].x;
                 lim[o] = v[u].y;
             }
@@ -442,7 +442,7 @@ __device__ __forceinline__ void row_group(const RowArgsT<T>& p, int64_t wg, T* l
             for (int m = 0; m < PPT / R0; ++m)
 #pragma unroll
                 for (int r = 0; r < R0; ++r) {
-                    const uint32_t k1 = (uint32_t)in_elem<LA, false, T>(ja, m, r);
+                    const uint32_t k1 = (uint32_t)in_elem<LA, false, T, PX>(ja, m, r);
                     cx<T>          x  = src.ld(ok ? poff(kb + (k1 << p.rows_lg), p.src_lp) : 0u, p.nt_load);
                     v[m * R0 + r]     = ok ? x : cx<T>{T(0), T(0)};
                 }
@@ -483,13 +483,13 @@ __device__ __forceinline__ void row_group(const RowArgsT<T>& p, int64_t wg, T* l
     // ------------------------------------------------------------------ transform
     // (the barrier before the first stage's LDS writes also orders them after the reads above)
     if constexpr (ALT_LD)
-        stockham<LA, false, decltype(addr), PREC, T, 0, true, false, decltype(addrA), decltype(addrA)>(
+        stockham<LA, false, decltype(addr), PREC, T, PX, true, false, decltype(addrA), decltype(addrA)>(
             v, lre, lim, jt, addr, ltw, mf, &mapA, &mapA);
     else if constexpr (ALT_ST)
-        stockham<LA, false, decltype(addr), PREC, T, 0, false, true, decltype(addrA), decltype(addrA)>(
+        stockham<LA, false, decltype(addr), PREC, T, PX, false, true, decltype(addrA), decltype(addrA)>(
             v, lre, lim, jt, addr, ltw, mf, &mapA, &mapA);
     else
-        stockham<LA, false, decltype(addr), PREC, T>(v, lre, lim, jt, addr, ltw, mf);
+        stockham<LA, false, decltype(addr), PREC, T, PX>(v, lre, lim, jt, addr, ltw, mf);
     // the thread's outputs: butterfly index and row of the last stage's assignment
     const int      jo  = ALT_ST ? ja : jt;
     const uint32_t ko2 = ALT_ST ? (uint32_t)((row0 + ga) & rmask) : k2;
@@ -499,7 +499,7 @@ __device__ __forceinline__ void row_group(const RowArgsT<T>& p, int64_t wg, T* l
         for (int m = 0; m < PPT / RL; ++m)
 #pragma unroll
             for (int r = 0; r < RL; ++r) {
-                uint32_t e    = (uint32_t)out_elem<LA, false, T>(jo, m, r) * ko2;
+                uint32_t e    = (uint32_t)out_elem<LA, false, T, PX>(jo, m, r) * ko2;
                 v[m * RL + r] = cmul(v[m * RL + r], PassTw<TW, T>::get(lptw, p.twl, p.twh, p.tw_ls, p.tw_lg, e));
                 __builtin_amdgcn_sched_barrier(0); // one element's table reads in flight at a time
             }
@@ -526,7 +526,7 @@ __device__ __forceinline__ void row_group(const RowArgsT<T>& p, int64_t wg, T* l
                 for (int m = 0; m < PPT / RL; ++m)
 #pragma unroll
                     for (int r = 0; r < RL; ++r) {
-                        uint32_t c = (uint32_t)(out_elem<LA, false, T>(jt, m, r) - jt) + (k2 << LA);
+                        uint32_t c = (uint32_t)(out_elem<LA, false, T, PX>(jt, m, r) - jt) + (k2 << LA);
                         dst.st(vo, poff(c, p.dst_lp), v[m * RL + r], p.nt_store);
                     }
             } else {
@@ -534,7 +534,7 @@ __device__ __forceinline__ void row_group(const RowArgsT<T>& p, int64_t wg, T* l
                 for (int m = 0; m < PPT / RL; ++m)
 #pragma unroll
                     for (int r = 0; r < RL; ++r) {
-                        uint32_t i = (uint32_t)out_elem<LA, false, T>(jt, m, r) + (k2 << LA);
+                        uint32_t i = (uint32_t)out_elem<LA, false, T, PX>(jt, m, r) + (k2 << LA);
                         dst.st(poff(i, p.dst_lp), v[m * RL + r], p.nt_store);
                     }
             }
@@ -544,9 +544,9 @@ __device__ __forceinline__ void row_group(const RowArgsT<T>& p, int64_t wg, T* l
         auto pos = [&](int u) {
             if constexpr (ST == ST_PERM) {
                 if constexpr (TA >= 32)
-                    return paddr(perm_base<LA, ORD, T>(jt), brevc(u, ilog2c(PPT)));
+                    return paddr(perm_base<LA, ORD, T, PX>(jt), brevc(u, ilog2c(PPT)));
                 else
-                    return paddr(perm_elem<LA, ORD, T>(jt, u), 0);
+                    return paddr(perm_elem<LA, ORD, T, PX>(jt, u), 0);
             } else {
                 int idx = t + u * NT;
                 int k1  = idx / G;
@@ -603,7 +603,7 @@ __device__ __forceinline__ void row_group(const RowArgsT<T>& p, int64_t wg, T* l
             for (int m = 0; m < PPT / RL; ++m)
 #pragma unroll
                 for (int r = 0; r < RL; ++r) {
-                    const uint32_t k1 = (uint32_t)out_elem<LA, false, T>(ja, m, r);
+                    const uint32_t k1 = (uint32_t)out_elem<LA, false, T, PX>(ja, m, r);
                     dst.st(poff(ko2 + (k1 << p.rows_lg), p.dst_lp), v[m * RL + r], p.nt_store);
                 }
         }
@@ -627,9 +627,9 @@ __device__ __forceinline__ void row_group(const RowArgsT<T>& p, int64_t wg, T* l
                     int s = jt + u * TA;
                     int o;
                     if constexpr (TA >= 32)
-                        o = paddr(perm_base<LA, ORD, T>(jt), brevc(u, ilog2c(PPT)));
+                        o = paddr(perm_base<LA, ORD, T, PX>(jt), brevc(u, ilog2c(PPT)));
                     else
-                        o = paddr(perm_elem<LA, ORD, T>(jt, u), 0);
+                        o = paddr(perm_elem<LA, ORD, T, PX>(jt, u), 0);
                     dst.st(poff(base + s, p.dst_lp), cx<T>{lre[o], lim[o]}, p.nt_store);
                 }
             }

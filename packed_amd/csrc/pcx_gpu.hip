@@ -1179,7 +1179,7 @@ extern "C" int pcx_generate_f64(double* dst, int64_t pack, int64_t n, uint64_t s
 // ---------------------------------------------------------------- uneven-batch work queue
 namespace {
 
-struct QGroup { // transforms of one size and direction outside the persistent kernel
+struct QGroup { // transforms of one size above 2^20 and one direction: grouped four-step launches
     int      lg   = 0;
     int      inv  = 0;
     int64_t  count = 0;
@@ -1194,16 +1194,24 @@ struct pcx_queue_s {
     int                 normalized = 1;
     int64_t             njobs      = 0;
     double              flops = 0, bytes = 0;
-    // persistent part
-    int32_t             nitems    = 0;
-    QItem*              items     = nullptr; // device
+    // persistent part: every transform of 2^1 .. 2^20 points
+    int32_t             nitems     = 0;
+    QItem*              items      = nullptr; // device
     int64_t*            q_src_offs = nullptr;
     int64_t*            q_dst_offs = nullptr;
-    uint32_t*           counter   = nullptr;
-    float2*             tw        = nullptr; // W_A for A = 2 .. 4096
-    int                 grid      = 1024;
-    // grouped part
+    uint32_t*           counter    = nullptr;
+    float2*             tw         = nullptr; // W_A for A = 2 .. 2^14
+    float2*             ptw        = nullptr; // pass tables of 2^15 .. 2^20
+    int32_t             ptw_off[kQueueMaxLog2 + 1] = {};
+    float*              tmp        = nullptr; // four-step intermediates, one per workgroup
+    int64_t             tmp_floats = 0;
+    int                 grid       = 0;       // workgroups of the persistent launch
+    int                 grid_cap   = 0;       // workgroups the intermediates were sized for
+    // grouped part: sizes above 2^20, and the classes the autotuner moved out of the persistent
+    // launch (grouped[log2n][inverse])
     std::vector<QGroup> groups;
+    bool                grouped[kMaxLog2 + 1][2] = {};
+    std::vector<pcx_job> jobs; // the job list, to rebuild the items when the split changes
     // captured graph
     hipGraph_t          graph = nullptr;
     hipGraphExec_t      exec  = nullptr;
@@ -1215,10 +1223,12 @@ int queue_run(pcx_queue_t q, float* dst, const float* src, hipStream_t st) {
     if (q->nitems > 0) {
         launch_reset_counter(q->counter, st);
         QueueArgs a{};
-        a.src = src, a.dst = dst, a.src_offs = q->q_src_offs, a.dst_offs = q->q_dst_offs;
-        a.items = q->items, a.nitems = q->nitems, a.normalized = q->normalized, a.counter = q->counter, a.tw = q->tw;
-        int grid = q->grid < q->nitems ? q->grid : q->nitems;
-        launch_queue(a, grid, st);
+        a.src = src, a.dst = dst, a.tmp = q->tmp, a.tmp_floats = q->tmp_floats;
+        a.src_offs = q->q_src_offs, a.dst_offs = q->q_dst_offs;
+        a.items = q->items, a.nitems = q->nitems, a.normalized = q->normalized, a.counter = q->counter;
+        a.tw = q->tw, a.ptw = q->ptw;
+        std::memcpy(a.ptw_off, q->ptw_off, sizeof(a.ptw_off));
+        launch_queue(a, std::min<int>(q->grid, q->nitems), st);
         HIPCHK(hipGetLastError());
     }
     for (auto& g : q->groups) {
@@ -1230,22 +1240,136 @@ int queue_run(pcx_queue_t q, float* dst, const float* src, hipStream_t st) {
     return PCX_SUCCESS;
 }
 
-void queue_free(pcx_queue_t q) {
+// Frees the device side of the item list and the groups (the job list and settings stay).
+void queue_release(pcx_queue_t q) {
     if (q->exec)
         (void)hipGraphExecDestroy(q->exec);
     if (q->graph)
         (void)hipGraphDestroy(q->graph);
+    q->exec = nullptr, q->graph = nullptr;
     (void)hipFree(q->items);
     (void)hipFree(q->q_src_offs);
     (void)hipFree(q->q_dst_offs);
     (void)hipFree(q->counter);
     (void)hipFree(q->tw);
+    (void)hipFree(q->ptw);
+    (void)hipFree(q->tmp);
+    q->items = nullptr, q->q_src_offs = q->q_dst_offs = nullptr, q->counter = nullptr;
+    q->tw = q->ptw = nullptr, q->tmp = nullptr, q->tmp_floats = 0, q->nitems = 0;
     for (auto& g : q->groups) {
         (void)hipFree(g.src_offs);
         (void)hipFree(g.dst_offs);
         pcx_plan_destroy(g.plan);
     }
+    q->groups.clear();
+}
+
+void queue_free(pcx_queue_t q) {
+    queue_release(q);
     delete q;
+}
+
+// Builds the persistent items (every class up to 2^20 not marked grouped, largest first) and the
+// grouped launches (the rest, largest first) from the job list.
+int queue_build(pcx_queue_t q) {
+    const pcx_job* jobs  = q->jobs.data();
+    const int64_t  njobs = (int64_t)q->jobs.size();
+    std::vector<std::vector<int64_t>> bucket((kMaxLog2 + 1) * 2);
+    for (int64_t i = 0; i < njobs; ++i)
+        bucket[jobs[i].log2n * 2 + jobs[i].inverse].push_back(i);
+    std::vector<QItem>   items;
+    std::vector<int64_t> qs, qd;
+    int                  max_four = 0;
+    // persistent items, largest size first (longest-processing-time order)
+    for (int lg = kQueueMaxLog2; lg >= 1; --lg) {
+        const int per = lg >= kQueueMinFour ? 1 : kQueueThreads / (lg >= 4 ? (1 << lg) / 16 : 1);
+        for (int inv = 0; inv < 2; ++inv) {
+            if (q->grouped[lg][inv])
+                continue;
+            auto& b = bucket[lg * 2 + inv];
+            for (size_t k = 0; k < b.size(); k += per) {
+                int cnt = (int)std::min<size_t>(per, b.size() - k);
+                items.push_back({lg >= kQueueMinFour ? QK_FOUR : QK_SINGLE, lg, inv, (int32_t)qs.size(), cnt});
+                for (int c = 0; c < cnt; ++c) {
+                    qs.push_back(jobs[b[k + c]].src_offset);
+                    qd.push_back(jobs[b[k + c]].dst_offset);
+                }
+                if (lg >= kQueueMinFour)
+                    max_four = std::max(max_four, lg);
+            }
+        }
+    }
+    q->nitems = (int32_t)items.size();
+    hipStream_t setup = nullptr;
+    hipError_t  he    = setup_stream(&setup);
+    if (!items.empty() && he == hipSuccess) {
+        he = hipMalloc(&q->items, sizeof(QItem) * items.size());
+        if (he == hipSuccess)
+            he = hipMalloc(&q->q_src_offs, sizeof(int64_t) * qs.size());
+        if (he == hipSuccess)
+            he = hipMalloc(&q->q_dst_offs, sizeof(int64_t) * qd.size());
+        if (he == hipSuccess)
+            he = hipMalloc(&q->counter, sizeof(uint32_t));
+        if (he == hipSuccess)
+            he = hipMalloc(&q->tw, sizeof(float2) * ((1 << (kQueueMaxSingle + 1)) - 2));
+        if (he == hipSuccess)
+            he = hipMemcpy(q->items, items.data(), sizeof(QItem) * items.size(), hipMemcpyHostToDevice);
+        if (he == hipSuccess)
+            he = hipMemcpy(q->q_src_offs, qs.data(), sizeof(int64_t) * qs.size(), hipMemcpyHostToDevice);
+        if (he == hipSuccess)
+            he = hipMemcpy(q->q_dst_offs, qd.data(), sizeof(int64_t) * qd.size(), hipMemcpyHostToDevice);
+        for (int lg = 1; lg <= kQueueMaxSingle && he == hipSuccess; ++lg)
+            he = gen_twiddles(q->tw + ((1 << lg) - 2), (int64_t)1 << lg, 1, (int64_t)1 << lg, setup);
+        if (max_four > 0 && he == hipSuccess) {
+            // pass tables W_N^e = lo[e mod 2^ls] * hi[e >> ls] of every four-step size, and one
+            // intermediate of the largest size per workgroup
+            int64_t total = 0;
+            for (int lg = kQueueMinFour; lg <= max_four; ++lg) {
+                q->ptw_off[lg] = (int32_t)total;
+                total += ((int64_t)1 << ((lg + 1) / 2)) + ((int64_t)1 << (lg - (lg + 1) / 2));
+            }
+            he = hipMalloc(&q->ptw, sizeof(float2) * total);
+            for (int lg = kQueueMinFour; lg <= max_four && he == hipSuccess; ++lg) {
+                const int     ls = (lg + 1) / 2;
+                const int64_t nl = (int64_t)1 << ls, nh = ((int64_t)1 << lg) >> ls;
+                he = gen_twiddles(q->ptw + q->ptw_off[lg], (int64_t)1 << lg, 1, nl, setup);
+                if (he == hipSuccess)
+                    he = gen_twiddles(q->ptw + q->ptw_off[lg] + nl, (int64_t)1 << lg, nl, nh, setup);
+            }
+            q->tmp_floats = (int64_t)2 << max_four;
+            if (he == hipSuccess)
+                he = hipMalloc(&q->tmp, sizeof(float) * q->tmp_floats * q->grid_cap);
+        }
+    }
+    // grouped launches, largest first
+    for (int lg = kMaxLog2; lg >= 1 && he == hipSuccess; --lg) {
+        for (int inv = 0; inv < 2; ++inv) {
+            auto& b = bucket[lg * 2 + inv];
+            if (b.empty() || (lg <= kQueueMaxLog2 && !q->grouped[lg][inv]))
+                continue;
+            QGroup g;
+            g.lg = lg, g.inv = inv, g.count = (int64_t)b.size();
+            std::vector<int64_t> so(b.size()), dof(b.size());
+            for (size_t k = 0; k < b.size(); ++k)
+                so[k] = jobs[b[k]].src_offset, dof[k] = jobs[b[k]].dst_offset;
+            he = hipMalloc(&g.src_offs, sizeof(int64_t) * b.size());
+            if (he == hipSuccess)
+                he = hipMalloc(&g.dst_offs, sizeof(int64_t) * b.size());
+            if (he == hipSuccess)
+                he = hipMemcpy(g.src_offs, so.data(), sizeof(int64_t) * b.size(), hipMemcpyHostToDevice);
+            if (he == hipSuccess)
+                he = hipMemcpy(g.dst_offs, dof.data(), sizeof(int64_t) * b.size(), hipMemcpyHostToDevice);
+            q->groups.push_back(g);
+            if (he == hipSuccess)
+                if (int e = pcx_plan_create((int64_t)1 << lg, 2048, PCX_ORDER_NORMAL, &q->groups.back().plan))
+                    return e;
+        }
+    }
+    if (he == hipSuccess)
+        he = hipStreamSynchronize(setup);
+    if (he != hipSuccess)
+        return fail(PCX_ERR_HIP, std::string("queue setup: ") + hipGetErrorString(he));
+    return PCX_SUCCESS;
 }
 
 } // namespace
@@ -1264,89 +1388,23 @@ extern "C" int pcx_queue_create(const pcx_job* jobs, int64_t njobs, int normaliz
         if (jobs[i].src_offset < 0 || jobs[i].dst_offset < 0)
             return fail(PCX_ERR_INVALID_ARGUMENT, "job " + s(i) + ": negative offset");
     }
+    int dev = 0, ncu = 0;
+    HIPCHK(hipGetDevice(&dev));
+    HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
     auto* q       = new pcx_queue_s;
     q->normalized = normalized ? 1 : 0;
     q->njobs      = njobs;
-    // bucket by (size, direction)
-    std::vector<std::vector<int64_t>> bucket((kMaxLog2 + 1) * 2);
+    // one workgroup per CU (the largest item takes most of a CU's LDS); the autotuner may pick fewer
+    q->grid = q->grid_cap = std::max(1, ncu);
     for (int64_t i = 0; i < njobs; ++i) {
-        bucket[jobs[i].log2n * 2 + jobs[i].inverse].push_back(i);
         double n = (double)((int64_t)1 << jobs[i].log2n);
         q->flops += 5.0 * n * (double)jobs[i].log2n;
         q->bytes += 16.0 * n;
     }
-    std::vector<QItem>   items;
-    std::vector<int64_t> qs, qd;
-    hipStream_t          setup = nullptr;
-    hipError_t           he    = setup_stream(&setup);
-    // persistent items, largest size first (longest-processing-time order)
-    for (int lg = kQueueMaxLog2; lg >= 1; --lg) {
-        int per = kQueueThreads / (lg >= 4 ? (1 << lg) / 16 : 1);
-        for (int inv = 0; inv < 2; ++inv) {
-            auto& b = bucket[lg * 2 + inv];
-            for (size_t k = 0; k < b.size(); k += per) {
-                int cnt = (int)std::min<size_t>(per, b.size() - k);
-                items.push_back({lg, inv, (int32_t)qs.size(), cnt});
-                for (int c = 0; c < cnt; ++c) {
-                    qs.push_back(jobs[b[k + c]].src_offset);
-                    qd.push_back(jobs[b[k + c]].dst_offset);
-                }
-            }
-        }
-    }
-    q->nitems = (int32_t)items.size();
-    if (!items.empty()) {
-        he = hipMalloc(&q->items, sizeof(QItem) * items.size());
-        if (he == hipSuccess)
-            he = hipMalloc(&q->q_src_offs, sizeof(int64_t) * qs.size());
-        if (he == hipSuccess)
-            he = hipMalloc(&q->q_dst_offs, sizeof(int64_t) * qd.size());
-        if (he == hipSuccess)
-            he = hipMalloc(&q->counter, sizeof(uint32_t));
-        if (he == hipSuccess)
-            he = hipMalloc(&q->tw, sizeof(float2) * ((1 << (kQueueMaxLog2 + 1)) - 2));
-        if (he == hipSuccess)
-            he = hipMemcpy(q->items, items.data(), sizeof(QItem) * items.size(), hipMemcpyHostToDevice);
-        if (he == hipSuccess)
-            he = hipMemcpy(q->q_src_offs, qs.data(), sizeof(int64_t) * qs.size(), hipMemcpyHostToDevice);
-        if (he == hipSuccess)
-            he = hipMemcpy(q->q_dst_offs, qd.data(), sizeof(int64_t) * qd.size(), hipMemcpyHostToDevice);
-        for (int lg = 1; lg <= kQueueMaxLog2 && he == hipSuccess; ++lg)
-            he = gen_twiddles(q->tw + ((1 << lg) - 2), (int64_t)1 << lg, 1, (int64_t)1 << lg, setup);
-    }
-    // grouped launches for the larger sizes, largest first
-    for (int lg = kMaxLog2; lg > kQueueMaxLog2 && he == hipSuccess; --lg) {
-        for (int inv = 0; inv < 2; ++inv) {
-            auto& b = bucket[lg * 2 + inv];
-            if (b.empty())
-                continue;
-            QGroup g;
-            g.lg = lg, g.inv = inv, g.count = (int64_t)b.size();
-            std::vector<int64_t> so(b.size()), dof(b.size());
-            for (size_t k = 0; k < b.size(); ++k)
-                so[k] = jobs[b[k]].src_offset, dof[k] = jobs[b[k]].dst_offset;
-            he = hipMalloc(&g.src_offs, sizeof(int64_t) * b.size());
-            if (he == hipSuccess)
-                he = hipMalloc(&g.dst_offs, sizeof(int64_t) * b.size());
-            if (he == hipSuccess)
-                he = hipMemcpy(g.src_offs, so.data(), sizeof(int64_t) * b.size(), hipMemcpyHostToDevice);
-            if (he == hipSuccess)
-                he = hipMemcpy(g.dst_offs, dof.data(), sizeof(int64_t) * b.size(), hipMemcpyHostToDevice);
-            if (he == hipSuccess) {
-                if (int e = pcx_plan_create((int64_t)1 << lg, 2048, PCX_ORDER_NORMAL, &g.plan)) {
-                    q->groups.push_back(g);
-                    queue_free(q);
-                    return e;
-                }
-            }
-            q->groups.push_back(g);
-        }
-    }
-    if (he == hipSuccess)
-        he = hipStreamSynchronize(setup);
-    if (he != hipSuccess) {
+    q->jobs.assign(jobs, jobs + njobs);
+    if (int e = queue_build(q)) {
         queue_free(q);
-        return fail(PCX_ERR_HIP, std::string("queue setup: ") + hipGetErrorString(he));
+        return e;
     }
     *out = q;
     return PCX_SUCCESS;
@@ -1456,8 +1514,9 @@ float time_ms(hipStream_t st, F&& fn, int reps = 3) {
 
 } // namespace
 
-// Autotuner: persistent grid size, then per group the four-step shape and grid caps.  Winners are
-// written to cache_path (one "key value..." line each) and read back from it when present.
+// Autotuner: workgroups of the persistent launch, then per grouped size the four-step shape and
+// grid caps.  Winners are written to cache_path (one "key value..." line each) and read back
+// from it when present.
 extern "C" int pcx_queue_autotune(pcx_queue_t q, float* dst, const float* src, void* stream, const char* cache_path) {
     if (!q)
         return fail(PCX_ERR_INVALID_ARGUMENT, "queue is null");
@@ -1482,10 +1541,29 @@ extern "C" int pcx_queue_autotune(pcx_queue_t q, float* dst, const float* src, v
             char key[32];
             int  a = 0, b = 0, c = 0, d = 0, e = 0;
             long long gc = 0, gr = 0;
+            // the split between the persistent launch and grouped launches comes first
+            bool mask[kMaxLog2 + 1][2] = {};
+            long pos = std::ftell(f);
             while (std::fscanf(f, "%31s", key) == 1) {
-                if (!std::strcmp(key, "queue_grid") && std::fscanf(f, "%d", &a) == 1)
-                    q->grid = a;
-                else if (!std::strcmp(key, "group") && std::fscanf(f, "%d %d %d %d %d %lld %lld", &a, &b, &c, &d, &e, &gc, &gr) == 7) {
+                if (!std::strcmp(key, "grouped") && std::fscanf(f, "%d %d", &a, &b) == 2 && a >= 1 && a <= kQueueMaxLog2 &&
+                    (b == 0 || b == 1))
+                    mask[a][b] = true;
+            }
+            std::fseek(f, pos, SEEK_SET);
+            if (std::memcmp(mask, q->grouped, sizeof(mask)) != 0) {
+                std::memcpy(q->grouped, mask, sizeof(mask));
+                queue_release(q);
+                if (int err = queue_build(q)) {
+                    std::fclose(f);
+                    return err;
+                }
+            }
+            while (std::fscanf(f, "%31s", key) == 1) {
+                if (!std::strcmp(key, "grouped") && std::fscanf(f, "%d %d", &a, &b) == 2) {
+                    continue;
+                } else if (!std::strcmp(key, "queue_grid") && std::fscanf(f, "%d", &a) == 1) {
+                    q->grid = std::max(1, std::min(a, q->grid_cap));
+                } else if (!std::strcmp(key, "group") && std::fscanf(f, "%d %d %d %d %d %lld %lld", &a, &b, &c, &d, &e, &gc, &gr) == 7) {
                     for (auto& g : q->groups)
                         if (g.lg == a && g.inv == b && g.plan->two_pass) {
                             if (const FourStep* sh = find_shape(a, c, d, e)) {
@@ -1507,7 +1585,9 @@ extern "C" int pcx_queue_autotune(pcx_queue_t q, float* dst, const float* src, v
     if (q->nitems > 0) {
         float best = 1e30f;
         int   bg   = q->grid;
-        for (int grid : {256, 512, 1024, 2048, 4096}) {
+        for (int grid : {q->grid_cap / 2, (3 * q->grid_cap) / 4, q->grid_cap}) {
+            if (grid < 1)
+                continue;
             q->grid  = grid;
             float ms = time_ms(st, [&] {
                 std::vector<QGroup> keep;
@@ -1520,6 +1600,45 @@ extern "C" int pcx_queue_autotune(pcx_queue_t q, float* dst, const float* src, v
                 best = ms, bg = grid;
         }
         q->grid = bg;
+    }
+    // which size classes run in the persistent launch and which as grouped launches after it:
+    // each four-step class (largest first), then the single-pass classes together, moves out when
+    // that makes the whole pass faster
+    {
+        bool present[kMaxLog2 + 1] = {};
+        for (const auto& j : q->jobs)
+            present[j.log2n] = true;
+        auto whole = [&] { return queue_run(q, dst, src, st); };
+        auto rebuild = [&]() -> int {
+            queue_release(q);
+            return queue_build(q);
+        };
+        float best = time_ms(st, whole);
+        auto  trial = [&](int lo, int hi) -> int {
+            bool any = false;
+            for (int lg = lo; lg <= hi; ++lg)
+                any = any || (present[lg] && !q->grouped[lg][0]);
+            if (!any)
+                return PCX_SUCCESS;
+            bool keep[kMaxLog2 + 1][2];
+            std::memcpy(keep, q->grouped, sizeof(keep));
+            for (int lg = lo; lg <= hi; ++lg)
+                q->grouped[lg][0] = q->grouped[lg][1] = present[lg];
+            if (int e = rebuild())
+                return e;
+            const float ms = time_ms(st, whole);
+            if (ms < best) {
+                best = ms;
+                return PCX_SUCCESS;
+            }
+            std::memcpy(q->grouped, keep, sizeof(keep));
+            return rebuild();
+        };
+        for (int lg = kQueueMaxLog2; lg >= kQueueMinFour; --lg)
+            if (int e = trial(lg, lg))
+                return e;
+        if (int e = trial(1, kQueueMaxSingle))
+            return e;
     }
     for (auto& g : q->groups) {
         if (!g.plan->two_pass)
@@ -1553,6 +1672,10 @@ extern "C" int pcx_queue_autotune(pcx_queue_t q, float* dst, const float* src, v
     }
     if (cache_path) {
         if (FILE* f = std::fopen(cache_path, "w")) {
+            for (int lg = 1; lg <= kQueueMaxLog2; ++lg)
+                for (int inv = 0; inv < 2; ++inv)
+                    if (q->grouped[lg][inv])
+                        std::fprintf(f, "grouped %d %d\n", lg, inv);
             std::fprintf(f, "queue_grid %d\n", q->grid);
             for (auto& g : q->groups)
                 if (g.plan->two_pass)
