@@ -45,25 +45,29 @@ struct XcArgs {
     int64_t      hblk0;              // block whose saved overlap is first in hre / him
 };
 
-template<int LA, int G>
-constexpr int xc_lds_floats() {
-    return RowShape<LA, G>::DATA + TwTable<LA>::FLOATS;
-}
-
-// Points per thread of the correlation kernel: 32 at 2^14 (512 threads; the two transforms per
-// block are bound by the latency of the LDS exchanges, and twice the waves hide more of it:
-// L = 2^24, N = 2^14 runs in 0.139 ms against 0.160 ms with 64 points), else the default rule.
+// Plan options of the correlation kernel.  The two transforms per block are bound by the
+// latency of the LDS exchanges, and more waves per CU hide more of it:
+//  * 2^14: 16 points per thread (1024 threads): L = 2^24, N = 2^14 runs in 0.135 ms against
+//    0.138 ms with 32 points and 0.160 ms with 64; separate real / imaginary exchange arrays,
+//    which leave one workgroup per CU, take 0.18 ms;
+//  * 2^13: the one-array exchange (35 KiB, four workgroups per CU instead of two): 512 channels
+//    of 2^20 samples with N = 2^13 take 2.65 ms against 3.52 ms;
+//  * else the default rule.
 template<int LA>
-constexpr int xc_ppt() {
-    return LA >= 14 ? 32 : 0;
+constexpr int xc_px() {
+    return LA >= 14 ? 16 : (LA == 13 ? kHalfExchange : 0);
 }
 template<int LA>
 constexpr int xc_threads() {
-    return Plan<LA, false, float, xc_ppt<LA>()>::TA;
+    return Plan<LA, false, float, xc_px<LA>()>::TA;
 }
 template<int LA>
 constexpr int xc_rows() {
     return xc_threads<LA>() >= 256 ? 1 : 256 / xc_threads<LA>();
+}
+template<int LA, int G>
+constexpr int xc_lds_floats() {
+    return RowShape<LA, G, float, xc_px<LA>()>::DATA + TwTable<LA>::FLOATS;
 }
 
 // B32: the signal fits 32-bit buffer offsets (one SGPR descriptor, one VGPR per address).
@@ -75,7 +79,7 @@ __global__ __launch_bounds__(G * xc_threads<LA>(),
                                                       ? waves_per_simd(G * xc_threads<LA>(), 4 * xc_lds_floats<LA, G>()) / 2
                                                       : 2))
 void k_fxcorr(XcArgs p) {
-    constexpr int PX   = xc_ppt<LA>();
+    constexpr int PX   = xc_px<LA>();
     using PF           = Plan<LA, false, float, PX>;
     using PI           = Plan<LA, true, float, PX>;
     constexpr int TA   = PF::TA;
@@ -83,12 +87,13 @@ void k_fxcorr(XcArgs p) {
     constexpr int R0   = PF::first_radix;
     constexpr int RL   = PF::last_radix;
     constexpr int RLI  = PI::last_radix;
-    constexpr int STR  = RowShape<LA, G>::STRIDE;
+    using RS           = RowShape<LA, G, float, PX>;
+    constexpr int STR  = RS::STRIDE;
     static_assert(PI::first_radix == RL, "fused multiply needs matching register maps");
     __shared__ float lds[xc_lds_floats<LA, G>()];
     float*           lre = lds;
-    float*           lim = RowShape<LA, G>::im(lds);
-    float*           ltw = lds + RowShape<LA, G>::DATA;
+    float*           lim = RS::im(lds);
+    float*           ltw = lds + RS::DATA;
     TwTable<LA>::fill(ltw, p.twA, threadIdx.x, G * TA);
     __syncthreads();
 

@@ -24,25 +24,29 @@
 
 namespace pcxk {
 
-// PX > 0 overrides the points per thread (the latency-bound fused kernels run 2^14 with 32).
+// Plan options PX: a points-per-thread override in the low byte (the latency-bound fused
+// kernels run 2^14 with 16), and kHalfExchange to take the one-array exchange (Plan::HALF) below
+// the size where it is the default.
+constexpr int kHalfExchange = 0x200;
 template<int LA, bool REV = false, class T = float, int PX = 0>
 struct Plan {
     static constexpr bool F64    = sizeof(T) == 8;
     static constexpr int A       = 1 << LA;
+    static constexpr int PXV     = PX & 0xFF;
     static constexpr int NFULL   = LA / 4;           // radix-16 stages
     static constexpr int REM     = LA % 4;           // bits left for the small-radix stage
     static constexpr int NSTAGES = NFULL + (REM ? 1 : 0);
     // points per thread: more for the largest sizes, so that a 2^13 transform takes 256 threads
     // and a 2^14 transform 256 threads as well (two workgroups share a CU: one loads or stores
     // while the other computes).  FP64 values take two VGPRs each: at most 32 points per thread.
-    static constexpr int PPT     = PX > 0 ? PX
+    static constexpr int PPT     = PXV > 0 ? PXV
                                  : F64  ? (LA >= 13 ? 32 : (LA >= 4 ? 16 : A))
                                         : (LA >= 14 ? 64 : LA >= 13 ? 32 : (LA >= 4 ? 16 : A));
     static constexpr int TA      = A / PPT;          // threads per transform
     // HALF: the stage exchange moves real parts, then imaginary parts, through one A-element LDS
     // array (2^14 FP32 points: 66 KiB instead of 132 KiB, so two workgroups fit a CU; FP64 from
-    // 2^12 points)
-    static constexpr bool HALF   = F64 ? LA >= 12 : LA >= 14;
+    // 2^12 points, or wherever PX asks for it)
+    static constexpr bool HALF   = (PX & kHalfExchange) || (F64 ? LA >= 12 : LA >= 14);
     static constexpr int radix(int s) {
         if (REM == 0)
             return 16;
