@@ -247,6 +247,9 @@ constexpr int waves_per_simd(int threads, int lds_bytes) {
     return w < 1 ? 1 : (w > 8 ? 8 : w);
 }
 
+// Largest rows that take the row-major tile load / store of row_group (below 16 threads per row).
+constexpr int kTileMaxLog2 = 7;
+
 // One workgroup's group of G rows (rows wg*G .. wg*G + G - 1).
 // UNI: the rows of a wavefront share one transform (TA >= 64, or all rows of the workgroup lie
 // in one transform), so its base address is wave-uniform and buffer addressing applies.
@@ -318,9 +321,60 @@ __device__ __forceinline__ void row_group(const RowArgsT<T>& p, int64_t wg, T* l
     auto           addrA  = [&](int b, int o) { return ga * STR + b + (b >> 5) + o + (o >> 5); };
     const StageMap<decltype(addrA)> mapA{ja, addrA};
 
+    // Rows of fewer than 16 threads (2^5 points and below) load and store contiguous rows as a
+    // row-major tile: element e of the group's row gg belongs to thread (gg * A + e) mod NT, so a
+    // wave covers whole rows with consecutive addresses, and LDS hands the threads their
+    // butterflies (a thread of a 2^5 row would otherwise touch 8 bytes of every 128-byte line).
+    // It applies when the rows are whole transforms at a constant distance and the group's tile
+    // fits 32-bit buffer offsets; the buffer range check drops the rows past the batch.
+    constexpr bool TILE  = TA < 16 && !P::HALF && LA <= kTileMaxLog2;
+    constexpr int  TROWS = NT >> LA; // tile rows per pass over the threads
+    const int      te    = t & (P::A - 1);
+    const int      tg    = t >> LA;
+    // A row's footprint in either view (elements poff(0) .. poff(A - 1)); a single row takes it
+    // as its distance, so the range check leaves the tile's other rows out.
+    auto tile_dist = [&](int lp, int64_t dist) {
+        const int64_t foot = (int64_t)poff((uint32_t)(P::A - 1), lp) + 1;
+        return p.nrows == 1 ? foot : (dist >= foot ? dist : (int64_t)-1);
+    };
+    auto tile_ok = [&](const int64_t* offs, int64_t d) {
+        return TILE && offs == nullptr && p.rows_lg == 0 && d > 0 && (int64_t)G * d * (int64_t)sizeof(T) < 0x7FFFFFFF;
+    };
+    const uint32_t tile_n = (uint32_t)(p.nrows - row0 < G ? p.nrows - row0 : G); // rows of this group
+    // range of the group's tile: its valid rows, up to the last element of the last one
+    auto tile_range = [&](int lp, uint32_t d) { return (tile_n - 1) * d + poff((uint32_t)(P::A - 1), lp) + 1; };
+
     cx<T> v[PPT];
     // ------------------------------------------------------------------ load
-    if constexpr (LD == LD_CONTIG) {
+    bool loaded = false;
+    if constexpr (LD == LD_CONTIG && TILE) {
+        const int64_t sd = tile_dist(p.src_lp, p.src_dist);
+        if (tile_ok(p.src_offs, sd)) {
+            const uint32_t dist = (uint32_t)sd;
+            const BufR<T>  src(p.src + row0 * p.src_dist, p.src_lp, tile_range(p.src_lp, dist));
+            const bool     ez = (int64_t)te < p.src_size; // zero extension past src_size
+#pragma unroll
+            for (int u = 0; u < PPT; ++u) {
+                const int gg = tg + u * TROWS;
+                cx<T>     x  = src.ld((uint32_t)gg * dist + poff((uint32_t)te, p.src_lp), p.nt_load);
+                x            = ez ? x : cx<T>{T(0), T(0)};
+                const int o  = gg * STR + te + (te >> 5);
+                lre[o]       = x.x;
+                lim[o]       = x.y;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int m = 0; m < PPT / R0; ++m)
+#pragma unroll
+                for (int r = 0; r < R0; ++r) {
+                    const int o   = addr(jt + m * TA, r * (P::A / R0));
+                    v[m * R0 + r] = cx<T>{lre[o], lim[o]};
+                }
+            loaded = true;
+        }
+    }
+    if (loaded) {
+    } else if constexpr (LD == LD_CONTIG) {
         const RdView<UNI, T> src(tbase(p.src, p.src_offs, p.src_dist, btu), p.src_lp);
         if (row_ok && (int64_t)(k2 + 1) << LA <= p.src_size) { // whole row inside the source
             if constexpr (P::HALF) {
@@ -482,14 +536,16 @@ __device__ __forceinline__ void row_group(const RowArgsT<T>& p, int64_t wg, T* l
     }
     // ------------------------------------------------------------------ transform
     // (the barrier before the first stage's LDS writes also orders them after the reads above)
+    // (jt = t % TA: butterfly index jt of a transform that fits a wave sits in lane jt % 64)
+    using AA = decltype(addrA);
     if constexpr (ALT_LD)
-        stockham<LA, false, decltype(addr), PREC, T, PX, true, false, decltype(addrA), decltype(addrA)>(
-            v, lre, lim, jt, addr, ltw, mf, &mapA, &mapA);
+        stockham<LA, false, decltype(addr), PREC, T, PX, true, false, AA, AA, true>(v, lre, lim, jt, addr, ltw, mf,
+                                                                                  &mapA, &mapA);
     else if constexpr (ALT_ST)
-        stockham<LA, false, decltype(addr), PREC, T, PX, false, true, decltype(addrA), decltype(addrA)>(
-            v, lre, lim, jt, addr, ltw, mf, &mapA, &mapA);
+        stockham<LA, false, decltype(addr), PREC, T, PX, false, true, AA, AA, true>(v, lre, lim, jt, addr, ltw, mf,
+                                                                                  &mapA, &mapA);
     else
-        stockham<LA, false, decltype(addr), PREC, T, PX>(v, lre, lim, jt, addr, ltw, mf);
+        stockham<LA, false, decltype(addr), PREC, T, PX, false, false, AA, AA, true>(v, lre, lim, jt, addr, ltw, mf);
     // the thread's outputs: butterfly index and row of the last stage's assignment
     const int      jo  = ALT_ST ? ja : jt;
     const uint32_t ko2 = ALT_ST ? (uint32_t)((row0 + ga) & rmask) : k2;
@@ -517,7 +573,33 @@ __device__ __forceinline__ void row_group(const RowArgsT<T>& p, int64_t wg, T* l
             v[q].y = -v[q].y;
     }
     // ------------------------------------------------------------------ store
-    if constexpr (ST == ST_CONTIG) {
+    bool stored = false;
+    if constexpr (ST == ST_CONTIG && TILE) {
+        const int64_t dd = tile_dist(p.dst_lp, p.dst_dist);
+        if (tile_ok(p.dst_offs, dd)) {
+            __syncthreads(); // the last stage's LDS reads are done
+#pragma unroll
+            for (int m = 0; m < PPT / RL; ++m)
+#pragma unroll
+                for (int r = 0; r < RL; ++r) {
+                    const int o = addr(jt + m * TA, r * (P::A / RL));
+                    lre[o]      = v[m * RL + r].x;
+                    lim[o]      = v[m * RL + r].y;
+                }
+            __syncthreads();
+            const uint32_t dist = (uint32_t)dd;
+            const BufW<T>  dst(p.dst + row0 * p.dst_dist, p.dst_lp, tile_range(p.dst_lp, dist));
+#pragma unroll
+            for (int u = 0; u < PPT; ++u) {
+                const int gg = tg + u * TROWS;
+                const int o  = gg * STR + te + (te >> 5);
+                dst.st((uint32_t)gg * dist + poff((uint32_t)te, p.dst_lp), cx<T>{lre[o], lim[o]}, p.nt_store);
+            }
+            stored = true;
+        }
+    }
+    if (stored) {
+    } else if constexpr (ST == ST_CONTIG) {
         if (row_ok) {
             const WrView<UNI, T> dst(tbase(p.dst, p.dst_offs, p.dst_dist, btu), p.dst_lp);
             if constexpr (P::HALF) {

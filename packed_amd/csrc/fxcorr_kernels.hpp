@@ -153,7 +153,8 @@ void k_fxcorr(XcArgs p) {
                 v[m * R0 + r] = x;
             }
     }
-    stockham<LA, false, decltype(addr), PREC, float, PX>(v, lre, lim, jt, addr, ltw, &mf);
+    stockham<LA, false, decltype(addr), PREC, float, PX, false, false, decltype(addr), decltype(addr), true>(
+        v, lre, lim, jt, addr, ltw, &mf);
     {
         // Y = conj(X) * K.  The spectrum is read in two halves: the second half's offset depends
         // (opaquely) on the first half's products, so at most half of it is in registers beside v.

@@ -153,6 +153,58 @@ __device__ __forceinline__ void stage_write(const cx<T>* v, T* lre, T* lim, int 
     }
 }
 
+// ---------------------------------------------------------------------------------------------
+// In-register exchange inside a wavefront.  Exchanging lane bit LB with a register-index bit
+// (the value of lane l in register q and the value of lane l ^ 2^LB in register q ^ 2^i trade
+// places when the two bits differ) moves each pair (a, b) = (v[q], v[q | 2^i]) of a lane with one
+// v_permlane32_swap (LB = 5) or v_permlane16_swap (LB = 4) per dword.  Both sizes it serves,
+// 2^9 (lane bit 4) and 2^10 (lane bits 4 and 5), then skip one of their two LDS exchanges.
+// Lane bits 0..3 would need a partner fetch (DPP or ds_swizzle) and selects, three or four VALU
+// instructions per dword and bit: on MI355X that lost to the LDS exchange (2^6: 1.18 against
+// 0.95 ms, 2^7: 0.95 against 0.84 ms per 2 GiB batch; 2^5 unchanged), so those sizes keep LDS.
+constexpr bool lane_exchange_on(int la) { return la == 9 || la == 10; }
+
+template<int LB>
+__device__ __forceinline__ void lane_swap(uint32_t& a, uint32_t& b) {
+    static_assert(LB == 4 || LB == 5, "half-row / half-wave swaps only");
+    if constexpr (LB == 5) {
+        auto w = __builtin_amdgcn_permlane32_swap(a, b, false, false);
+        a      = w[0];
+        b      = w[1];
+    } else {
+        auto w = __builtin_amdgcn_permlane16_swap(a, b, false, false);
+        a      = w[0];
+        b      = w[1];
+    }
+}
+template<int LB, class T>
+__device__ __forceinline__ void lane_swap(T& a, T& b) {
+    if constexpr (sizeof(T) == 4) {
+        uint32_t x = __builtin_bit_cast(uint32_t, a), y = __builtin_bit_cast(uint32_t, b);
+        lane_swap<LB>(x, y);
+        a = __builtin_bit_cast(T, x), b = __builtin_bit_cast(T, y);
+    } else {
+        uint2 x = __builtin_bit_cast(uint2, a), y = __builtin_bit_cast(uint2, b);
+        lane_swap<LB>(x.x, y.x);
+        lane_swap<LB>(x.y, y.y);
+        a = __builtin_bit_cast(T, x), b = __builtin_bit_cast(T, y);
+    }
+}
+// Lane bits LT - LR + i <-> register bits i (i < LR) of the PPT = 16 values of every lane.
+template<int LT, int LR, class T>
+__device__ __forceinline__ void lane_exchange(cx<T>* v) {
+    sfor<0, LR>([&](auto ii) {
+        constexpr int i  = decltype(ii)::value;
+        constexpr int LB = LT - LR + i;
+#pragma unroll
+        for (int q = 0; q < 16; ++q)
+            if ((q >> i & 1) == 0) {
+                lane_swap<LB>(v[q].x, v[q | 1 << i].x);
+                lane_swap<LB>(v[q].y, v[q | 1 << i].y);
+            }
+    });
+}
+
 // Matrix-core DFT16 of v (FP32 only; the FP64 path never selects it).
 template<int PREC, class T>
 __device__ __forceinline__ void dft16_mfma(const MfmaDft16* mf, cx<T>* v) {
@@ -176,8 +228,12 @@ struct StageMap {
 // stage another assignment of butterflies to threads than the middle stages (jt0, addr): the
 // exchange before the last stage (or after the first) then also transposes, so a strided store
 // (load) needs no LDS round trip of its own.
+// LANES: the caller holds butterfly index jt0 in lane jt0 % 64 (jt0 = threadIdx.x % TA).  With 16
+// points per thread, a small last radix RL and TA <= 64, the exchange before the last stage is
+// then a swap of lane bits [log2(TA/RL), log2 TA) with register bits [0, log2 RL): for 2^9 and
+// 2^10 it runs in registers (lane_exchange), with no LDS round trip and no barrier.
 template<int LA, bool REV, class Addr, int PREC = PREC_FP32, class T = float, int PX = 0, bool FM = false,
-         bool LM = false, class AddrF = Addr, class AddrL = Addr>
+         bool LM = false, class AddrF = Addr, class AddrL = Addr, bool LANES = false>
 __device__ __forceinline__ void stockham(cx<T>* v, T* lre, T* lim, int jt0, const Addr& addr, const T* ltw,
                                          const MfmaDft16* mf = nullptr, const StageMap<AddrF>* first = nullptr,
                                          const StageMap<AddrL>* last = nullptr) {
@@ -186,6 +242,9 @@ __device__ __forceinline__ void stockham(cx<T>* v, T* lre, T* lim, int jt0, cons
     static_assert(!(FM || LM) || (!P::HALF && P::NSTAGES >= 2), "remapped stages need two stages and two arrays");
     constexpr int TA  = P::TA;
     constexpr int PPT = P::PPT;
+    constexpr int RLX = P::radix(P::NSTAGES - 1);
+    constexpr bool RX = LANES && !REV && !P::HALF && !LM && !(FM && P::NSTAGES == 2) && PPT == 16 && TA <= 64 &&
+                        P::NSTAGES >= 2 && RLX < 16 && lane_exchange_on(LA);
     sfor<0, P::NSTAGES>([&](auto ss) {
         constexpr int  s    = decltype(ss)::value;
         constexpr int  R    = P::radix(s);
@@ -204,7 +263,7 @@ __device__ __forceinline__ void stockham(cx<T>* v, T* lre, T* lim, int jt0, cons
             jt = first->jt;
         else
             jt = jt0;
-        if constexpr (s > 0 && !P::HALF) {
+        if constexpr (s > 0 && !P::HALF && !(RX && s + 1 == P::NSTAGES)) {
             __syncthreads(); // previous stage's LDS writes are visible
             if constexpr (MAPL)
                 stage_read<LA, REV, s, 2, PX>(v, lre, lim, jt, last->addr);
@@ -228,7 +287,9 @@ __device__ __forceinline__ void stockham(cx<T>* v, T* lre, T* lim, int jt0, cons
             if constexpr (NB > 1)
                 __builtin_amdgcn_sched_barrier(0);
         }
-        if constexpr (s + 1 < P::NSTAGES) {
+        if constexpr (RX && s + 2 == P::NSTAGES) {
+            lane_exchange<ilog2c(TA), ilog2c(RLX)>(v); // into the last stage's layout
+        } else if constexpr (s + 1 < P::NSTAGES) {
             __syncthreads(); // every read of this stage is done before LDS is overwritten
             if constexpr (!P::HALF) {
                 if constexpr (MAPF)
