@@ -247,8 +247,16 @@ constexpr int waves_per_simd(int threads, int lds_bytes) {
     return w < 1 ? 1 : (w > 8 ? 8 : w);
 }
 
-// Largest rows that take the row-major tile load / store of row_group (below 16 threads per row).
-constexpr int kTileMaxLog2 = 7;
+// Largest rows that take the row-major tile load / store of row_group (at most 16 threads per
+// row), for natural-order and for permuted (bit_reversed / unordered) rows.  Per 2 GiB batch on
+// MI355X: natural order FP32 2^1 .. 2^8 0.75-0.92 ms with it against 0.78-2.95 without, FP64
+// 2^2 .. 2^5 0.77-0.79 against 1.0-2.0 (FP64 2^7, 2^8: 4-6 % slower with it); bit_reversed FP32
+// 2^1 .. 2^5 0.84-1.29 against 1.21-2.43 (2^6 .. 2^8: 1.17-1.33 against 0.80-0.91), FP64
+// 2^1 .. 2^7 0.76-0.88 against 0.84-1.42.
+constexpr int kTileMaxLog2        = 8;
+constexpr int kTileMaxLog2_64     = 5;
+constexpr int kTilePermMaxLog2    = 5;
+constexpr int kTilePermMaxLog2_64 = 7;
 
 // One workgroup's group of G rows (rows wg*G .. wg*G + G - 1).
 // UNI: the rows of a wavefront share one transform (TA >= 64, or all rows of the workgroup lie
@@ -327,8 +335,11 @@ __device__ __forceinline__ void row_group(const RowArgsT<T>& p, int64_t wg, T* l
     // butterflies (a thread of a 2^5 row would otherwise touch 8 bytes of every 128-byte line).
     // It applies when the rows are whole transforms at a constant distance and the group's tile
     // fits 32-bit buffer offsets; the buffer range check drops the rows past the batch.
-    constexpr bool TILE  = TA < 16 && !P::HALF && LA <= kTileMaxLog2;
+    constexpr bool F64T  = sizeof(T) == 8;
+    constexpr bool TILE  = TA <= 16 && !P::HALF && LA <= (F64T ? kTileMaxLog2_64 : kTileMaxLog2);
+    constexpr bool TILEP = TA <= 16 && !P::HALF && LA <= (F64T ? kTilePermMaxLog2_64 : kTilePermMaxLog2);
     constexpr int  TROWS = NT >> LA; // tile rows per pass over the threads
+    static_assert(!(TILE || TILEP) || (NT >= P::A && NT % P::A == 0), "a tile pass covers whole rows");
     const int      te    = t & (P::A - 1);
     const int      tg    = t >> LA;
     // A row's footprint in either view (elements poff(0) .. poff(A - 1)); a single row takes it
@@ -338,7 +349,7 @@ __device__ __forceinline__ void row_group(const RowArgsT<T>& p, int64_t wg, T* l
         return p.nrows == 1 ? foot : (dist >= foot ? dist : (int64_t)-1);
     };
     auto tile_ok = [&](const int64_t* offs, int64_t d) {
-        return TILE && offs == nullptr && p.rows_lg == 0 && d > 0 && (int64_t)G * d * (int64_t)sizeof(T) < 0x7FFFFFFF;
+        return offs == nullptr && p.rows_lg == 0 && d > 0 && (int64_t)G * d * (int64_t)sizeof(T) < 0x7FFFFFFF;
     };
     const uint32_t tile_n = (uint32_t)(p.nrows - row0 < G ? p.nrows - row0 : G); // rows of this group
     // range of the group's tile: its valid rows, up to the last element of the last one
@@ -347,18 +358,20 @@ __device__ __forceinline__ void row_group(const RowArgsT<T>& p, int64_t wg, T* l
     cx<T> v[PPT];
     // ------------------------------------------------------------------ load
     bool loaded = false;
-    if constexpr (LD == LD_CONTIG && TILE) {
+    if constexpr ((LD == LD_CONTIG && TILE) || (LD == LD_PERM && TILEP)) {
         const int64_t sd = tile_dist(p.src_lp, p.src_dist);
         if (tile_ok(p.src_offs, sd)) {
             const uint32_t dist = (uint32_t)sd;
             const BufR<T>  src(p.src + row0 * p.src_dist, p.src_lp, tile_range(p.src_lp, dist));
             const bool     ez = (int64_t)te < p.src_size; // zero extension past src_size
+            // a permuted row holds frequency slot_freq(e) at position e
+            const int      f  = LD == LD_PERM ? (int)slot_freq(ORD, (uint32_t)te, LA) : te;
 #pragma unroll
             for (int u = 0; u < PPT; ++u) {
                 const int gg = tg + u * TROWS;
                 cx<T>     x  = src.ld((uint32_t)gg * dist + poff((uint32_t)te, p.src_lp), p.nt_load);
                 x            = ez ? x : cx<T>{T(0), T(0)};
-                const int o  = gg * STR + te + (te >> 5);
+                const int o  = gg * STR + f + (f >> 5);
                 lre[o]       = x.x;
                 lim[o]       = x.y;
             }
@@ -574,7 +587,7 @@ __device__ __forceinline__ void row_group(const RowArgsT<T>& p, int64_t wg, T* l
     }
     // ------------------------------------------------------------------ store
     bool stored = false;
-    if constexpr (ST == ST_CONTIG && TILE) {
+    if constexpr ((ST == ST_CONTIG && TILE) || (ST == ST_PERM && TILEP)) {
         const int64_t dd = tile_dist(p.dst_lp, p.dst_dist);
         if (tile_ok(p.dst_offs, dd)) {
             __syncthreads(); // the last stage's LDS reads are done
@@ -589,10 +602,11 @@ __device__ __forceinline__ void row_group(const RowArgsT<T>& p, int64_t wg, T* l
             __syncthreads();
             const uint32_t dist = (uint32_t)dd;
             const BufW<T>  dst(p.dst + row0 * p.dst_dist, p.dst_lp, tile_range(p.dst_lp, dist));
+            const int      f = ST == ST_PERM ? (int)slot_freq(ORD, (uint32_t)opaque_after_memory(te), LA) : te;
 #pragma unroll
             for (int u = 0; u < PPT; ++u) {
                 const int gg = tg + u * TROWS;
-                const int o  = gg * STR + te + (te >> 5);
+                const int o  = gg * STR + f + (f >> 5);
                 dst.st((uint32_t)gg * dist + poff((uint32_t)te, p.dst_lp), cx<T>{lre[o], lim[o]}, p.nt_store);
             }
             stored = true;
