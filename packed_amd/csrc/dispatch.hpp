@@ -58,13 +58,14 @@ void launch_row(const RowArgsT<T>& a, int64_t nrows, int64_t grid_cap, hipStream
     hipLaunchKernelGGL((k_row<LA, G, LD, ST, TW, UNI, ORD, PREC, T>), dim3((unsigned)nwg),
                        dim3(G * Plan<LA, false, T>::TA), 0, st, a);
 }
-template<int LA, int C, int TW, int PREC = PREC_FP32, class T = float>
+template<int LA, int C, int TW, int PREC = PREC_FP32, class T = float, int PX = 0>
 void launch_col(const ColArgsT<T>& a, int64_t grid_cap, hipStream_t st) {
-    int64_t nwg = a.ntiles;
+    constexpr int NT  = C * Plan<LA, false, T, PX>::TA;
+    int64_t       nwg = a.ntiles;
     if (grid_cap > 0 && nwg > grid_cap)
         nwg = grid_cap;
-    nwg = fit_grid(grid8(nwg), C * Plan<LA, false, T>::TA);
-    hipLaunchKernelGGL((k_col<LA, C, TW, PREC, T>), dim3((unsigned)nwg), dim3(C * Plan<LA, false, T>::TA), 0, st, a);
+    nwg = fit_grid(grid8(nwg), NT);
+    hipLaunchKernelGGL((k_col<LA, C, TW, PREC, T, PX>), dim3((unsigned)nwg), dim3(NT), 0, st, a);
 }
 template<int LA, int PREC = PREC_FP32>
 void launch_xc(const XcArgs& a, hipStream_t st) {

@@ -948,10 +948,10 @@ void k_dif(RowArgs p) {
     }
 }
 
-template<int LA, int C, int TW, int PREC = PREC_FP32, class T = float>
+template<int LA, int C, int TW, int PREC = PREC_FP32, class T = float, int PX = 0>
 __device__ __forceinline__ void col_tile(const ColArgsT<T>& p, int64_t tile, T* lre, T* lim, const T* ltw,
                                          const T* lptw, const MfmaDft16* mf = nullptr) {
-    using P           = Plan<LA, false, T>;
+    using P           = Plan<LA, false, T, PX>;
     constexpr int PPT = P::PPT;
     constexpr int R0  = P::first_radix;
     constexpr int RL  = P::last_radix;
@@ -966,13 +966,26 @@ __device__ __forceinline__ void col_tile(const ColArgsT<T>& p, int64_t tile, T* 
     const BufR<T>  src(tbase(p.src, p.src_offs, p.src_dist, bt), p.src_lp);
     const BufW<T>  dst(tbase(p.dst, p.dst_offs, p.dst_dist, bt), p.dst_lp);
 
+    // Element e = jt + c of the column sits at n1 + (e << n1_lg).  When poff adds over multiples
+    // of 2^n1_lg (pack at most 2^n1_lg, or at least the whole transform), each access is one
+    // per-lane offset poff(n1 + (jt << n1_lg)) plus the scalar poff(c << n1_lg).
+    auto additive = [&](int lp) { return lp <= p.n1_lg || lp >= p.n1_lg + LA; };
     cx<T> v[PPT];
-    if (((int64_t)1 << (p.n1_lg + LA)) <= p.src_size) { // the whole transform is inside the source
+    if (((int64_t)1 << (p.n1_lg + LA)) <= p.src_size && additive(p.src_lp)) {
+        const uint32_t vo = poff(n1 + ((uint32_t)jt << p.n1_lg), p.src_lp);
 #pragma unroll
         for (int m = 0; m < PPT / R0; ++m)
 #pragma unroll
             for (int r = 0; r < R0; ++r) {
-                uint32_t i    = n1 + ((uint32_t)in_elem<LA, false, T>(jt, m, r) << p.n1_lg);
+                const uint32_t c = (uint32_t)(in_elem<LA, false, T, PX>(jt, m, r) - jt);
+                v[m * R0 + r]    = src.ld(vo, poff(c << p.n1_lg, p.src_lp), p.nt_load);
+            }
+    } else if (((int64_t)1 << (p.n1_lg + LA)) <= p.src_size) { // the whole transform is inside
+#pragma unroll
+        for (int m = 0; m < PPT / R0; ++m)
+#pragma unroll
+            for (int r = 0; r < R0; ++r) {
+                uint32_t i    = n1 + ((uint32_t)in_elem<LA, false, T, PX>(jt, m, r) << p.n1_lg);
                 v[m * R0 + r] = src.ld(poff(i, p.src_lp), p.nt_load);
             }
     } else {
@@ -980,7 +993,7 @@ __device__ __forceinline__ void col_tile(const ColArgsT<T>& p, int64_t tile, T* 
         for (int m = 0; m < PPT / R0; ++m)
 #pragma unroll
             for (int r = 0; r < R0; ++r) {
-                uint32_t i    = n1 + ((uint32_t)in_elem<LA, false, T>(jt, m, r) << p.n1_lg);
+                uint32_t i    = n1 + ((uint32_t)in_elem<LA, false, T, PX>(jt, m, r) << p.n1_lg);
                 bool     ok   = i < p.src_size;
                 cx<T>    x    = src.ld(ok ? poff(i, p.src_lp) : 0u, p.nt_load);
                 v[m * R0 + r] = ok ? x : cx<T>{T(0), T(0)};
@@ -991,13 +1004,13 @@ __device__ __forceinline__ void col_tile(const ColArgsT<T>& p, int64_t tile, T* 
         for (int q = 0; q < PPT; ++q)
             v[q].y = -v[q].y;
     }
-    stockham<LA, false, decltype(addr), PREC, T>(v, lre, lim, jt, addr, ltw, mf);
+    stockham<LA, false, decltype(addr), PREC, T, PX>(v, lre, lim, jt, addr, ltw, mf);
     if constexpr (TW > 0) {
 #pragma unroll
         for (int m = 0; m < PPT / RL; ++m)
 #pragma unroll
             for (int r = 0; r < RL; ++r) {
-                uint32_t e    = n1 * (uint32_t)out_elem<LA, false, T>(jt, m, r);
+                uint32_t e    = n1 * (uint32_t)out_elem<LA, false, T, PX>(jt, m, r);
                 v[m * RL + r] = cmul(v[m * RL + r], PassTw<TW, T>::get(lptw, p.twl, p.twh, p.tw_ls, p.tw_lg, e));
                 __builtin_amdgcn_sched_barrier(0); // one element's table reads in flight at a time
             }
@@ -1014,32 +1027,44 @@ __device__ __forceinline__ void col_tile(const ColArgsT<T>& p, int64_t tile, T* 
         for (int q = 0; q < PPT; ++q)
             v[q].y = -v[q].y;
     }
+    if (additive(p.dst_lp)) {
+        const uint32_t vo = poff(n1 + ((uint32_t)opaque_after_memory(jt) << p.n1_lg), p.dst_lp);
 #pragma unroll
-    for (int m = 0; m < PPT / RL; ++m)
+        for (int m = 0; m < PPT / RL; ++m)
 #pragma unroll
-        for (int r = 0; r < RL; ++r) {
-            uint32_t i = n1 + ((uint32_t)out_elem<LA, false, T>(jt, m, r) << p.n1_lg);
-            dst.st(poff(i, p.dst_lp), v[m * RL + r], p.nt_store);
-        }
+            for (int r = 0; r < RL; ++r) {
+                const uint32_t c = (uint32_t)(out_elem<LA, false, T, PX>(jt, m, r) - jt);
+                dst.st(vo, poff(c << p.n1_lg, p.dst_lp), v[m * RL + r], p.nt_store);
+            }
+    } else {
+#pragma unroll
+        for (int m = 0; m < PPT / RL; ++m)
+#pragma unroll
+            for (int r = 0; r < RL; ++r) {
+                uint32_t i = n1 + ((uint32_t)out_elem<LA, false, T, PX>(jt, m, r) << p.n1_lg);
+                dst.st(poff(i, p.dst_lp), v[m * RL + r], p.nt_store);
+            }
+    }
 }
 
-template<int LA, int C, int TW, class T = float>
+template<int LA, int C, int TW, class T = float, int PX = 0>
 constexpr int col_lds_floats() {
-    return 2 * (1 << LA) * C + TwTable<LA, T>::FLOATS + 2 * TW;
+    return (Plan<LA, false, T, PX>::HALF ? 1 : 2) * (1 << LA) * C + TwTable<LA, T>::FLOATS + 2 * TW;
 }
 
-template<int LA, int C, int TW, int PREC = PREC_FP32, class T = float>
-__global__ __launch_bounds__((C * Plan<LA, false, T>::TA),
-                             mfma_waves(waves_per_simd(C * Plan<LA, false, T>::TA,
-                                                       (int)sizeof(T) * col_lds_floats<LA, C, TW, T>()),
+// PX: plan options of the columns (points per thread, one-array exchange).
+template<int LA, int C, int TW, int PREC = PREC_FP32, class T = float, int PX = 0>
+__global__ __launch_bounds__((C * Plan<LA, false, T, PX>::TA),
+                             mfma_waves(waves_per_simd(C * Plan<LA, false, T, PX>::TA,
+                                                       (int)sizeof(T) * col_lds_floats<LA, C, TW, T, PX>()),
                                         PREC))
 void k_col(ColArgsT<T> p) {
-    static_assert(!Plan<LA, false, T>::HALF, "column tiles keep separate re / im arrays");
-    constexpr int NT   = C * Plan<LA, false, T>::TA;
-    constexpr int DATA = 2 * (1 << LA) * C;
-    __shared__ T  lds[col_lds_floats<LA, C, TW, T>()];
+    using P            = Plan<LA, false, T, PX>;
+    constexpr int NT   = C * P::TA;
+    constexpr int DATA = (P::HALF ? 1 : 2) * (1 << LA) * C;
+    __shared__ T  lds[col_lds_floats<LA, C, TW, T, PX>()];
     T*            lre  = lds;
-    T*            lim  = lds + DATA / 2;
+    T*            lim  = P::HALF ? lds : lds + DATA / 2;
     T*            ltw  = lds + DATA;
     T*            lptw = ltw + TwTable<LA, T>::FLOATS;
     TwTable<LA, T>::fill(ltw, p.twA, threadIdx.x, NT);
@@ -1053,7 +1078,7 @@ void k_col(ColArgsT<T> p) {
         const int64_t tile = xcd_item(base);
         if (tile >= p.ntiles)
             break; // uniform over the workgroup; later rounds have no items for it either
-        col_tile<LA, C, TW, PREC, T>(p, tile, lre, lim, ltw, lptw, &mf);
+        col_tile<LA, C, TW, PREC, T, PX>(p, tile, lre, lim, ltw, lptw, &mf);
         __syncthreads(); // LDS is reused by the next tile
     }
 }

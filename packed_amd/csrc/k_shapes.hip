@@ -10,8 +10,10 @@ constexpr FourStep kDefault[] = {
     // per row workgroup) 1.59 against 1.70 ms per 2 GiB
     shape<15, 8, 7, 64, 128>(), shape<16, 9, 7, 32, 128>(), shape<17, 9, 8, 32, 64>(), shape<18, 9, 9, 32, 32>(),
     // 2^19: 512-point rows, 32 per workgroup (128-byte store segments), 1.91 against 2.03 ms per
-    // 2 GiB with 1024-point rows (an alternative)
-    shape<19, 9, 10, 32, 16>(), shape<20, 10, 10, 16, 16>(), shape<21, 11, 10, 8, 16>(), shape<22, 12, 10, 4, 16>(),
+    // 2 GiB with 1024-point rows (an alternative); 1024-point columns 32 per tile (128-byte
+    // segments) with 32 points per thread and one LDS array: 1.83 / 1.83 ms forward / inverse
+    // against 1.85 / 1.89 with 16 per tile.  (2^20 gained nothing from the same columns.)
+    shape<19, 9, 10, 32, 32, 32 | kHalfExchange>(), shape<20, 10, 10, 16, 16>(), shape<21, 11, 10, 8, 16>(), shape<22, 12, 10, 4, 16>(),
     // 2^23, 2^24: more rows per workgroup than the LDS of two workgroups per CU allows (they
     // spill, but store 4 / 2 floats per segment instead of 2 / 1): 5.03 / 5.87 ms per 2 GiB
     // forward against 7.47 / 8.94

@@ -9,8 +9,8 @@ constexpr FourStep kAlt[] = {
     shape<16, 8, 8, 32, 64>(), shape<16, 8, 8, 16, 64>(), shape<16, 8, 8, 64, 64>(), shape<16, 8, 8, 32, 32>(),
     shape<16, 8, 8, 16, 32>(), shape<16, 10, 6, 16, 64>(), shape<16, 7, 9, 64, 32>(), shape<16, 6, 10, 64, 16>(),
     shape<16, 9, 7, 32, 32>(), shape<16, 10, 6, 16, 32>(), shape<16, 9, 7, 32, 64>(),
-    // N = 2^15, 2^19
-    shape<15, 8, 7, 32, 64>(), shape<15, 8, 7, 32, 128>(), shape<19, 10, 9, 16, 32>(),
+    // N = 2^15, 2^19 (the second: 2^19's former default, 16-column tiles)
+    shape<15, 8, 7, 32, 64>(), shape<15, 8, 7, 32, 128>(), shape<19, 10, 9, 16, 32>(), shape<19, 9, 10, 32, 16>(),
     // the largest sizes
     shape<23, 13, 10, 2, 16>(), shape<24, 14, 10, 1, 16>(), shape<24, 13, 11, 2, 8>(),
 };

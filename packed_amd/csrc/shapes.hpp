@@ -15,14 +15,15 @@ constexpr std::array<RowLaunch, 6> pass_set() {
             &launch_row<LA, G, LD_PERM, ST_CONTIG, TW, true, ORD_BITREV>,
             &launch_row<LA, G, LD_PERM, ST_CONTIG, TW, true, ORD_UNORD>};
 }
-template<int LA, int C, int LG>
+// PX: plan options of the columns (points per thread, one-array exchange).
+template<int LA, int C, int LG, int PX = 0>
 constexpr std::array<ColLaunch, 2> col_set() {
-    return {&launch_col<LA, C, 0>, &launch_col<LA, C, tw_cap<LG>()>};
+    return {&launch_col<LA, C, 0, PREC_FP32, float, PX>, &launch_col<LA, C, tw_cap<LG>(), PREC_FP32, float, PX>};
 }
-template<int LG, int N1, int N2, int G, int C>
+template<int LG, int N1, int N2, int G, int C, int PXC = 0>
 constexpr FourStep shape() {
     static_assert(N1 + N2 == LG, "shape");
-    return {LG, N1, N2, G, C, pass_set<N1, G, LG>(), col_set<N2, C, LG>()};
+    return {LG, N1, N2, G, C, pass_set<N1, G, LG>(), col_set<N2, C, LG, PXC>()};
 }
 
 // FP64 shapes, 256-512 threads per workgroup.  C and G are the doubles per column segment and
