@@ -81,6 +81,7 @@ SIGNATURES = {
     "pcx_plan_set_grid": (_int, [_vp, _i64, _i64]),
     "pcx_plan_set_chunk": (_int, [_vp, _i64]),
     "pcx_plan_set_one_pass": (_int, [_vp, _int]),
+    "pcx_plan_set_three_pass": (_int, [_vp, _int]),
     "pcx_fft": (_int, [_vp, _vp, _i64, _vp, _i64, _i64, _vp]),
     "pcx_ifft": (_int, [_vp, _int, _vp, _i64, _vp, _i64, _vp]),
     "pcx_fft_batch": (_int, [_vp, _int, _int, _vp, _i64, _i64, _vp, _i64, _i64, _i64, _i64, _vp]),

@@ -38,6 +38,10 @@ struct RowArgsT {
     int           nt_load, nt_store;  // non-temporal (streaming) global loads / stores
     const int64_t* src_offs;          // optional per-transform element offsets (instead of b * dist)
     const int64_t* dst_offs;
+    // rsplit_lg > 0 (contiguous loads only): row r of a transform is read from row
+    // ((r mod 2^rsplit_lg) << (rows_lg - rsplit_lg)) + (r >> rsplit_lg), i.e. the two parts of the
+    // row index trade places (the last pass of the three-pass transform)
+    int           rsplit_lg;
 };
 using RowArgs = RowArgsT<float>;
 
@@ -303,6 +307,10 @@ __device__ __forceinline__ void row_group(const RowArgsT<T>& p, int64_t wg, T* l
     const int64_t  rmask  = ((int64_t)1 << p.rows_lg) - 1;
     const int64_t  bt     = row >> p.rows_lg;
     const uint32_t k2     = (uint32_t)(row & rmask);
+    // the row the contiguous loads read (k2 unless the row index is split)
+    const uint32_t k2l    = p.rsplit_lg == 0 ? k2
+                                             : ((k2 & ((1u << p.rsplit_lg) - 1)) << (p.rows_lg - p.rsplit_lg)) |
+                                                   (k2 >> p.rsplit_lg);
     // transform of this thread's row; with UNI the same for the whole wavefront.  Rows past the
     // batch (the last group may be partial) take the group's first row, which always exists:
     // their loads stay inside the source (values masked) and they store nothing.
@@ -389,7 +397,7 @@ __device__ __forceinline__ void row_group(const RowArgsT<T>& p, int64_t wg, T* l
     if (loaded) {
     } else if constexpr (LD == LD_CONTIG) {
         const RdView<UNI, T> src(tbase(p.src, p.src_offs, p.src_dist, btu), p.src_lp);
-        if (row_ok && (int64_t)(k2 + 1) << LA <= p.src_size) { // whole row inside the source
+        if (row_ok && (int64_t)(k2l + 1) << LA <= p.src_size) { // whole row inside the source
             if constexpr (P::HALF) {
                 // one per-lane offset plus scalar offsets: the 2^14-point plans need the registers
                 const uint32_t vo = poff((uint32_t)jt, p.src_lp);
@@ -397,7 +405,7 @@ __device__ __forceinline__ void row_group(const RowArgsT<T>& p, int64_t wg, T* l
                 for (int m = 0; m < PPT / R0; ++m)
 #pragma unroll
                     for (int r = 0; r < R0; ++r) {
-                        uint32_t c    = (uint32_t)(in_elem<LA, false, T, PX>(jt, m, r) - jt) + (k2 << LA);
+                        uint32_t c    = (uint32_t)(in_elem<LA, false, T, PX>(jt, m, r) - jt) + (k2l << LA);
                         v[m * R0 + r] = src.ld(vo, poff(c, p.src_lp), p.nt_load);
                     }
             } else {
@@ -405,7 +413,7 @@ __device__ __forceinline__ void row_group(const RowArgsT<T>& p, int64_t wg, T* l
                 for (int m = 0; m < PPT / R0; ++m)
 #pragma unroll
                     for (int r = 0; r < R0; ++r) {
-                        uint32_t i    = (uint32_t)in_elem<LA, false, T, PX>(jt, m, r) + (k2 << LA);
+                        uint32_t i    = (uint32_t)in_elem<LA, false, T, PX>(jt, m, r) + (k2l << LA);
                         v[m * R0 + r] = src.ld(poff(i, p.src_lp), p.nt_load);
                     }
             }
@@ -414,7 +422,7 @@ __device__ __forceinline__ void row_group(const RowArgsT<T>& p, int64_t wg, T* l
             for (int m = 0; m < PPT / R0; ++m)
 #pragma unroll
                 for (int r = 0; r < R0; ++r) {
-                    uint32_t i    = (uint32_t)in_elem<LA, false, T, PX>(jt, m, r) + (k2 << LA);
+                    uint32_t i    = (uint32_t)in_elem<LA, false, T, PX>(jt, m, r) + (k2l << LA);
                     bool     ok   = row_ok && i < p.src_size;
                     cx<T>    x    = src.ld(ok ? poff(i, p.src_lp) : 0u, p.nt_load);
                     v[m * R0 + r] = ok ? x : cx<T>{T(0), T(0)};

@@ -19,6 +19,15 @@ constexpr FourStep kDefault[] = {
     // forward against 7.47 / 8.94
     shape<23, 12, 11, 4, 8>(), shape<24, 14, 10, 2, 16>(),
 };
+template<int LG>
+constexpr ThreePass three_pass() {
+    constexpr int N1 = LG - kThreeN2Log2, NB = N1 - kThreeNaLog2;
+    return {LG, &launch_col<kThreeN2Log2, 16, tw_cap<LG>()>, &launch_col<NB, 1 << kThreeNaLog2, tw_cap<N1>()>,
+            &launch_row<kThreeNaLog2, 128, LD_CONTIG, ST_STRIDED, 0, true, 0>};
+}
+const std::array<ThreePass, kMaxLog2 - kThreeMinLog2 + 1> kThree = {three_pass<21>(), three_pass<22>(),
+                                                                     three_pass<23>(), three_pass<24>()};
 } // namespace
 ShapeList<float> shapes_default() { return {kDefault, (int)(sizeof(kDefault) / sizeof(kDefault[0]))}; }
+const std::array<ThreePass, kMaxLog2 - kThreeMinLog2 + 1>& three_pass_tab() { return kThree; }
 } // namespace pcxd

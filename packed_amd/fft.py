@@ -243,6 +243,10 @@ class fft_unit:
     def set_one_pass(self, enable: bool) -> None:
         check(lib().pcx_plan_set_one_pass(self._h, int(bool(enable))))
 
+    def set_three_pass(self, enable: bool) -> None:
+        """FP32 natural order, 2^21 .. 2^24: three passes (the default from 2^23) or the four-step."""
+        check(lib().pcx_plan_set_three_pass(self._h, int(bool(enable))))
+
     # ---------------------------------------------------------------- raw pointer forms
     def fft_raw(self, dst: int, src: int | None = None, dst_pack: int = 1, src_pack: int | None = None,
                 src_size: int | None = None, stream=None) -> None:

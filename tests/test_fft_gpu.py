@@ -211,6 +211,44 @@ def test_one_pass_dif_agrees_with_four_step(orc, pa, order):
     assert diff < fft_tol(n), diff
 
 
+@pytest.mark.parametrize("lg", [21, 22, 23, 24])
+def test_three_pass_agrees_with_four_step(orc, pa, lg):
+    """FP32 natural order above 2^20 runs in three passes; the same batches through the four-step
+    split agree, forward (pack 8 in, N out, zero-extended source, two chunks) and inverse
+    (normalised, in place), and the forward output matches the FP64 oracle."""
+    import torch
+
+    n, batch = 1 << lg, 2
+    xs = [orc.gen_normal(0x5EED0080 + b, n) for b in range(batch)]
+    cut = n - 12345
+    src = to_device(batch_buffer([x[:cut] for x in xs], 8, orc.real_size(n, 8)))
+    sd = orc.real_size(n, 8)
+    outs, backs = [], []
+    for three in (True, False):
+        unit = pa.fft_unit(n)
+        unit.set_three_pass(three)
+        unit.set_chunk(8 * n)  # one transform per chunk
+        d = torch.zeros(2 * n * batch, dtype=torch.float32, device="cuda")
+        unit.batch(d.data_ptr(), src.data_ptr(), batch, n, 2 * n, 8, sd, src_size=cut)
+        torch.cuda.synchronize()
+        outs.append(d.cpu().numpy().astype(np.float64))
+        unit.batch(d.data_ptr(), d.data_ptr(), batch, n, 2 * n, n, 2 * n, inverse=True)
+        torch.cuda.synchronize()
+        backs.append(d.cpu().numpy().astype(np.float64))
+    diff = np.linalg.norm(outs[0] - outs[1]) / np.linalg.norm(outs[1])
+    assert diff < fft_tol(n), diff
+    diff = np.linalg.norm(backs[0] - backs[1]) / np.linalg.norm(backs[1])
+    assert diff < fft_tol(n), diff
+    for b in (0, batch - 1):
+        x = np.zeros(n, dtype=np.complex128)
+        x[:cut] = xs[b][:cut]
+        want = orc.fft64(x)
+        o = outs[0][b * 2 * n:(b + 1) * 2 * n]
+        assert orc.rel_l2(o[:n] + 1j * o[n:], want) < fft_tol(n), b
+        r = backs[0][b * 2 * n:(b + 1) * 2 * n]
+        assert orc.rel_l2(r[:n] + 1j * r[n:], x) < 2 * fft_tol(n), b
+
+
 def test_shared_plan_on_two_streams(orc, pa):
     """One four-step plan (as copies of a C++ fft_unit share it) called on two streams without a
     synchronisation between the calls: the second call waits for the first one's use of the
