@@ -104,24 +104,23 @@ constexpr int tw_cap64() {
     return pass_tw_entries(LG) <= 512 ? pass_tw_entries(LG) : 2;
 }
 
-// Three-pass transform of 2^21 .. 2^24 points in natural order: N = N1 * N2, N2 = 2^10 and
-// N1 = Na * Nb with Na = 2^7, Nb = 2^(lg - 17):
+// Three-pass transform of 2^21 .. 2^24 points in natural order: N = N1 * N2, N1 = Na * Nb with
+// Nb = 2^7:
 //   1. columns of length N2 (stride N1), times W_N^(n1*k2), into an intermediate with one
 //      N1-point row (pack N1) per k2;
-//   2. in each row, columns of length Nb (stride Na, all 128 in one tile), times
-//      W_N1^(na*kb), in place;
+//   2. in each row, columns of length Nb (stride Na, 128 per tile), times W_N1^(na*kb), in place;
 //   3. rows of Na points, read in (kb, k2) order and stored to k2 + N2*kb + N2*Nb*ka.
 // Every pass moves 128-byte or wider segments, where the natural-order four-step row pass of
 // 2^11..2^14-point rows stores 8 to 2 rows (32 to 8 bytes) per segment.
-constexpr int kThreeMinLog2     = 21;
-constexpr int kThreeDefaultLog2 = 23; // the default form from this size on
-constexpr int kThreeN2Log2  = 10;
-constexpr int kThreeNaLog2  = 7;
+constexpr int kThreeMinLog2 = 21;
+constexpr int kThreeNbLog2  = 7;
 struct ThreePass {
     int       lg;
-    ColLaunch col1; // pass 1 (with the W_N pass twiddle)
-    ColLaunch col2; // pass 2 (with the W_N1 pass twiddle)
-    RowLaunch row3; // pass 3
+    int       n2_lg;  // pass 1 column length
+    int       na_lg;  // pass 3 row length
+    ColLaunch col1;   // pass 1 (with the W_N pass twiddle)
+    ColLaunch col2;   // pass 2 (with the W_N1 pass twiddle)
+    RowLaunch row3;   // pass 3
 };
 const std::array<ThreePass, kMaxLog2 - kThreeMinLog2 + 1>& three_pass_tab(); // index lg - kThreeMinLog2
 

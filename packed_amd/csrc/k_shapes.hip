@@ -19,14 +19,19 @@ constexpr FourStep kDefault[] = {
     // forward against 7.47 / 8.94
     shape<23, 12, 11, 4, 8>(), shape<24, 14, 10, 2, 16>(),
 };
-template<int LG>
+// N2 and Na by size: 2^7 / 2^7 at 2^21, then 2^8 columns (256-byte segments; 2^10 columns
+// of 64 bytes at the 2^14-element stride of 2^24 took 1.64 ms for pass 1) and rows of
+// 2^7 / 2^8 / 2^9 (128 / 64 / 32 rows per workgroup, 512- to 128-byte stores).
+template<int LG, int N2, int NA, int C1, int G3>
 constexpr ThreePass three_pass() {
-    constexpr int N1 = LG - kThreeN2Log2, NB = N1 - kThreeNaLog2;
-    return {LG, &launch_col<kThreeN2Log2, 16, tw_cap<LG>()>, &launch_col<NB, 1 << kThreeNaLog2, tw_cap<N1>()>,
-            &launch_row<kThreeNaLog2, 128, LD_CONTIG, ST_STRIDED, 0, true, 0>};
+    constexpr int N1 = LG - N2;
+    static_assert(N1 == NA + kThreeNbLog2 && C1 << N2 == 16384 && G3 << NA == 16384, "three-pass split");
+    return {LG, N2, NA, &launch_col<N2, C1, tw_cap<LG>()>, &launch_col<kThreeNbLog2, 128, tw_cap<N1>()>,
+            &launch_row<NA, G3, LD_CONTIG, ST_STRIDED, 0, true, 0>};
 }
-const std::array<ThreePass, kMaxLog2 - kThreeMinLog2 + 1> kThree = {three_pass<21>(), three_pass<22>(),
-                                                                     three_pass<23>(), three_pass<24>()};
+const std::array<ThreePass, kMaxLog2 - kThreeMinLog2 + 1> kThree = {
+    three_pass<21, 7, 7, 128, 128>(), three_pass<22, 8, 7, 64, 128>(), three_pass<23, 8, 8, 64, 64>(),
+    three_pass<24, 8, 9, 64, 32>()};
 } // namespace
 ShapeList<float> shapes_default() { return {kDefault, (int)(sizeof(kDefault) / sizeof(kDefault[0]))}; }
 const std::array<ThreePass, kMaxLog2 - kThreeMinLog2 + 1>& three_pass_tab() { return kThree; }

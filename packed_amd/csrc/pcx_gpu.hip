@@ -191,12 +191,13 @@ constexpr int64_t kChunkBytes = (int64_t)2048 << 20;
 // Tables of the three-pass form: stage tables W_N2, W_Nb, W_Na and the two-level W_N1 table of
 // pass 2 (the W_N pass table twl / twh is the four-step one).
 hipError_t three_pass_tables(pcx_plan_t p, hipStream_t st) {
-    const int     l1  = p->lg - kThreeN2Log2;
-    const int64_t n1  = (int64_t)1 << l1, n2 = (int64_t)1 << kThreeN2Log2;
-    const int64_t na  = (int64_t)1 << kThreeNaLog2, nb = n1 / na;
-    p->tw_ls1         = (l1 + 1) / 2;
-    const int64_t nl  = (int64_t)1 << p->tw_ls1, nh = n1 >> p->tw_ls1;
-    hipError_t    he  = hipMalloc(&p->tw3_n2, sizeof(float2) * n2);
+    const ThreePass& tp = three_pass_tab()[p->lg - kThreeMinLog2];
+    const int        l1 = p->lg - tp.n2_lg;
+    const int64_t    n1 = (int64_t)1 << l1, n2 = (int64_t)1 << tp.n2_lg;
+    const int64_t    na = (int64_t)1 << tp.na_lg, nb = (int64_t)1 << kThreeNbLog2;
+    p->tw_ls1           = (l1 + 1) / 2;
+    const int64_t nl    = (int64_t)1 << p->tw_ls1, nh = n1 >> p->tw_ls1;
+    hipError_t    he    = hipMalloc(&p->tw3_n2, sizeof(float2) * n2);
     if (he == hipSuccess)
         he = hipMalloc(&p->tw3_nb, sizeof(float2) * nb);
     if (he == hipSuccess)
@@ -293,10 +294,8 @@ extern "C" int pcx_plan_create(int64_t fft_size, int64_t sub_size, int order, pc
         }
         if (he == hipSuccess && lg >= kThreeMinLog2)
             he = three_pass_tables(p, st);
-        // three passes are the default where they win (2^23 / 2^24: 3.55 / 3.69 ms per 2 GiB
-        // forward against 4.41 / 5.99 with the four-step; 2^21 / 2^22: 3.22 / 3.45 against
-        // 3.02 / 3.65)
-        p->use_three = p->use_three && lg >= kThreeDefaultLog2;
+        // three passes are the default: 2^21 / 2^22 / 2^23 / 2^24 in 2.74 / 2.83 / 2.93 /
+        // 2.95 ms per 2 GiB forward against 3.03 / 3.67 / 4.45 / 6.00 with the four-step
         if (he == hipSuccess)
             he = hipEventCreateWithFlags(&p->tmp_ev, hipEventDisableTiming);
     }
@@ -392,10 +391,8 @@ extern "C" int pcx_plan_create_f64(int64_t fft_size, int64_t sub_size, int order
         p->chunk = std::max<int64_t>(1, kChunkBytes / (fft_size * 16));
         if (he == hipSuccess && lg >= kThreeMinLog2)
             he = three_pass_tables(p, st);
-        // three passes are the default where they win (2^23 / 2^24: 3.55 / 3.69 ms per 2 GiB
-        // forward against 4.41 / 5.99 with the four-step; 2^21 / 2^22: 3.22 / 3.45 against
-        // 3.02 / 3.65)
-        p->use_three = p->use_three && lg >= kThreeDefaultLog2;
+        // three passes are the default: 2^21 / 2^22 / 2^23 / 2^24 in 2.74 / 2.83 / 2.93 /
+        // 2.95 ms per 2 GiB forward against 3.03 / 3.67 / 4.45 / 6.00 with the four-step
         if (he == hipSuccess)
             he = hipEventCreateWithFlags(&p->tmp_ev, hipEventDisableTiming);
     }
@@ -643,24 +640,26 @@ int run_batch(pcx_plan_t p, int inverse, int normalized, T* dst, int64_t dst_pac
         if (three) {
             // natural order, forward or inverse (conj in pass 1, conj and 1/N in pass 3)
             const ThreePass& tp = three_pass_tab()[p->lg - kThreeMinLog2];
-            const int        l1 = p->lg - kThreeN2Log2; // log2 N1; the intermediate's pack
+            const int        l1 = p->lg - tp.n2_lg; // log2 N1; the intermediate's pack
             const int64_t    m1 = (int64_t)1 << l1;
+            const int        c1n = 16384 >> tp.n2_lg; // pass 1 columns per tile
             ColArgs          c1{}, c2{};
             RowArgs          r3{};
             c1.src = cs, c1.src_dist = src_dist, c1.src_lp = src_lp, c1.src_size = src_size, c1.src_offs = cso;
             c1.dst = tmp, c1.dst_dist = 2 * n, c1.dst_lp = l1;
-            c1.ntiles = cb * (m1 / 16), c1.n1_lg = l1, c1.conj_in = inverse, c1.conj_out = 0, c1.scale = 1.f;
+            c1.ntiles = cb * (m1 / c1n), c1.n1_lg = l1, c1.conj_in = inverse, c1.conj_out = 0, c1.scale = 1.f;
             c1.twA = p->tw3_n2, c1.twl = p->twl, c1.twh = p->twh, c1.tw_ls = p->tw_ls, c1.tw_lg = p->lg;
             c1.nt_load = p->nt_flags & 1, c1.nt_store = (p->nt_flags >> 2) & 1;
             tp.col1(c1, p->grid_col, st);
             c2.src = tmp, c2.src_dist = 2 * m1, c2.src_lp = l1, c2.src_size = m1;
             c2.dst = tmp, c2.dst_dist = 2 * m1, c2.dst_lp = l1;
-            c2.ntiles = cb << kThreeN2Log2, c2.n1_lg = kThreeNaLog2, c2.conj_in = 0, c2.conj_out = 0, c2.scale = 1.f;
+            c2.ntiles = (cb << tp.n2_lg) << (tp.na_lg - kThreeNbLog2), c2.n1_lg = tp.na_lg;
+            c2.conj_in = 0, c2.conj_out = 0, c2.scale = 1.f;
             c2.twA = p->tw3_nb, c2.twl = p->twl1, c2.twh = p->twh1, c2.tw_ls = p->tw_ls1, c2.tw_lg = l1;
             tp.col2(c2, p->grid_col, st);
             r3.src = tmp, r3.src_dist = 2 * n, r3.src_lp = l1, r3.src_size = n;
             r3.dst = cd, r3.dst_dist = dst_dist, r3.dst_lp = dst_lp, r3.dst_offs = cdo;
-            r3.rows_lg = p->lg - kThreeNaLog2, r3.rsplit_lg = kThreeN2Log2, r3.nrows = cb << (p->lg - kThreeNaLog2);
+            r3.rows_lg = p->lg - tp.na_lg, r3.rsplit_lg = tp.n2_lg, r3.nrows = cb << (p->lg - tp.na_lg);
             r3.order = PCX_ORDER_NORMAL, r3.conj_in = 0, r3.conj_out = inverse, r3.scale = scale;
             r3.twA = p->tw3_na;
             r3.nt_load = (p->nt_flags >> 3) & 1, r3.nt_store = (p->nt_flags >> 1) & 1;
