@@ -104,25 +104,46 @@ constexpr int tw_cap64() {
     return pass_tw_entries(LG) <= 512 ? pass_tw_entries(LG) : 2;
 }
 
-// Three-pass transform of 2^21 .. 2^24 points in natural order: N = N1 * N2, N1 = Na * Nb with
-// Nb = 2^7:
+// Three-pass transform of 2^21 .. 2^24 points (FP64: 2^21 .. 2^23) in natural order:
+// N = N1 * N2, N1 = Na * Nb with Nb = 2^7:
 //   1. columns of length N2 (stride N1), times W_N^(n1*k2), into an intermediate with one
 //      N1-point row (pack N1) per k2;
-//   2. in each row, columns of length Nb (stride Na, 128 per tile), times W_N1^(na*kb), in place;
+//   2. in each row, columns of length Nb (stride Na), times W_N1^(na*kb), in place;
 //   3. rows of Na points, read in (kb, k2) order and stored to k2 + N2*kb + N2*Nb*ka.
 // Every pass moves 128-byte or wider segments, where the natural-order four-step row pass of
 // 2^11..2^14-point rows stores 8 to 2 rows (32 to 8 bytes) per segment.
 constexpr int kThreeMinLog2 = 21;
 constexpr int kThreeNbLog2  = 7;
-struct ThreePass {
-    int       lg;
-    int       n2_lg;  // pass 1 column length
-    int       na_lg;  // pass 3 row length
-    ColLaunch col1;   // pass 1 (with the W_N pass twiddle)
-    ColLaunch col2;   // pass 2 (with the W_N1 pass twiddle)
-    RowLaunch row3;   // pass 3
+template<class T>
+struct ThreePassT {
+    int            lg;
+    int            n2_lg; // pass 1 column length
+    int            na_lg; // pass 3 row length
+    int            c1;    // pass 1 columns per tile
+    int            c2;    // pass 2 columns per tile
+    ColLaunchT<T>  col1;  // pass 1 (with the W_N pass twiddle)
+    ColLaunchT<T>  col2;  // pass 2 (with the W_N1 pass twiddle)
+    RowLaunchT<T>  row3;  // pass 3
 };
-const std::array<ThreePass, kMaxLog2 - kThreeMinLog2 + 1>& three_pass_tab(); // index lg - kThreeMinLog2
+template<class T>
+struct ThreeList {
+    const ThreePassT<T>* e;
+    int                  count;
+};
+ThreeList<float>  three_pass_tab();   // 2^21 .. 2^24
+ThreeList<double> three_pass64_tab(); // 2^21 .. 2^23
+template<class T>
+inline const ThreePassT<T>* three_of(int lg) {
+    ThreeList<T> l;
+    if constexpr (sizeof(T) == 8)
+        l = three_pass64_tab();
+    else
+        l = three_pass_tab();
+    for (int k = 0; k < l.count; ++k)
+        if (l.e[k].lg == lg)
+            return &l.e[k];
+    return nullptr;
+}
 
 // Four-step shapes: N = 2^lg = N1 * N2 with N1 = 2^n1 (row length, G rows per workgroup,
 // G * N1 / PPT <= 1024 threads) and N2 = 2^n2 (column length, C columns per tile, C * N2 <= 16384

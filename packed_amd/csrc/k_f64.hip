@@ -41,9 +41,25 @@ constexpr FourStepT<double> kDefault64[] = {
 const std::array<std::array<RowLaunchT<double>, 5>, kMaxSinglePass64> kSingle64 =
     single_table64(std::make_integer_sequence<int, kMaxSinglePass64>{});
 
+// Three-pass natural-order transforms of 2^21 .. 2^23 doubles: tiles of 8192 points (128 KiB),
+// first-pass columns of 2^7 / 2^8 (64 / 32 per tile, 512- / 256-byte segments), 64 second-pass
+// columns per tile and rows of 2^7 / 2^8 (64 / 32 per workgroup).
+template<int LG, int N2, int NA>
+constexpr ThreePassT<double> three_pass64() {
+    using D          = double;
+    constexpr int N1 = LG - N2, C1 = 8192 >> N2, G3 = 8192 >> NA;
+    static_assert(N1 == NA + kThreeNbLog2, "three-pass split");
+    return {LG, N2, NA, C1, 64, &launch_col<N2, C1, tw_cap64<LG>(), PREC_FP32, D>,
+            &launch_col<kThreeNbLog2, 64, tw_cap64<N1>(), PREC_FP32, D>,
+            &launch_row<NA, G3, LD_CONTIG, ST_STRIDED, 0, true, 0, PREC_FP32, D>};
+}
+constexpr ThreePassT<double> kThree64[] = {three_pass64<21, 7, 7>(), three_pass64<22, 8, 7>(),
+                                           three_pass64<23, 8, 8>()};
+
 } // namespace
 
 const std::array<std::array<RowLaunchT<double>, 5>, kMaxSinglePass64>& single64_tab() { return kSingle64; }
 ShapeList<double> shapes64_default() { return {kDefault64, (int)(sizeof(kDefault64) / sizeof(kDefault64[0]))}; }
+ThreeList<double> three_pass64_tab() { return {kThree64, (int)(sizeof(kThree64) / sizeof(kThree64[0]))}; }
 
 } // namespace pcxd

@@ -23,16 +23,15 @@ constexpr FourStep kDefault[] = {
 // of 64 bytes at the 2^14-element stride of 2^24 took 1.64 ms for pass 1) and rows of
 // 2^7 / 2^8 / 2^9 (128 / 64 / 32 rows per workgroup, 512- to 128-byte stores).
 template<int LG, int N2, int NA, int C1, int G3>
-constexpr ThreePass three_pass() {
+constexpr ThreePassT<float> three_pass() {
     constexpr int N1 = LG - N2;
     static_assert(N1 == NA + kThreeNbLog2 && C1 << N2 == 16384 && G3 << NA == 16384, "three-pass split");
-    return {LG, N2, NA, &launch_col<N2, C1, tw_cap<LG>()>, &launch_col<kThreeNbLog2, 128, tw_cap<N1>()>,
+    return {LG, N2, NA, C1, 128, &launch_col<N2, C1, tw_cap<LG>()>, &launch_col<kThreeNbLog2, 128, tw_cap<N1>()>,
             &launch_row<NA, G3, LD_CONTIG, ST_STRIDED, 0, true, 0>};
 }
-const std::array<ThreePass, kMaxLog2 - kThreeMinLog2 + 1> kThree = {
-    three_pass<21, 7, 7, 128, 128>(), three_pass<22, 8, 7, 64, 128>(), three_pass<23, 8, 8, 64, 64>(),
-    three_pass<24, 8, 9, 64, 32>()};
+constexpr ThreePassT<float> kThree[] = {three_pass<21, 7, 7, 128, 128>(), three_pass<22, 8, 7, 64, 128>(),
+                                        three_pass<23, 8, 8, 64, 64>(), three_pass<24, 8, 9, 64, 32>()};
 } // namespace
 ShapeList<float> shapes_default() { return {kDefault, (int)(sizeof(kDefault) / sizeof(kDefault[0]))}; }
-const std::array<ThreePass, kMaxLog2 - kThreeMinLog2 + 1>& three_pass_tab() { return kThree; }
+ThreeList<float> three_pass_tab() { return {kThree, (int)(sizeof(kThree) / sizeof(kThree[0]))}; }
 } // namespace pcxd

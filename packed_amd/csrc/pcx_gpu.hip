@@ -158,14 +158,10 @@ struct pcx_plan_s {
     int     nt_flags     = 0; // bit 0/1: streaming load/store of src/dst, bit 2/3: of the intermediate
     float2* tw_dif       = nullptr; // W_M for the one-pass k_dif kernel
     bool    use_dif      = false;   // out-of-place transforms of 2^15 run as one k_dif pass
-    // three-pass natural-order transforms of 2^21 .. 2^24 (FP32): W_N2, W_Nb, W_Na and the
-    // two-level W_N1 pass table (three_pass_tab)
+    // three-pass natural-order transforms of 2^21 .. 2^24 (FP64: 2^21 .. 2^23): stage tables
+    // W_N2, W_Nb, W_Na and the two-level W_N1 pass table (lo, hi), in the plan's precision
     bool    use_three    = false;
-    float2* tw3_n2       = nullptr;
-    float2* tw3_nb       = nullptr;
-    float2* tw3_na       = nullptr;
-    float2* twl1         = nullptr;
-    float2* twh1         = nullptr;
+    void*   tw3[5]       = {};
     int     tw_ls1       = 0;
     // FP64 plans (pcx_plan_create_f64): the same roles in double precision
     bool     f64      = false;
@@ -189,33 +185,26 @@ constexpr int64_t kChunkBytes = (int64_t)2048 << 20;
 // Twiddle tables of a four-step FP32 plan for its shape's N1 x N2 split (W_N1, W_N2) and the
 // pass tables W_N = lo[e mod 2^ls] * hi[e >> ls].
 // Tables of the three-pass form: stage tables W_N2, W_Nb, W_Na and the two-level W_N1 table of
-// pass 2 (the W_N pass table twl / twh is the four-step one).
+// pass 2 (the W_N pass table is the four-step one).  T = float or double.
+template<class T>
 hipError_t three_pass_tables(pcx_plan_t p, hipStream_t st) {
-    const ThreePass& tp = three_pass_tab()[p->lg - kThreeMinLog2];
-    const int        l1 = p->lg - tp.n2_lg;
-    const int64_t    n1 = (int64_t)1 << l1, n2 = (int64_t)1 << tp.n2_lg;
-    const int64_t    na = (int64_t)1 << tp.na_lg, nb = (int64_t)1 << kThreeNbLog2;
-    p->tw_ls1           = (l1 + 1) / 2;
-    const int64_t nl    = (int64_t)1 << p->tw_ls1, nh = n1 >> p->tw_ls1;
-    hipError_t    he    = hipMalloc(&p->tw3_n2, sizeof(float2) * n2);
-    if (he == hipSuccess)
-        he = hipMalloc(&p->tw3_nb, sizeof(float2) * nb);
-    if (he == hipSuccess)
-        he = hipMalloc(&p->tw3_na, sizeof(float2) * na);
-    if (he == hipSuccess)
-        he = hipMalloc(&p->twl1, sizeof(float2) * nl);
-    if (he == hipSuccess)
-        he = hipMalloc(&p->twh1, sizeof(float2) * nh);
-    if (he == hipSuccess)
-        he = gen_twiddles(p->tw3_n2, n2, 1, n2, st);
-    if (he == hipSuccess)
-        he = gen_twiddles(p->tw3_nb, nb, 1, nb, st);
-    if (he == hipSuccess)
-        he = gen_twiddles(p->tw3_na, na, 1, na, st);
-    if (he == hipSuccess)
-        he = gen_twiddles(p->twl1, n1, 1, nl, st);
-    if (he == hipSuccess)
-        he = gen_twiddles(p->twh1, n1, nl, nh, st);
+    using V                   = vec2<T>;
+    const ThreePassT<T>* tp   = three_of<T>(p->lg);
+    if (!tp)
+        return hipSuccess;
+    const int     l1 = p->lg - tp->n2_lg;
+    const int64_t n1 = (int64_t)1 << l1;
+    p->tw_ls1        = (l1 + 1) / 2;
+    const int64_t len[5]   = {(int64_t)1 << tp->n2_lg, (int64_t)1 << kThreeNbLog2, (int64_t)1 << tp->na_lg,
+                              (int64_t)1 << p->tw_ls1, n1 >> p->tw_ls1};
+    const int64_t m[5]     = {len[0], len[1], len[2], n1, n1};
+    const int64_t stride[5] = {1, 1, 1, 1, len[3]};
+    hipError_t    he       = hipSuccess;
+    for (int k = 0; k < 5 && he == hipSuccess; ++k) {
+        he = hipMalloc(&p->tw3[k], sizeof(V) * len[k]);
+        if (he == hipSuccess)
+            he = gen_twiddles((V*)p->tw3[k], m[k], stride[k], len[k], st);
+    }
     if (he == hipSuccess)
         p->use_three = true;
     return he;
@@ -292,10 +281,10 @@ extern "C" int pcx_plan_create(int64_t fft_size, int64_t sub_size, int order, pc
             if (he == hipSuccess)
                 he = gen_twiddles(p->tw_dif, m, 1, m, st);
         }
-        if (he == hipSuccess && lg >= kThreeMinLog2)
-            he = three_pass_tables(p, st);
         // three passes are the default: 2^21 / 2^22 / 2^23 / 2^24 in 2.74 / 2.83 / 2.93 /
         // 2.95 ms per 2 GiB forward against 3.03 / 3.67 / 4.45 / 6.00 with the four-step
+        if (he == hipSuccess)
+            he = three_pass_tables<float>(p, st);
         if (he == hipSuccess)
             he = hipEventCreateWithFlags(&p->tmp_ev, hipEventDisableTiming);
     }
@@ -321,11 +310,8 @@ extern "C" int pcx_plan_destroy(pcx_plan_t p) {
     (void)hipFree(p->twh);
     (void)hipFree(p->tmp);
     (void)hipFree(p->tw_dif);
-    (void)hipFree(p->tw3_n2);
-    (void)hipFree(p->tw3_nb);
-    (void)hipFree(p->tw3_na);
-    (void)hipFree(p->twl1);
-    (void)hipFree(p->twh1);
+    for (void* t : p->tw3)
+        (void)hipFree(t);
     (void)hipFree(p->tw64_row);
     (void)hipFree(p->tw64_col);
     (void)hipFree(p->twl64);
@@ -389,10 +375,8 @@ extern "C" int pcx_plan_create_f64(int64_t fft_size, int64_t sub_size, int order
         if (he == hipSuccess)
             he = gen_twiddles(p->twh64, fft_size, nl, nh, st);
         p->chunk = std::max<int64_t>(1, kChunkBytes / (fft_size * 16));
-        if (he == hipSuccess && lg >= kThreeMinLog2)
-            he = three_pass_tables(p, st);
-        // three passes are the default: 2^21 / 2^22 / 2^23 / 2^24 in 2.74 / 2.83 / 2.93 /
-        // 2.95 ms per 2 GiB forward against 3.03 / 3.67 / 4.45 / 6.00 with the four-step
+        if (he == hipSuccess)
+            he = three_pass_tables<double>(p, st);
         if (he == hipSuccess)
             he = hipEventCreateWithFlags(&p->tmp_ev, hipEventDisableTiming);
     }
@@ -627,43 +611,42 @@ int run_batch(pcx_plan_t p, int inverse, int normalized, T* dst, int64_t dst_pac
         C    = fm->c; // the split kernels keep their own column tile
     }
     const int64_t n1 = (int64_t)1 << p->n1_lg, n2 = (int64_t)1 << p->n2_lg;
-    bool three = false;
-    if constexpr (!F64)
-        three = p->use_three && p->order == PCX_ORDER_NORMAL && p->precision == PCX_PREC_FP32;
+    const ThreePassT<T>* tp3 = p->use_three && p->order == PCX_ORDER_NORMAL && p->precision == PCX_PREC_FP32
+                                   ? three_of<T>(p->lg)
+                                   : nullptr;
     for (int64_t b0 = 0; b0 < batch; b0 += chunk) {
         int64_t      cb = batch - b0 < chunk ? batch - b0 : chunk;
         const T*       cs  = src_offs ? src : src + b0 * src_dist;
         T*             cd  = dst_offs ? dst : dst + b0 * dst_dist;
         const int64_t* cso = src_offs ? src_offs + b0 : nullptr;
         const int64_t* cdo = dst_offs ? dst_offs + b0 : nullptr;
-        if constexpr (!F64)
-        if (three) {
+        if (tp3) {
             // natural order, forward or inverse (conj in pass 1, conj and 1/N in pass 3)
-            const ThreePass& tp = three_pass_tab()[p->lg - kThreeMinLog2];
-            const int        l1 = p->lg - tp.n2_lg; // log2 N1; the intermediate's pack
-            const int64_t    m1 = (int64_t)1 << l1;
-            const int        c1n = 16384 >> tp.n2_lg; // pass 1 columns per tile
-            ColArgs          c1{}, c2{};
-            RowArgs          r3{};
+            using V          = vec2<T>;
+            const int     l1 = p->lg - tp3->n2_lg; // log2 N1; the intermediate's pack
+            const int64_t m1 = (int64_t)1 << l1;
+            ColArgsT<T>   c1{}, c2{};
+            RowArgsT<T>   r3{};
             c1.src = cs, c1.src_dist = src_dist, c1.src_lp = src_lp, c1.src_size = src_size, c1.src_offs = cso;
             c1.dst = tmp, c1.dst_dist = 2 * n, c1.dst_lp = l1;
-            c1.ntiles = cb * (m1 / c1n), c1.n1_lg = l1, c1.conj_in = inverse, c1.conj_out = 0, c1.scale = 1.f;
-            c1.twA = p->tw3_n2, c1.twl = p->twl, c1.twh = p->twh, c1.tw_ls = p->tw_ls, c1.tw_lg = p->lg;
+            c1.ntiles = cb * (m1 / tp3->c1), c1.n1_lg = l1, c1.conj_in = inverse, c1.conj_out = 0, c1.scale = T(1);
+            c1.twA = (const V*)p->tw3[0], c1.twl = twl, c1.twh = twh, c1.tw_ls = p->tw_ls, c1.tw_lg = p->lg;
             c1.nt_load = p->nt_flags & 1, c1.nt_store = (p->nt_flags >> 2) & 1;
-            tp.col1(c1, p->grid_col, st);
+            tp3->col1(c1, p->grid_col, st);
             c2.src = tmp, c2.src_dist = 2 * m1, c2.src_lp = l1, c2.src_size = m1;
             c2.dst = tmp, c2.dst_dist = 2 * m1, c2.dst_lp = l1;
-            c2.ntiles = (cb << tp.n2_lg) << (tp.na_lg - kThreeNbLog2), c2.n1_lg = tp.na_lg;
-            c2.conj_in = 0, c2.conj_out = 0, c2.scale = 1.f;
-            c2.twA = p->tw3_nb, c2.twl = p->twl1, c2.twh = p->twh1, c2.tw_ls = p->tw_ls1, c2.tw_lg = l1;
-            tp.col2(c2, p->grid_col, st);
+            c2.ntiles = (cb << tp3->n2_lg) * (((int64_t)1 << tp3->na_lg) / tp3->c2), c2.n1_lg = tp3->na_lg;
+            c2.conj_in = 0, c2.conj_out = 0, c2.scale = T(1);
+            c2.twA = (const V*)p->tw3[1], c2.twl = (const V*)p->tw3[3], c2.twh = (const V*)p->tw3[4];
+            c2.tw_ls = p->tw_ls1, c2.tw_lg = l1;
+            tp3->col2(c2, p->grid_col, st);
             r3.src = tmp, r3.src_dist = 2 * n, r3.src_lp = l1, r3.src_size = n;
             r3.dst = cd, r3.dst_dist = dst_dist, r3.dst_lp = dst_lp, r3.dst_offs = cdo;
-            r3.rows_lg = p->lg - tp.na_lg, r3.rsplit_lg = tp.n2_lg, r3.nrows = cb << (p->lg - tp.na_lg);
+            r3.rows_lg = p->lg - tp3->na_lg, r3.rsplit_lg = tp3->n2_lg, r3.nrows = cb << (p->lg - tp3->na_lg);
             r3.order = PCX_ORDER_NORMAL, r3.conj_in = 0, r3.conj_out = inverse, r3.scale = scale;
-            r3.twA = p->tw3_na;
+            r3.twA = (const V*)p->tw3[2];
             r3.nt_load = (p->nt_flags >> 3) & 1, r3.nt_store = (p->nt_flags >> 1) & 1;
-            tp.row3(r3, r3.nrows, p->grid_row, st);
+            tp3->row3(r3, r3.nrows, p->grid_row, st);
             HIPCHK(hipGetLastError());
             continue;
         }
@@ -786,7 +769,7 @@ extern "C" int pcx_plan_set_chunk(pcx_plan_t p, int64_t bytes) {
 extern "C" int pcx_plan_set_three_pass(pcx_plan_t p, int enable) {
     if (!p)
         return fail(PCX_ERR_INVALID_ARGUMENT, "plan is null");
-    if (!p->tw3_n2)
+    if (!p->tw3[0])
         return enable ? fail(PCX_ERR_UNSUPPORTED, "no three-pass form for fft_size " + s(p->n)) : PCX_SUCCESS;
     p->use_three = enable != 0;
     return PCX_SUCCESS;

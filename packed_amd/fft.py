@@ -244,7 +244,7 @@ class fft_unit:
         check(lib().pcx_plan_set_one_pass(self._h, int(bool(enable))))
 
     def set_three_pass(self, enable: bool) -> None:
-        """FP32 natural order, 2^21 .. 2^24: three passes (default) or the four-step."""
+        """Natural order, 2^21 .. 2^24 (FP64: 2^21 .. 2^23): three passes (default) or the four-step."""
         check(lib().pcx_plan_set_three_pass(self._h, int(bool(enable))))
 
     # ---------------------------------------------------------------- raw pointer forms

@@ -166,6 +166,42 @@ def test_batch_and_in_place_f64(O, pa, lg, order):
         assert O.rel_l2(unpacked64(O, h[b * dist:], n, pack), x) < 2 * tol64(n)
 
 
+@pytest.mark.parametrize("lg", [21, 22, 23])
+def test_three_pass_f64(O, pa, lg):
+    """Natural order above 2^20 runs in three passes: two transforms at a distance beyond the
+    footprint, pack 4, in place (one transform per chunk), forward and normalised inverse, against
+    the oracle and against the four-step."""
+    import torch
+
+    n, batch, pack = 1 << lg, 2, 4
+    dist = O.real_size(n, pack) + 64
+    xs = [x64(O, 0x5EED0601 + b, n) for b in range(batch)]
+    buf = np.zeros(dist * batch, np.float64)
+    for b, x in enumerate(xs):
+        pb = packed64(O, x, pack)
+        buf[b * dist: b * dist + len(pb)] = pb
+    outs = []
+    for three in (True, False):
+        d = dev(buf)
+        unit = pa.fft_unit(n, dtype="float64")
+        unit.set_three_pass(three)
+        unit.set_chunk(16 * n)
+        unit.batch(d.data_ptr(), d.data_ptr(), batch, pack, dist, pack, dist)
+        torch.cuda.synchronize()
+        outs.append(d.cpu().numpy())
+        if three:
+            h = outs[-1]
+            for b, x in enumerate(xs):
+                assert O.rel_l2(unpacked64(O, h[b * dist:], n, pack), O.fft64(x)) < tol64(n), b
+                assert np.all(h[b * dist + O.real_size(n, pack): (b + 1) * dist] == 0)  # gaps untouched
+            unit.batch(d.data_ptr(), d.data_ptr(), batch, pack, dist, pack, dist, inverse=True)
+            torch.cuda.synchronize()
+            h = d.cpu().numpy()
+            for b, x in enumerate(xs):
+                assert O.rel_l2(unpacked64(O, h[b * dist:], n, pack), x) < 2 * tol64(n), b
+    assert O.rel_l2(outs[0], outs[1]) < tol64(n)
+
+
 def test_round_trip_f64_large(O, pa):
     """ifft(fft(x)) == x at 2^20 x 8 (a size-independent property; 128 MiB)."""
     import torch
