@@ -81,9 +81,9 @@ int     pcx_plan_precision(pcx_plan_t plan);
  * pcx_plan_set_grid: workgroup caps of the column and row passes (0 = one per tile; the kernels
  *   loop over the rest).  pcx_plan_set_chunk: bytes of the intermediate of one chunk of a batch
  *   (0 = the default 2 GiB).  pcx_plan_set_one_pass: 2^15 runs as one decimation-in-frequency
- *   pass (default on) or as the four-step.  pcx_plan_set_three_pass: FP32 natural-order
- *   transforms of 2^21 .. 2^24 (FP64: 2^21 .. 2^23) run in three passes over the intermediate (default on) or as the
- *   four-step (the permuted orders always take the four-step). */
+ *   pass (default on) or as the four-step.  pcx_plan_set_three_pass: natural-order transforms
+ *   of 2^21 .. 2^24 points (FP64: 2^21 .. 2^23) run in three passes over the intermediate
+ *   (default on) or as the four-step (the permuted orders always take the four-step). */
 int pcx_plan_set_shape(pcx_plan_t plan, int n2_log2, int cols, int rows);
 int pcx_plan_shape(pcx_plan_t plan, int* n2_log2, int* cols, int* rows);
 int pcx_plan_set_grid(pcx_plan_t plan, int64_t grid_col, int64_t grid_row);
