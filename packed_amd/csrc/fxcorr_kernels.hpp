@@ -70,6 +70,28 @@ constexpr int xc_lds_floats() {
     return RowShape<LA, G, float, xc_px<LA>()>::DATA + TwTable<LA>::FLOATS;
 }
 
+// 2^14 blocks, 16 points per thread, FP32 butterflies (xc_lanes): the exchange before the radix-4
+// stage of both transforms runs in registers.  Thread t takes butterfly index xl_jt(t), t with
+// bits 4, 5 and 8, 9 swapped, so index bits 8, 9 (the two that exchange trades with register
+// bits 0, 1) sit in lane bits 4, 5, where v_permlane16_swap / v_permlane32_swap reach.  The
+// other two exchanges stay in LDS, at element e's address xl_lds(e): e bits 0-3 -> 0-3,
+// 8-9 -> 4-5, 10-11 -> 6-7, 4-7 -> 8-11, 12-13 -> 12-13, plus e bits 4-7 (as 0..15) and
+// 16 * e bits 12-13.  The pad is linear in the bits (so base + offset addressing stays exact)
+// and never decreases as the address bits above 5 count up (so no two elements share an
+// address; the highest is 16446, inside the row stride); a wavefront's lanes vary element bits
+// {0-3, 8, 9} in the stage reads, {4-7, 12, 13} in the stage-0 writes and {0-3, 12, 13} in the
+// stage-1 writes, and each of these sets lands on 64 distinct banks.
+template<int LA, int PPT, int RL, int PREC>
+constexpr bool xc_lanes() {
+    return LA == 14 && PPT == 16 && RL == 4 && PREC == PREC_FP32;
+}
+__device__ __forceinline__ int xl_jt(int t) {
+    return (t & ~0x330) | ((t >> 4) & 0x30) | ((t << 4) & 0x300);
+}
+__device__ __forceinline__ int xl_lds(int e) {
+    return ((e & 0x300F) | ((e >> 4) & 0xF0) | ((e << 4) & 0xF00)) + ((e >> 4) & 0xF) + ((e >> 8) & 0x30);
+}
+
 // B32: the signal fits 32-bit buffer offsets (one SGPR descriptor, one VGPR per address).
 // PREC_BF16X3 runs the radix-16 butterflies of both transforms on the matrix cores.
 template<int LA, int G, bool B32, int PREC = PREC_FP32>
@@ -81,7 +103,10 @@ __global__ __launch_bounds__(G * xc_threads<LA>(),
 void k_fxcorr(XcArgs p) {
     constexpr int PX   = xc_px<LA>();
     using PF           = Plan<LA, false, float, PX>;
-    using PI           = Plan<LA, true, float, PX>;
+    constexpr bool XL  = xc_lanes<LA, PF::PPT, PF::last_radix, PREC>();
+    // the inverse: the reversed plan (its first radix-RL stage takes the forward output's
+    // register map), or with XL the forward plan again after a register renaming
+    using PI           = Plan<LA, !XL, float, PX>;
     constexpr int TA   = PF::TA;
     constexpr int PPT  = PF::PPT;
     constexpr int R0   = PF::first_radix;
@@ -89,11 +114,13 @@ void k_fxcorr(XcArgs p) {
     constexpr int RLI  = PI::last_radix;
     using RS           = RowShape<LA, G, float, PX>;
     constexpr int STR  = RS::STRIDE;
-    static_assert(PI::first_radix == RL, "fused multiply needs matching register maps");
+    static_assert(XL || PI::first_radix == RL, "fused multiply needs matching register maps");
+    static_assert(!XL || (G == 1 && R0 == PPT), "register renaming assumes one block and one first-stage butterfly");
+    // the twiddle table first: its addresses then fit the LDS instructions' immediate offsets
     __shared__ float lds[xc_lds_floats<LA, G>()];
-    float*           lre = lds;
-    float*           lim = RS::im(lds);
-    float*           ltw = lds + RS::DATA;
+    float*           ltw = lds;
+    float*           lre = lds + TwTable<LA>::FLOATS;
+    float*           lim = RS::im(lre);
     TwTable<LA>::fill(ltw, p.twA, threadIdx.x, G * TA);
     __syncthreads();
 
@@ -103,7 +130,7 @@ void k_fxcorr(XcArgs p) {
     const int     t    = threadIdx.x;
     // a wavefront lies in one block when TA >= 64: keep the block's offsets in SGPRs
     const int     g    = TA >= 64 ? __builtin_amdgcn_readfirstlane(t / TA) : t / TA;
-    const int     jt   = t % TA;
+    const int     jt   = XL ? xl_jt(t % TA) : t % TA;
     const int64_t bg   = p.blk0 + xcd_item(0) * G + g; // block over all channels; neighbours share one L2
     const bool    ok   = bg < p.blk1;
     const int64_t ch   = ok ? bg / p.nblocks : 0;
@@ -111,7 +138,12 @@ void k_fxcorr(XcArgs p) {
     const int64_t off  = b * p.step;
     const float*  src  = p.src + ch * p.src_dist;
     float*        dst  = p.dst + ch * p.dst_dist;
-    auto          addr = [&](int b, int o) { return g * STR + b + (b >> 5) + o + (o >> 5); };
+    auto          addr = [&](int b, int o) {
+        if constexpr (XL)
+            return g * STR + xl_lds(b) + xl_lds(o);
+        else
+            return g * STR + b + (b >> 5) + o + (o >> 5);
+    };
 
     // Fast addressing (B32, block offset a multiple of both packs): element off + a of the
     // signal sits at poff(off) + poff(a), so a resource based at poff(off) serves the block, and
@@ -153,8 +185,8 @@ void k_fxcorr(XcArgs p) {
                 v[m * R0 + r] = x;
             }
     }
-    stockham<LA, false, decltype(addr), PREC, float, PX, false, false, decltype(addr), decltype(addr), true>(
-        v, lre, lim, jt, addr, ltw, &mf);
+    stockham<LA, false, decltype(addr), PREC, float, PX, false, false, decltype(addr), decltype(addr), true,
+             XL ? 4 : -1>(v, lre, lim, jt, addr, ltw, &mf);
     {
         // Y = conj(X) * K.  The spectrum is read in two halves: the second half's offset depends
         // (opaquely) on the first half's products, so at most half of it is in registers beside v.
@@ -175,17 +207,32 @@ void k_fxcorr(XcArgs p) {
             }
         }
     }
-    stockham<LA, true, decltype(addr), PREC, float, PX>(v, lre, lim, jt, addr, ltw, &mf);
+    if constexpr (XL) {
+        // forward output slot m*RL + r holds element jt + (m + r*PPT/RL)*TA; the forward plan's
+        // first stage takes element jt + k*TA in slot k
+        constexpr int NG = PPT / RL;
+        cf            u[PPT];
+#pragma unroll
+        for (int k = 0; k < PPT; ++k)
+            u[k] = v[(k % NG) * RL + k / NG];
+#pragma unroll
+        for (int k = 0; k < PPT; ++k)
+            v[k] = u[k];
+        stockham<LA, false, decltype(addr), PREC, float, PX, false, false, decltype(addr), decltype(addr), true, 4>(
+            v, lre, lim, jt, addr, ltw, &mf);
+    } else {
+        stockham<LA, true, decltype(addr), PREC, float, PX>(v, lre, lim, jt, addr, ltw, &mf);
+    }
     const int64_t keep = p.len - off < p.step ? p.len - off : p.step;
     if (aligned) {
-        const int      jt = opaque_after_memory(t % TA);
+        const int      jt = opaque_after_memory(XL ? xl_jt(t % TA) : t % TA);
         const BufW<>   dw(dst + poff((uint32_t)off, p.dst_lp), p.dst_lp, ok ? poff((uint32_t)keep, p.dst_lp) : 0u);
         const uint32_t vo = poff((uint32_t)jt, p.dst_lp);
 #pragma unroll
         for (int m = 0; m < PPT / RLI; ++m)
 #pragma unroll
             for (int r = 0; r < RLI; ++r) {
-                uint32_t c = (uint32_t)(out_elem<LA, true, float, PX>(jt, m, r) - jt);
+                uint32_t c = (uint32_t)(out_elem<LA, !XL, float, PX>(jt, m, r) - jt);
                 dw.st(vo + poff(c, p.dst_lp), cconj(v[m * RLI + r]), 0);
             }
     } else if (ok) {
@@ -193,7 +240,7 @@ void k_fxcorr(XcArgs p) {
         for (int m = 0; m < PPT / RLI; ++m)
 #pragma unroll
             for (int r = 0; r < RLI; ++r) {
-                int a = out_elem<LA, true, float, PX>(jt, m, r);
+                int a = out_elem<LA, !XL, float, PX>(jt, m, r);
                 if (a < keep) {
                     if constexpr (B32)
                         BufW<>(dst, p.dst_lp).st(poff((uint32_t)(off + a), p.dst_lp), cconj(v[m * RLI + r]), 0);

@@ -81,6 +81,33 @@ struct TwTable {
             t[2 * NLO + NHI + h]    = w.y;
         }
     }
+    // The twiddles W^(r * e1), r = 1, 2, ..., of one butterfly, e1 = k << SH.  The exponent advances
+    // by one addition per twiddle (kept opaque, so it is not folded back into a multiply per r) and
+    // is held in bytes: one address per table lookup, with the imaginary parts and the high factor
+    // at immediate offsets.  From SH = LOB on the low factor is W^0 = 1 (exactly): one lookup.
+    template<int SH>
+    struct Seq {
+        static constexpr uint32_t S = sizeof(T);
+        const char*               tb;
+        uint32_t                  e, step; // exponent * S
+        __device__ Seq(const T* t, uint32_t k) : tb((const char*)t), e(0), step((k << SH) * S) {}
+        __device__ static T at(const char* p) { return *(const T*)p; }
+        __device__ cx<T> next() {
+            e += step;
+            asm volatile("" : "+v"(e));
+            if constexpr (direct) {
+                return {at(tb + e), at(tb + NLO * S + e)};
+            } else if constexpr (SH >= LOB) {
+                const uint32_t h = e >> LOB;
+                return {at(tb + 2 * NLO * S + h), at(tb + (2 * NLO + NHI) * S + h)};
+            } else {
+                const uint32_t l = e & ((NLO - 1) * S), h = (e >> LOB) & ~(S - 1);
+                cx<T>          a = {at(tb + l), at(tb + NLO * S + l)};
+                cx<T>          b = {at(tb + 2 * NLO * S + h), at(tb + (2 * NLO + NHI) * S + h)};
+                return cmul(a, b);
+            }
+        }
+    };
     __device__ static cx<T> get(const T* t, uint32_t e) {
         if constexpr (direct) {
             return {t[e], t[NLO + e]};
@@ -190,12 +217,12 @@ __device__ __forceinline__ void lane_swap(T& a, T& b) {
         a = __builtin_bit_cast(T, x), b = __builtin_bit_cast(T, y);
     }
 }
-// Lane bits LT - LR + i <-> register bits i (i < LR) of the PPT = 16 values of every lane.
-template<int LT, int LR, class T>
+// Lane bits LB0 + i <-> register bits i (i < LR) of the PPT = 16 values of every lane.
+template<int LB0, int LR, class T>
 __device__ __forceinline__ void lane_exchange(cx<T>* v) {
     sfor<0, LR>([&](auto ii) {
         constexpr int i  = decltype(ii)::value;
-        constexpr int LB = LT - LR + i;
+        constexpr int LB = LB0 + i;
 #pragma unroll
         for (int q = 0; q < 16; ++q)
             if ((q >> i & 1) == 0) {
@@ -229,11 +256,14 @@ struct StageMap {
 // exchange before the last stage (or after the first) then also transposes, so a strided store
 // (load) needs no LDS round trip of its own.
 // LANES: the caller holds butterfly index jt0 in lane jt0 % 64 (jt0 = threadIdx.x % TA).  With 16
-// points per thread, a small last radix RL and TA <= 64, the exchange before the last stage is
-// then a swap of lane bits [log2(TA/RL), log2 TA) with register bits [0, log2 RL): for 2^9 and
-// 2^10 it runs in registers (lane_exchange), with no LDS round trip and no barrier.
+// points per thread and a small last radix RL, the exchange before the last stage is a swap of
+// butterfly index bits [log2(TA/RL), log2 TA) with register bits [0, log2 RL).  With TA <= 64
+// those index bits are lane bits [log2(TA/RL), log2 TA): for 2^9 and 2^10 the exchange runs in
+// registers (lane_exchange), with no LDS round trip and no barrier.
+// XLB >= 0 (with LANES, any TA): the caller placed butterfly index bits [log2(TA/RL), log2 TA)
+// in lane bits [XLB, XLB + log2 RL) (4 or 5), so the same exchange runs in registers.
 template<int LA, bool REV, class Addr, int PREC = PREC_FP32, class T = float, int PX = 0, bool FM = false,
-         bool LM = false, class AddrF = Addr, class AddrL = Addr, bool LANES = false>
+         bool LM = false, class AddrF = Addr, class AddrL = Addr, bool LANES = false, int XLB = -1>
 __device__ __forceinline__ void stockham(cx<T>* v, T* lre, T* lim, int jt0, const Addr& addr, const T* ltw,
                                          const MfmaDft16* mf = nullptr, const StageMap<AddrF>* first = nullptr,
                                          const StageMap<AddrL>* last = nullptr) {
@@ -243,8 +273,9 @@ __device__ __forceinline__ void stockham(cx<T>* v, T* lre, T* lim, int jt0, cons
     constexpr int TA  = P::TA;
     constexpr int PPT = P::PPT;
     constexpr int RLX = P::radix(P::NSTAGES - 1);
-    constexpr bool RX = LANES && !REV && !P::HALF && !LM && !(FM && P::NSTAGES == 2) && PPT == 16 && TA <= 64 &&
-                        P::NSTAGES >= 2 && RLX < 16 && lane_exchange_on(LA);
+    constexpr bool RX = LANES && !REV && !LM && !(FM && P::NSTAGES == 2) && PPT == 16 && P::NSTAGES >= 2 &&
+                        RLX < 16 && (XLB >= 0 || (!P::HALF && TA <= 64 && lane_exchange_on(LA)));
+    constexpr int  XL0 = XLB >= 0 ? XLB : ilog2c(TA) - ilog2c(RLX); // lane bit of index bit log2(TA/RL)
     sfor<0, P::NSTAGES>([&](auto ss) {
         constexpr int  s    = decltype(ss)::value;
         constexpr int  R    = P::radix(s);
@@ -276,9 +307,10 @@ __device__ __forceinline__ void stockham(cx<T>* v, T* lre, T* lim, int jt0, cons
             if constexpr (Ns > 1) {
                 constexpr int shift = LA - ilog2c(Ns * R);
                 uint32_t      k     = (uint32_t)(jt + m * TA) & (Ns - 1);
+                typename TwTable<LA, T>::template Seq<shift> tw(ltw, k);
 #pragma unroll
                 for (int r = 1; r < R; ++r)
-                    v[m * R + r] = cmul(v[m * R + r], TwTable<LA, T>::get(ltw, (uint32_t)(r * k) << shift));
+                    v[m * R + r] = cmul(v[m * R + r], tw.next());
             }
             if constexpr (R == 16 && PREC != PREC_FP32)
                 dft16_mfma<PREC>(mf, v + m * R);
@@ -288,7 +320,7 @@ __device__ __forceinline__ void stockham(cx<T>* v, T* lre, T* lim, int jt0, cons
                 __builtin_amdgcn_sched_barrier(0);
         }
         if constexpr (RX && s + 2 == P::NSTAGES) {
-            lane_exchange<ilog2c(TA), ilog2c(RLX)>(v); // into the last stage's layout
+            lane_exchange<XL0, ilog2c(RLX)>(v); // into the last stage's layout
         } else if constexpr (s + 1 < P::NSTAGES) {
             __syncthreads(); // every read of this stage is done before LDS is overwritten
             if constexpr (!P::HALF) {

@@ -109,3 +109,52 @@ def test_primitives_swap_one_lane_bit(lb):
             # expected: the value of (lane with bit lb := q, register := lane's bit lb)
             src_lane = (lane & ~(1 << lb)) | (q << lb)
             assert reg[lane] == 2 * src_lane + (lane >> lb & 1), (lb, lane, q)
+
+
+# ---- 2^14 correlation blocks (fxcorr_kernels.hpp, xc_lanes): TA = 1024 threads per transform
+def xl_jt(t):
+    """Butterfly index of thread t: t with bits 4, 5 and 8, 9 swapped (fxcorr_kernels.hpp)."""
+    return (t & ~0x330) | ((t >> 4) & 0x30) | ((t << 4) & 0x300)
+
+
+def test_xl_jt_places_index_bits_8_9_in_lane_bits_4_5():
+    t = np.arange(1024)
+    jt = xl_jt(t)
+    assert sorted(jt) == list(range(1024))  # a bijection
+    assert np.array_equal(jt >> 8 & 3, t >> 4 & 3)
+    for lb, ib in ((4, 8), (5, 9)):  # flipping the lane bit flips the index bit, nothing else
+        assert np.array_equal(xl_jt(t ^ (1 << lb)), jt ^ (1 << ib))
+
+
+def permlane_exchange(vals, jt_of):
+    """Swap lane bits 4, 5 with register bits 0, 1 by permlane swaps inside each wavefront, with
+    thread t holding butterfly index jt_of(t); returns the result indexed by butterfly index."""
+    ta = vals.shape[0]
+    t = np.arange(ta)
+    phys = vals[jt_of(t)]  # indexed by thread
+    for i, lb in enumerate((4, 5)):
+        new = phys.copy()
+        for q in range(16):
+            if q >> i & 1:
+                continue
+            a, b = phys[:, q].copy(), phys[:, q | 1 << i].copy()
+            for base in range(0, ta, 64):
+                w = slice(base, base + 64)
+                new[w, q], new[w, q | 1 << i] = permlane_swap(a[w], b[w], 1 << lb)
+        phys = new
+    out = np.empty_like(phys)
+    out[jt_of(t)] = phys
+    return out
+
+
+def test_register_exchange_2_14_on_physical_lanes():
+    """The 2^14 plan (radices 16, 16, 16, 4): the LDS exchange before the radix-4 stage equals
+    permlane swaps of lane bits 4 and 5 with register bits 0 and 1 when thread t holds butterfly
+    index xl_jt(t)."""
+    la, ta = 14, 1024
+    vals = np.arange(ta * 16).reshape(ta, 16)
+    want = lds_exchange(la, vals)
+    assert np.array_equal(lane_exchange(la, vals), want)  # the algebra, in butterfly-index space
+    assert np.array_equal(permlane_exchange(vals, xl_jt), want)
+    # with thread t holding index t, index bits 8, 9 are wave bits: the same swaps are wrong
+    assert not np.array_equal(permlane_exchange(vals, lambda t: t), want)

@@ -109,3 +109,51 @@ def test_old_padding_conflicts_in_permuted_order():
     la, (ppt, ta) = 14, plan(14, False)
     b = np.array([perm_base(j, la, ppt, ta, False) for j in range(64)])
     assert len(set((b + (b >> 5)) % 64)) == 8
+
+
+# ---- the 2^14 correlation layout with butterflies in permuted thread order (fxcorr_kernels.hpp)
+def xl_lds(e):
+    return ((e & 0x300F) | ((e >> 4) & 0xF0) | ((e << 4) & 0xF00)) + ((e >> 4) & 0xF) + ((e >> 8) & 0x30)
+
+
+def xl_jt(t):
+    return (t & ~0x330) | ((t >> 4) & 0x30) | ((t << 4) & 0x300)
+
+
+def xl_accesses():
+    """(name, base[thread], offsets) of the LDS exchanges of the 2^14 plan (radices 16, 16, 16, 4)
+    that stay in LDS: stage-0 writes, stage-1 reads, stage-1 writes, stage-2 reads."""
+    jt = xl_jt(np.arange(1024))
+    r = np.arange(16)
+    return [
+        ("write0", jt * 16, r),                          # Ns = 1: hi = 16 j, offset r
+        ("read1", jt, r * 1024),                         # j + r * A/16
+        ("write1", (jt // 16) * 256 + (jt & 15), r * 16),  # Ns = 16
+        ("read2", jt, r * 1024),
+    ]
+
+
+def test_xl_layout_is_injective_and_fits_the_row():
+    e = np.arange(1 << 14)
+    a = xl_lds(e)
+    assert len(np.unique(a)) == a.size
+    assert a.max() < (1 << 14) + (1 << 9) + 1  # the row stride the LDS array is sized for
+
+
+def test_xl_layout_is_additive_and_conflict_free():
+    for name, base, offs in xl_accesses():
+        full = base[:, None] + offs[None, :]
+        assert np.array_equal(xl_lds(full), xl_lds(base)[:, None] + xl_lds(offs)[None, :]), name
+        for w0 in range(0, 1024, 64):  # 64 lanes of one wavefront on 64 distinct banks
+            banks = xl_lds(full[w0:w0 + 64]) % 64
+            assert all(len(set(banks[:, u])) == 64 for u in range(16)), (name, w0)
+
+
+def test_xl_order_needs_its_layout():
+    """The plain padded layout with the same thread order puts up to four lanes on one bank."""
+    worst = 64
+    for name, base, offs in xl_accesses():
+        full = base[:, None] + offs[None, :]
+        banks = pad(full[:64]) % 64
+        worst = min(worst, min(len(set(banks[:, u])) for u in range(16)))
+    assert worst <= 16
