@@ -228,6 +228,17 @@ constexpr int default_rows() {
     return Plan<LA, false, T>::TA >= 256 ? 1 : 256 / Plan<LA, false, T>::TA;
 }
 
+// LDS carving: DATA elements of exchange arrays, a twiddle table of TWF elements and PTW of pass
+// twiddles.  The tables go first when behind the data they would end past 64 KiB, the reach of
+// the LDS instructions' immediate offsets (each lookup would then need an address addition);
+// otherwise the data stays at offset 0.
+template<class T, int DATA, int TWF, int PTW>
+struct LdsCarve {
+    static constexpr bool FIRST = (DATA + TWF + PTW) * (int)sizeof(T) > 65536;
+    static constexpr int  DOFF  = FIRST ? TWF + PTW : 0; // exchange arrays
+    static constexpr int  TOFF  = FIRST ? 0 : DATA;      // twiddle table, then pass twiddles
+};
+
 template<int LA, int G_, class T = float, int PX = 0>
 struct RowShape {
     static constexpr int TA = Plan<LA, false, T, PX>::TA;
@@ -774,10 +785,11 @@ __global__ __launch_bounds__((G * Plan<LA, false, T>::TA),
 void k_row(RowArgsT<T> p) {
     constexpr int NT = G * Plan<LA, false, T>::TA;
     __shared__ T  lds[row_lds_floats<LA, G, TW, T>()];
-    T*            lre  = lds;
-    T*            lim  = RowShape<LA, G, T>::im(lds);
-    T*            ltw  = lds + RowShape<LA, G, T>::DATA;
+    using LC           = LdsCarve<T, RowShape<LA, G, T>::DATA, TwTable<LA, T>::FLOATS, 2 * TW>;
+    T*            ltw  = lds + LC::TOFF;
     T*            lptw = ltw + TwTable<LA, T>::FLOATS;
+    T*            lre  = lds + LC::DOFF;
+    T*            lim  = RowShape<LA, G, T>::im(lre);
     TwTable<LA, T>::fill(ltw, p.twA, threadIdx.x, NT);
     if constexpr (TW > 0)
         PassTw<TW, T>::fill(lptw, p.twl, p.twh, p.tw_ls, p.tw_lg, threadIdx.x, NT);
@@ -814,11 +826,12 @@ template<int LM, int LR>
 constexpr int dif_lds_floats() {
     return RowShape<LM, 1>::DATA + TwTable<LM>::FLOATS + 2 * pass_tw_entries(LM + LR);
 }
-// Points per thread of k_dif: 32 for M = 2^14 (512 threads; 1.32 against 1.43 ms per 2 GiB at
-// N = 2^15 with 64), else the default rule.
+// Plan options of k_dif: 32 points per thread for M = 2^14 (512 threads; 1.32 against 1.43 ms
+// per 2 GiB at N = 2^15 with 64), else the default rule; twiddle exponents by addition chains
+// (one sub-transform per workgroup: no loop to hoist the twiddle addresses out of).
 template<int LM>
 constexpr int dif_ppt() {
-    return LM >= 14 ? 32 : 0;
+    return kTwChain | (LM >= 14 ? 32 : 0);
 }
 
 template<int LM, int LR, int ORD, int PREC = PREC_FP32>
@@ -839,10 +852,11 @@ void k_dif(RowArgs p) {
     static_assert(LM >= 5 && TA >= 32, "k_dif needs the block-permuted store layout (TA >= 32)");
     static_assert(!P::HALF || ORD == ORD_NORMAL, "permuted k_dif stores stage through separate re / im arrays");
     __shared__ float lds[dif_lds_floats<LM, LR>()];
-    float*           lre  = lds;
-    float*           lim  = RowShape<LM, 1>::im(lds);
-    float*           ltw  = lds + RowShape<LM, 1>::DATA;
+    using LC              = LdsCarve<float, RowShape<LM, 1>::DATA, TwTable<LM>::FLOATS, 2 * TWN>;
+    float*           ltw  = lds + LC::TOFF;
     float*           lptw = ltw + TwTable<LM>::FLOATS;
+    float*           lre  = lds + LC::DOFF;
+    float*           lim  = RowShape<LM, 1>::im(lre);
 
     const uint32_t w   = blockIdx.x;
     const int      k1  = (int)((w >> 3) & (R - 1));
@@ -1071,10 +1085,11 @@ void k_col(ColArgsT<T> p) {
     constexpr int NT   = C * P::TA;
     constexpr int DATA = (P::HALF ? 1 : 2) * (1 << LA) * C;
     __shared__ T  lds[col_lds_floats<LA, C, TW, T, PX>()];
-    T*            lre  = lds;
-    T*            lim  = P::HALF ? lds : lds + DATA / 2;
-    T*            ltw  = lds + DATA;
+    using LC           = LdsCarve<T, DATA, TwTable<LA, T>::FLOATS, 2 * TW>;
+    T*            ltw  = lds + LC::TOFF;
     T*            lptw = ltw + TwTable<LA, T>::FLOATS;
+    T*            lre  = lds + LC::DOFF;
+    T*            lim  = P::HALF ? lre : lre + DATA / 2;
     TwTable<LA, T>::fill(ltw, p.twA, threadIdx.x, NT);
     if constexpr (TW > 0)
         PassTw<TW, T>::fill(lptw, p.twl, p.twh, p.tw_ls, p.tw_lg, threadIdx.x, NT);
@@ -1124,9 +1139,10 @@ __global__ __launch_bounds__((C * Plan<LA, false, T>::TA)) void k_across(AcrossA
     constexpr int DATA = 2 * (1 << LA) * C;
     static_assert(!P::HALF, "across tiles keep separate re / im arrays");
     __shared__ T lds[across_lds_floats<LA, C, T>()];
-    T*           lre = lds;
-    T*           lim = lds + DATA / 2;
-    T*           ltw = lds + DATA;
+    using LC         = LdsCarve<T, DATA, TwTable<LA, T>::FLOATS, 0>;
+    T*           ltw = lds + LC::TOFF;
+    T*           lre = lds + LC::DOFF;
+    T*           lim = lre + DATA / 2;
     TwTable<LA, T>::fill(ltw, p.twA, threadIdx.x, NT);
     __syncthreads();
     const int64_t ntiles = (p.len + C - 1) / C;

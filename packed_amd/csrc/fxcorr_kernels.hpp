@@ -52,10 +52,12 @@ struct XcArgs {
 //    which leave one workgroup per CU, take 0.18 ms;
 //  * 2^13: the one-array exchange (35 KiB, four workgroups per CU instead of two): 512 channels
 //    of 2^20 samples with N = 2^13 take 2.65 ms against 3.52 ms;
-//  * else the default rule.
+//  * else the default rule;
+//  * twiddle exponents by addition chains (kTwChain): one block per workgroup, so there is no
+//    loop to hoist the per-thread twiddle addresses out of.
 template<int LA>
 constexpr int xc_px() {
-    return LA >= 14 ? 16 : (LA == 13 ? kHalfExchange : 0);
+    return kTwChain | (LA >= 14 ? 16 : (LA == 13 ? kHalfExchange : 0));
 }
 template<int LA>
 constexpr int xc_threads() {
@@ -116,10 +118,10 @@ void k_fxcorr(XcArgs p) {
     constexpr int STR  = RS::STRIDE;
     static_assert(XL || PI::first_radix == RL, "fused multiply needs matching register maps");
     static_assert(!XL || (G == 1 && R0 == PPT), "register renaming assumes one block and one first-stage butterfly");
-    // the twiddle table first: its addresses then fit the LDS instructions' immediate offsets
     __shared__ float lds[xc_lds_floats<LA, G>()];
-    float*           ltw = lds;
-    float*           lre = lds + TwTable<LA>::FLOATS;
+    using LC             = LdsCarve<float, RS::DATA, TwTable<LA>::FLOATS, 0>;
+    float*           ltw = lds + LC::TOFF;
+    float*           lre = lds + LC::DOFF;
     float*           lim = RS::im(lre);
     TwTable<LA>::fill(ltw, p.twA, threadIdx.x, G * TA);
     __syncthreads();

@@ -77,10 +77,12 @@ __device__ __noinline__ void queue_single(const QueueArgs& q, const QItem it_) {
     p.conj_in = p.conj_out = it.inv;
     p.scale                = (it.inv && uni(q.normalized)) ? 1.0f / (float)(1 << LA) : 1.0f;
     p.twA                  = uni(q.tw) + ((1 << LA) - 2);
-    float* ltw             = lds + S::DATA;
+    using LC               = LdsCarve<float, S::DATA, TwTable<LA>::FLOATS, 0>;
+    float* ltw             = lds + LC::TOFF;
+    float* lre             = lds + LC::DOFF;
     TwTable<LA>::fill(ltw, p.twA, threadIdx.x, kQueueThreads);
     __syncthreads();
-    row_group<LA, G, LD_CONTIG, ST_CONTIG, 0, (TA >= 64), 0, PREC_FP32, float, PX>(p, 0, lds, S::im(lds), ltw, nullptr);
+    row_group<LA, G, LD_CONTIG, ST_CONTIG, 0, (TA >= 64), 0, PREC_FP32, float, PX>(p, 0, lre, S::im(lre), ltw, nullptr);
 }
 
 // ---- four-step item: one transform of 2^LG points through this workgroup's intermediate.  Each
@@ -118,10 +120,11 @@ __device__ __noinline__ void queue_cols(const QueueArgs& q, const QItem it_) {
         ca.src = tmp, ca.dst = uni(q.dst) + uni(uni(q.dst_offs)[it.first]), ca.conj_out = 1;
         ca.scale = uni(q.normalized) ? (float)(1.0 / (double)F::N) : 1.0f;
     }
-    float* lre  = lds;
-    float* lim  = lds + (1 << F::N2L) * F::C;
-    float* ltw  = lds + 2 * (1 << F::N2L) * F::C;
+    using LC    = LdsCarve<float, 2 * (1 << F::N2L) * F::C, TwTable<F::N2L>::FLOATS, 2 * CT>;
+    float* ltw  = lds + LC::TOFF;
     float* lptw = ltw + TwTable<F::N2L>::FLOATS;
+    float* lre  = lds + LC::DOFF;
+    float* lim  = lre + (1 << F::N2L) * F::C;
     TwTable<F::N2L>::fill(ltw, ca.twA, threadIdx.x, kQueueThreads);
     if constexpr (CT > 0)
         PassTw<CT>::fill(lptw, ca.twl, ca.twh, ca.tw_ls, ca.tw_lg, threadIdx.x, kQueueThreads);
@@ -152,17 +155,19 @@ __device__ __noinline__ void queue_rows(const QueueArgs& q, const QItem it_) {
         ra.src = uni(q.src) + uni(uni(q.src_offs)[it.first]), ra.dst = tmp, ra.conj_in = 1;
         ra.twl = twl, ra.twh = twl + (1 << F::LS), ra.tw_ls = F::LS;
     }
-    float* ltw  = lds + S::DATA;
+    using LC    = LdsCarve<float, S::DATA, TwTable<F::N1L>::FLOATS, 2 * RT>;
+    float* ltw  = lds + LC::TOFF;
     float* lptw = ltw + TwTable<F::N1L>::FLOATS;
+    float* lre  = lds + LC::DOFF;
     TwTable<F::N1L>::fill(ltw, ra.twA, threadIdx.x, kQueueThreads);
     if constexpr (RT > 0)
         PassTw<RT>::fill(lptw, ra.twl, ra.twh, ra.tw_ls, ra.tw_lg, threadIdx.x, kQueueThreads);
     __syncthreads();
     for (int64_t wg = 0; wg < ra.nrows / F::G; ++wg) {
         if constexpr (!INV)
-            row_group<F::N1L, F::G, LD_CONTIG, ST_STRIDED, 0, true, 0>(ra, wg, lds, S::im(lds), ltw, lptw);
+            row_group<F::N1L, F::G, LD_CONTIG, ST_STRIDED, 0, true, 0>(ra, wg, lre, S::im(lre), ltw, lptw);
         else
-            row_group<F::N1L, F::G, LD_STRIDED, ST_CONTIG, RT, true, 0>(ra, wg, lds, S::im(lds), ltw, lptw);
+            row_group<F::N1L, F::G, LD_STRIDED, ST_CONTIG, RT, true, 0>(ra, wg, lre, S::im(lre), ltw, lptw);
         __syncthreads(); // LDS is reused by the next group
     }
 }

@@ -25,9 +25,13 @@
 namespace pcxk {
 
 // Plan options PX: a points-per-thread override in the low byte (the latency-bound fused
-// kernels run 2^14 with 16), and kHalfExchange to take the one-array exchange (Plan::HALF) below
-// the size where it is the default.
+// kernels run 2^14 with 16), kHalfExchange to take the one-array exchange (Plan::HALF) below
+// the size where it is the default, and kTwChain (below).
 constexpr int kHalfExchange = 0x200;
+// kTwChain: stage twiddle exponents by one addition per twiddle (TwTable::Seq) instead of one
+// multiply each.  For kernels that run one transform per workgroup (no loop to hoist the
+// per-thread twiddle addresses out of, and no registers to keep them in).
+constexpr int kTwChain = 0x400;
 template<int LA, bool REV = false, class T = float, int PX = 0>
 struct Plan {
     static constexpr bool F64    = sizeof(T) == 8;
@@ -108,6 +112,16 @@ struct TwTable {
             }
         }
     };
+    // e a multiple of 2^SH: from SH = LOB on the low factor is W^0 = 1 (exactly), so one lookup
+    template<int SH>
+    __device__ static cx<T> get_sh(const T* t, uint32_t e) {
+        if constexpr (!direct && SH >= LOB) {
+            uint32_t hi = e >> LOB;
+            return {t[2 * NLO + hi], t[2 * NLO + NHI + hi]};
+        } else {
+            return get(t, e);
+        }
+    }
     __device__ static cx<T> get(const T* t, uint32_t e) {
         if constexpr (direct) {
             return {t[e], t[NLO + e]};
@@ -307,10 +321,17 @@ __device__ __forceinline__ void stockham(cx<T>* v, T* lre, T* lim, int jt0, cons
             if constexpr (Ns > 1) {
                 constexpr int shift = LA - ilog2c(Ns * R);
                 uint32_t      k     = (uint32_t)(jt + m * TA) & (Ns - 1);
-                typename TwTable<LA, T>::template Seq<shift> tw(ltw, k);
+                if constexpr ((PX & kTwChain) != 0) {
+                    typename TwTable<LA, T>::template Seq<shift> tw(ltw, k);
 #pragma unroll
-                for (int r = 1; r < R; ++r)
-                    v[m * R + r] = cmul(v[m * R + r], tw.next());
+                    for (int r = 1; r < R; ++r)
+                        v[m * R + r] = cmul(v[m * R + r], tw.next());
+                } else {
+#pragma unroll
+                    for (int r = 1; r < R; ++r)
+                        v[m * R + r] =
+                            cmul(v[m * R + r], TwTable<LA, T>::template get_sh<shift>(ltw, (uint32_t)(r * k) << shift));
+                }
             }
             if constexpr (R == 16 && PREC != PREC_FP32)
                 dft16_mfma<PREC>(mf, v + m * R);
