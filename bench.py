@@ -39,8 +39,8 @@ def parse():
     ap.add_argument("--log2n", type=int, default=N_LOG2)
     ap.add_argument("--batch", type=int, default=BATCH)
     ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16x3", "fp16x2"],
-                    help="arithmetic of the default workload (bf16x3 / fp16x2: split-BF16 / split-FP16 "
-                         "matrix-core butterflies)")
+                    help="arithmetic of the default workload and of --config 3 (bf16x3 / fp16x2: "
+                         "split-BF16 / split-FP16 matrix-core butterflies)")
     ap.add_argument("--dtype", default="float32", choices=["float32", "float64"],
                     help="element type of the default workload (float64: the double-precision path)")
     ap.add_argument("--copy", action="store_true",
@@ -397,7 +397,7 @@ def run_config3(args):
     for o in offs:
         f[o:o + M] += g
     gv = pa.vector.from_complex(g, 8)
-    xc = pa.fxcorr_unit(gv, N)
+    xc = pa.fxcorr_unit(gv, N, precision=args.precision)
     src = pa.vector.from_complex(f, 8)
     work = pa.vector(L, 8)
     out = pa.vector(L, 8)
@@ -436,7 +436,7 @@ def run_config3(args):
                                                "traffic_bytes_per_launch"),
                      "kernel": "k_fxcorr (fused forward FFT, conj-multiply, inverse FFT per block)"},
         "peaks_index_exact": bool(np.array_equal(top, offs)), "rel_l2_vs_fp64_first_2^16": float(f"{err:.3e}"),
-        "in_place_equals_out_of_place_maxabs": same,
+        "in_place_equals_out_of_place_maxabs": same, "precision": args.precision,
         "config": {"workload": "overlap-save cross-correlation, L=2^24, M=4096, N=16384, pack 8"},
     }
     if not args.no_cpu:

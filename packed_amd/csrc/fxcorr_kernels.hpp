@@ -72,20 +72,20 @@ constexpr int xc_lds_floats() {
     return RowShape<LA, G, float, xc_px<LA>()>::DATA + TwTable<LA>::FLOATS;
 }
 
-// 2^14 blocks, 16 points per thread, FP32 butterflies (xc_lanes): the exchange before the radix-4
-// stage of both transforms runs in registers.  Thread t takes butterfly index xl_jt(t), t with
-// bits 4, 5 and 8, 9 swapped, so index bits 8, 9 (the two that exchange trades with register
-// bits 0, 1) sit in lane bits 4, 5, where v_permlane16_swap / v_permlane32_swap reach.  The
-// other two exchanges stay in LDS, at element e's address xl_lds(e): e bits 0-3 -> 0-3,
-// 8-9 -> 4-5, 10-11 -> 6-7, 4-7 -> 8-11, 12-13 -> 12-13, plus e bits 4-7 (as 0..15) and
-// 16 * e bits 12-13.  The pad is linear in the bits (so base + offset addressing stays exact)
-// and never decreases as the address bits above 5 count up (so no two elements share an
-// address; the highest is 16446, inside the row stride); a wavefront's lanes vary element bits
-// {0-3, 8, 9} in the stage reads, {4-7, 12, 13} in the stage-0 writes and {0-3, 12, 13} in the
-// stage-1 writes, and each of these sets lands on 64 distinct banks.
+// 2^14 blocks, 16 points per thread (xc_lanes; FP32 and matrix-core butterflies alike): the
+// exchange before the radix-4 stage of both transforms runs in registers.  Thread t takes
+// butterfly index xl_jt(t), t with bits 4, 5 and 8, 9 swapped, so index bits 8, 9 (the two that
+// exchange trades with register bits 0, 1) sit in lane bits 4, 5, where v_permlane16_swap /
+// v_permlane32_swap reach.  The other two exchanges stay in LDS, at element e's address
+// xl_lds(e): e bits 0-3 -> 0-3, 8-9 -> 4-5, 10-11 -> 6-7, 4-7 -> 8-11, 12-13 -> 12-13, plus
+// e bits 4-7 (as 0..15) and 16 * e bits 12-13.  The pad is linear in the bits (so base + offset
+// addressing stays exact) and never decreases as the address bits above 5 count up (so no two
+// elements share an address; the highest is 16446, inside the row stride); a wavefront's lanes
+// vary element bits {0-3, 8, 9} in the stage reads, {4-7, 12, 13} in the stage-0 writes and
+// {0-3, 12, 13} in the stage-1 writes, and each of these sets lands on 64 distinct banks.
 template<int LA, int PPT, int RL, int PREC>
 constexpr bool xc_lanes() {
-    return LA == 14 && PPT == 16 && RL == 4 && PREC == PREC_FP32;
+    return LA == 14 && PPT == 16 && RL == 4;
 }
 __device__ __forceinline__ int xl_jt(int t) {
     return (t & ~0x330) | ((t >> 4) & 0x30) | ((t << 4) & 0x300);
