@@ -14,7 +14,10 @@ sampled transforms (outputs) against the oracle's FP64 definition:
   transform, then the in-place inverse against the input over the whole batch;
 * a correlation of 2^30 + 2^20 samples with step 1 (64-point blocks, 64 taps): 2^32 + 2^22
   work-items in two launches; output windows at both ends and around 2^30 against the direct
-  sum out[i] = sum_m f[i+m] conj(g[m]).
+  sum out[i] = sum_m f[i+m] conj(g[m]);
+* a correlation of 2^28 + 12345 samples (pack 8) with 16384-point blocks and 4096 taps, whose
+  footprint passes 2^29 floats: the 2^14 block kernel with 64-bit element offsets, windows at
+  both ends, around 2^28 and at block seams against the same direct sum.
 
 Device memory peaks at 128 GiB (the first case).
 """
@@ -146,6 +149,37 @@ def test_correlation_beyond_2pow32_work_items(orc, pa):
         want = orc.xcorr_naive(fw, g)[:w]  # exact for the first w outputs (f is zero past L)
         os_ = out[2 * a:2 * (a + w)].cpu().numpy().astype(np.float64)
         got = os_[0::2] + 1j * os_[1::2]
+        assert orc.rel_l2(got, want) < 64 * EPS32, (a, orc.rel_l2(got, want))
+    del f, out, xc, gv
+    _free()
+
+
+def test_correlation_2p14_blocks_beyond_32bit_offsets(orc, pa):
+    """N = 16384, M = 4096, pack 8, L = 2^28 + 12345: the signal's packed footprint passes 2^29
+    floats, so the blocks take 64-bit element offsets (fxcorr_kernels.hpp, B32 = false)."""
+    import torch
+
+    n_fft, m, pack = 16384, 4096, 8
+    length = (1 << 28) + 12345
+    g = orc.gen_normal(0x5EED0706, m)
+    gv = pa.vector.from_complex(g, pack)
+    xc = pa.fxcorr_unit(gv, n_fft)
+    step = xc.step(pack)
+    foot = 2 * (-(-length // pack)) * pack
+    f = torch.empty(foot, dtype=torch.float32, device="cuda")
+    out = torch.empty_like(f)
+    _gen(f, pack, length, 0x5EED0707)
+    xc.batch(out.data_ptr(), f.data_ptr(), length, 1, pack, foot)
+    torch.cuda.synchronize()
+    w = 2048
+    seams = [k * step - w // 2 for k in (1, 7, (1 << 28) // step)]
+    starts = [0, (1 << 28) - w, length - w] + seams
+    for a in starts:
+        a -= a % pack  # whole packs, so the host slices below stay aligned
+        hi = min(a + w + m, length)
+        fw = orc.from_packed(f[2 * a:2 * (a + (-(-(hi - a) // pack)) * pack)].cpu().numpy(), hi - a, pack)
+        want = orc.xcorr_naive(fw.astype(np.complex128), g)[:w]  # exact: f is zero past L
+        got = orc.from_packed(out[2 * a:2 * (a + w)].cpu().numpy(), w, pack)
         assert orc.rel_l2(got, want) < 64 * EPS32, (a, orc.rel_l2(got, want))
     del f, out, xc, gv
     _free()
