@@ -179,8 +179,11 @@ struct ShapeList {
 // its host launchers do not exist) and returns it from a host accessor.
 // index LA-1 for LA in [1, 14]
 const std::array<RowSet, kMaxSinglePass>& single_tab();
-// split-precision single-pass variants (natural order): index [precision - 1][LA - 4]
+// split-precision single-pass variants (every order, both directions, as single_tab's sets):
+// index [precision - 1][LA - 4]; natural order (k_mfma.hip), the permuted orders (k_mfma_p1.hip)
+using MfmaSets = std::array<std::array<RowSet, kMaxSinglePass - 3>, 2>;
 const std::array<std::array<RowLaunch, kMaxSinglePass - 3>, 2>& single_mfma_tab();
+const MfmaSets&                                                 single_mfma_perm_tab();
 const std::array<XcLaunch, kMaxSinglePass>& xc_tab();                         // index LA-1
 const std::array<std::array<XcLaunch, kMaxSinglePass - 3>, 2>& xc_mfma_tab(); // [prec - 1][LA - 4]
 const std::array<std::array<RowLaunchT<double>, 5>, kMaxSinglePass64>& single64_tab(); // index LA-1
@@ -191,6 +194,10 @@ ShapeList<float>  shapes_alt();
 ShapeList<double> shapes64_default();
 ShapeList<double> shapes64_alt();
 const std::array<FourStepMfma, 6>& shapes_mfma();
+// permuted-order row kernels of those shapes (k_mfma_p4.hip): [shape][precision - 1]
+// {forward bit_reversed, forward unordered, inverse bit_reversed, inverse unordered}
+using MfmaPermRows = std::array<std::array<std::array<RowLaunch, 4>, 2>, 6>;
+const MfmaPermRows& shapes_mfma_perm();
 // one-pass k_dif for N = 2^15, index [order]
 const std::array<DifLaunch, 3>& dif_tab();
 constexpr int dif_core_log2(int order) { return order == 0 ? 14 : 13; }

@@ -405,8 +405,6 @@ extern "C" int pcx_plan_set_precision(pcx_plan_t p, int precision) {
     }
     if (precision != PCX_PREC_BF16X3 && precision != PCX_PREC_FP16X2)
         return fail(PCX_ERR_UNSUPPORTED, "precision " + s(precision) + " is not implemented");
-    if (p->order != PCX_ORDER_NORMAL)
-        return fail(PCX_ERR_UNSUPPORTED, "split-precision transforms are built for the normal order only");
     if (p->lg < 4 || p->lg > 20 || (p->two_pass && !find_mfma(p->lg, p->shape)))
         return fail(PCX_ERR_UNSUPPORTED, "split-precision transforms cover 2^4 .. 2^20 (default four-step shapes)");
     p->precision = precision;
@@ -531,8 +529,10 @@ int run_batch(pcx_plan_t p, int inverse, int normalized, T* dst, int64_t dst_pac
         a.src_offs = src_offs, a.dst_offs = dst_offs;
         if constexpr (F64)
             single64_tab()[p->lg - 1][single_variant(inverse, p->order)](a, batch, 0, st);
-        else if (p->precision != PCX_PREC_FP32)
+        else if (p->precision != PCX_PREC_FP32 && p->order == PCX_ORDER_NORMAL)
             single_mfma_tab()[p->precision - 1][p->lg - 4](a, batch, 0, st);
+        else if (p->precision != PCX_PREC_FP32)
+            single_mfma_perm_tab()[p->precision - 1][p->lg - 4][single_variant(inverse, p->order)](a, batch, 0, st);
         else
             single_tab()[p->lg - 1][single_variant(inverse, p->order)](a, batch, 0, st);
         HIPCHK(hipGetLastError());
@@ -607,7 +607,9 @@ int run_batch(pcx_plan_t p, int inverse, int normalized, T* dst, int64_t dst_pac
         if (!fm)
             return fail(PCX_ERR_UNSUPPORTED, "no split-precision kernels for this four-step shape");
         colf = fm->cols[p->precision - 1][inverse ? 0 : 1];
-        rowf = fm->rows[p->precision - 1][inverse ? 1 : 0];
+        rowf = p->order == PCX_ORDER_NORMAL
+                   ? fm->rows[p->precision - 1][inverse ? 1 : 0]
+                   : shapes_mfma_perm()[fm - shapes_mfma().data()][p->precision - 1][(inverse ? 2 : 0) + p->order - 1];
         C    = fm->c; // the split kernels keep their own column tile
     }
     const int64_t n1 = (int64_t)1 << p->n1_lg, n2 = (int64_t)1 << p->n2_lg;
