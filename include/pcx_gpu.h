@@ -64,7 +64,8 @@ int     pcx_plan_order(pcx_plan_t plan);
 /* Arithmetic of the plan's radix-16 butterflies: PCX_PREC_FP32 (default), PCX_PREC_BF16X3
  * (matrix cores, three-term BF16 split operands, FP32 accumulation) or PCX_PREC_FP16X2 (matrix
  * cores, hi + lo FP16 operands scaled by a power of two per butterfly, FP32 accumulation); the
- * split forms cover every order, N in [2^4, 2^20].  No reference counterpart: the reference
+ * split forms cover every order, N in [2^4, 2^24] (single-precision plans with their default
+ * four-step shape).  No reference counterpart: the reference
  * computes in FP32 only. */
 int     pcx_plan_set_precision(pcx_plan_t plan, int precision);
 int     pcx_plan_precision(pcx_plan_t plan);

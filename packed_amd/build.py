@@ -18,7 +18,7 @@ OBJ = os.path.join(HERE, "_build")
 OUT = os.path.join(HERE, "libpcx_gpu.so")
 
 # the slowest units first
-SOURCES = ["k_mfma_p1.hip", "k_mfma_p4.hip", "k_f64.hip", "k_shapes_alt.hip", "k_mfma.hip", "k_f64_alt.hip",
+SOURCES = ["k_mfma_big.hip", "k_mfma_p1.hip", "k_mfma_p4.hip", "k_f64.hip", "k_shapes_alt.hip", "k_mfma.hip", "k_f64_alt.hip",
            "k_shapes.hip", "k_single.hip", "k_xc.hip", "k_misc.hip", "pcx_gpu.hip"]
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

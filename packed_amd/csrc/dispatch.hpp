@@ -198,6 +198,10 @@ const std::array<FourStepMfma, 6>& shapes_mfma();
 // {forward bit_reversed, forward unordered, inverse bit_reversed, inverse unordered}
 using MfmaPermRows = std::array<std::array<std::array<RowLaunch, 4>, 2>, 6>;
 const MfmaPermRows& shapes_mfma_perm();
+// split-precision four-step shapes of 2^21 .. 2^24 (k_mfma_big.hip), the splits and tiles of the
+// FP32 defaults: [precision - 1][LG - 21]
+using MfmaBig = std::array<std::array<FourStep, 4>, 2>;
+const MfmaBig& shapes_mfma_big();
 // one-pass k_dif for N = 2^15, index [order]
 const std::array<DifLaunch, 3>& dif_tab();
 constexpr int dif_core_log2(int order) { return order == 0 ? 14 : 13; }

@@ -128,6 +128,14 @@ const FourStepMfma* find_mfma(int lg, const FourStep* sh) {
             return &f;
     return nullptr;
 }
+// The same above 2^20: the split shapes of 2^21 .. 2^24 follow the default shapes.
+const FourStep* find_mfma_big(int lg, const FourStep* sh, int precision) {
+    if (lg < 21 || lg > 24 || precision < PCX_PREC_BF16X3 || precision > PCX_PREC_FP16X2 ||
+        sh != find_shape(lg, 0, 0, 0))
+        return nullptr;
+    const FourStep* f = &shapes_mfma_big()[precision - 1][lg - 21];
+    return f->n1 == sh->n1 && f->n2 == sh->n2 && f->g == sh->g && f->c == sh->c ? f : nullptr;
+}
 
 } // namespace
 
@@ -405,8 +413,10 @@ extern "C" int pcx_plan_set_precision(pcx_plan_t p, int precision) {
     }
     if (precision != PCX_PREC_BF16X3 && precision != PCX_PREC_FP16X2)
         return fail(PCX_ERR_UNSUPPORTED, "precision " + s(precision) + " is not implemented");
-    if (p->lg < 4 || p->lg > 20 || (p->two_pass && !find_mfma(p->lg, p->shape)))
-        return fail(PCX_ERR_UNSUPPORTED, "split-precision transforms cover 2^4 .. 2^20 (default four-step shapes)");
+    const bool covered = p->lg >= 4 && (p->lg <= 20 ? (!p->two_pass || find_mfma(p->lg, p->shape))
+                                                     : find_mfma_big(p->lg, p->shape, precision) != nullptr);
+    if (!covered)
+        return fail(PCX_ERR_UNSUPPORTED, "split-precision transforms cover 2^4 .. 2^24 (default four-step shapes)");
     p->precision = precision;
     return PCX_SUCCESS;
 }
@@ -602,7 +612,13 @@ int run_batch(pcx_plan_t p, int inverse, int normalized, T* dst, int64_t dst_pac
     ColLaunchT<T> colf = shape->cols[inverse ? 0 : 1];
     RowLaunchT<T> rowf = shape->rows[(inverse ? 3 : 0) + p->order];
     if constexpr (!F64)
-    if (p->precision != PCX_PREC_FP32) {
+    if (p->precision != PCX_PREC_FP32 && p->lg > 20) {
+        const FourStep* fb = find_mfma_big(p->lg, p->shape, p->precision);
+        if (!fb)
+            return fail(PCX_ERR_UNSUPPORTED, "no split-precision kernels for this four-step shape");
+        colf = fb->cols[inverse ? 0 : 1];
+        rowf = fb->rows[(inverse ? 3 : 0) + p->order];
+    } else if (p->precision != PCX_PREC_FP32) {
         const FourStepMfma* fm = find_mfma(p->lg, p->shape);
         if (!fm)
             return fail(PCX_ERR_UNSUPPORTED, "no split-precision kernels for this four-step shape");
