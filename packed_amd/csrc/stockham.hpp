@@ -315,30 +315,53 @@ __device__ __forceinline__ void stockham(cx<T>* v, T* lre, T* lim, int jt0, cons
             else
                 stage_read<LA, REV, s, 2, PX>(v, lre, lim, jt, addr);
         }
-        // twiddle and butterfly one radix-R group at a time, so only one group's twiddles are live
-#pragma unroll
-        for (int m = 0; m < NB; ++m) {
-            if constexpr (Ns > 1) {
-                constexpr int shift = LA - ilog2c(Ns * R);
-                uint32_t      k     = (uint32_t)(jt + m * TA) & (Ns - 1);
-                if constexpr ((PX & kTwChain) != 0) {
-                    typename TwTable<LA, T>::template Seq<shift> tw(ltw, k);
-#pragma unroll
-                    for (int r = 1; r < R; ++r)
-                        v[m * R + r] = cmul(v[m * R + r], tw.next());
-                } else {
-#pragma unroll
-                    for (int r = 1; r < R; ++r)
-                        v[m * R + r] =
-                            cmul(v[m * R + r], TwTable<LA, T>::template get_sh<shift>(ltw, (uint32_t)(r * k) << shift));
-                }
-            }
+        auto butterfly = [&](int m) {
             if constexpr (R == 16 && PREC != PREC_FP32)
                 dft16_mfma<PREC>(mf, v + m * R);
             else
                 dft<R>(v + m * R);
             if constexpr (NB > 1)
                 __builtin_amdgcn_sched_barrier(0);
+        };
+        // Last stage with several butterflies per thread (j = jt + m*TA < Ns, Ns*R = A) under
+        // kTwChain: W_A^(r*j) = W_A^(r*jt) * W_(A/TA)^(r*m), so the twiddles of m = 0 come from the
+        // table and the others from them by compile-time rotations (twc).
+        constexpr bool ROT = (PX & kTwChain) != 0 && NB > 1 && Ns > 1 && Ns * R == P::A && PPT <= 32;
+        if constexpr (ROT) {
+            cx<T> bw[R];
+            typename TwTable<LA, T>::template Seq<0> tw(ltw, (uint32_t)jt);
+#pragma unroll
+            for (int r = 1; r < R; ++r)
+                bw[r] = tw.next();
+            sfor<0, NB>([&](auto mm) {
+                constexpr int m = decltype(mm)::value;
+                sfor<1, R>([&](auto rr) {
+                    constexpr int r = decltype(rr)::value;
+                    v[m * R + r]    = cmul(v[m * R + r], twc<P::A / TA, r * m>(bw[r]));
+                });
+                butterfly(m);
+            });
+        } else {
+            // twiddle and butterfly one radix-R group at a time, so only one group's twiddles are live
+#pragma unroll
+            for (int m = 0; m < NB; ++m) {
+                if constexpr (Ns > 1) {
+                    constexpr int shift = LA - ilog2c(Ns * R);
+                    uint32_t      k     = (uint32_t)(jt + m * TA) & (Ns - 1);
+                    if constexpr ((PX & kTwChain) != 0) {
+                        typename TwTable<LA, T>::template Seq<shift> tw(ltw, k);
+#pragma unroll
+                        for (int r = 1; r < R; ++r)
+                            v[m * R + r] = cmul(v[m * R + r], tw.next());
+                    } else {
+#pragma unroll
+                        for (int r = 1; r < R; ++r)
+                            v[m * R + r] = cmul(v[m * R + r],
+                                                TwTable<LA, T>::template get_sh<shift>(ltw, (uint32_t)(r * k) << shift));
+                    }
+                }
+                butterfly(m);
+            }
         }
         if constexpr (RX && s + 2 == P::NSTAGES) {
             lane_exchange<XL0, ilog2c(RLX)>(v); // into the last stage's layout
