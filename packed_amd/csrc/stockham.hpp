@@ -341,6 +341,21 @@ __device__ __forceinline__ void stockham(cx<T>* v, T* lre, T* lim, int jt0, cons
                 });
                 butterfly(m);
             });
+        } else if constexpr ((PX & kTwChain) != 0 && NB > 1 && Ns > 1 && Ns <= TA) {
+            // every butterfly of the thread has the same k = jt mod Ns: one twiddle sequence,
+            // each twiddle applied to all of them while it is live
+            constexpr int shift = LA - ilog2c(Ns * R);
+            typename TwTable<LA, T>::template Seq<shift> tw(ltw, (uint32_t)jt & (Ns - 1));
+#pragma unroll
+            for (int r = 1; r < R; ++r) {
+                const cx<T> t = tw.next();
+#pragma unroll
+                for (int m = 0; m < NB; ++m)
+                    v[m * R + r] = cmul(v[m * R + r], t);
+            }
+#pragma unroll
+            for (int m = 0; m < NB; ++m)
+                butterfly(m);
         } else {
             // twiddle and butterfly one radix-R group at a time, so only one group's twiddles are live
 #pragma unroll
