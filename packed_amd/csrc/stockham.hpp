@@ -326,7 +326,8 @@ __device__ __forceinline__ void stockham(cx<T>* v, T* lre, T* lim, int jt0, cons
         // Last stage with several butterflies per thread (j = jt + m*TA < Ns, Ns*R = A) under
         // kTwChain: W_A^(r*j) = W_A^(r*jt) * W_(A/TA)^(r*m), so the twiddles of m = 0 come from the
         // table and the others from them by compile-time rotations (twc).
-        constexpr bool ROT = (PX & kTwChain) != 0 && NB > 1 && Ns > 1 && Ns * R == P::A && PPT <= 32;
+        constexpr bool ROT = ((PX & kTwChain) != 0 || !TwTable<LA, T>::direct) && NB > 1 && Ns > 1 &&
+                             Ns * R == P::A && PPT <= 32;
         if constexpr (ROT) {
             cx<T> bw[R];
             typename TwTable<LA, T>::template Seq<0> tw(ltw, (uint32_t)jt);
