@@ -233,3 +233,31 @@ def test_workspace_too_small_is_refused(orc, pa):
     pa.fxcorr_unit(g, 1024, precision="bf16x3")  # split precision covers fft_size 16 .. 16384
     with pytest.raises(PcxError):
         pa.fxcorr_unit(g, 1 << 15, precision="bf16x3")
+
+
+@pytest.mark.parametrize("pack", [1, 64, 1024])
+@pytest.mark.parametrize("inplace", [True, False])
+def test_2p14_blocks_every_pack(orc, pa, pack, inplace):
+    """16384-point blocks (the kernel whose exchange before the radix-4 stage runs in registers)
+    at packs 1, 64 and 1024 (the step, floor((N - M + 1) / P) * P, stays a whole number of packs:
+    fxcorr.hpp:62-69), three channels, in place (saved overlaps, per-element loads) and out of
+    place."""
+    import torch
+
+    L, M, N, C = 90000, 5000, 16384, 3
+    dist = orc.real_size(L, pack) + 2 * pack
+    g = orc.gen_normal(0x5EED0940, M)
+    fs = [orc.gen_normal(0x5EED0941 + c, L, np.sqrt(0.5)).astype(np.complex128) for c in range(C)]
+    buf = np.zeros(dist * C, np.float32)
+    for c in range(C):
+        pb = orc.to_packed(fs[c].astype(np.complex64), pack)
+        buf[c * dist:c * dist + len(pb)] = pb
+    src = torch.from_numpy(buf).cuda()
+    dst = src if inplace else torch.zeros_like(src)
+    xc = pa.fxcorr_unit(pa.vector.from_complex(g, pack), N)
+    xc.batch(dst.data_ptr(), src.data_ptr(), L, C, pack, dist)
+    torch.cuda.synchronize()
+    out = dst.cpu().numpy()
+    for c in range(C):
+        got = orc.from_packed(out[c * dist:], L, pack)
+        assert orc.rel_l2(got, orc.xcorr_naive(fs[c], g.astype(np.complex128))) < 4e-6, (pack, c)
