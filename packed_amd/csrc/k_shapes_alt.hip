@@ -12,7 +12,9 @@ constexpr FourStep kAlt[] = {
     // N = 2^15, 2^19 (the second: 2^19's former default, 16-column tiles)
     shape<15, 8, 7, 32, 64>(), shape<15, 8, 7, 32, 128>(), shape<19, 10, 9, 16, 32>(), shape<19, 9, 10, 32, 16>(),
     // the largest sizes
-    shape<23, 13, 10, 2, 16>(), shape<24, 14, 10, 1, 16>(), shape<24, 13, 11, 2, 8>(),
+    shape<21, 12, 9, 4, 32>(), shape<21, 13, 8, 2, 64>(), shape<21, 14, 7, 1, 128>(), shape<22, 13, 9, 2, 32>(),
+    shape<22, 14, 8, 1, 64>(), shape<23, 13, 10, 2, 16>(), shape<23, 14, 9, 1, 32>(), shape<24, 14, 10, 1, 16>(),
+    shape<24, 13, 11, 2, 8>(),
 };
 } // namespace
 ShapeList<float> shapes_alt() { return {kAlt, (int)(sizeof(kAlt) / sizeof(kAlt[0]))}; }

@@ -115,13 +115,15 @@ const FourStep* find_shape(int lg, int n2, int g, int c) {
         return f;
     return find_in(shapes_alt(), lg, n2, g, c);
 }
-// The default shape of a plan: the first listed one of its size, except where the permuted
-// orders (which store contiguous blocks from the row pass) measured faster with another.
-// 2^24 bit_reversed / unordered, forward / inverse per 2 GiB: one row per workgroup 2.75 / 2.99
-// ms against 2.93 / 3.29 with two (the natural order's four-step needs the two).
+// The default shape of a plan: the first listed one of its size, except for the permuted orders
+// of 2^21 .. 2^24.  Their row pass stores contiguous blocks, so rows of 2^14 points, one per
+// workgroup, with the widest column tiles that leaves win (per 2 GiB, forward / inverse, against
+// the first listed shape, whose wide row groups serve the natural order's strided store):
+// 2^21 2.09 / 2.12 ms (2.33 / 2.25), 2^22 2.05 / 2.19 (2.58 / 2.51), 2^23 2.29 / 2.46
+// (3.14 / 3.21), 2^24 2.75 / 2.99 (2.93 / 3.29).
 const FourStep* default_shape(int lg, int order) {
-    if (order != PCX_ORDER_NORMAL && lg == 24)
-        if (const FourStep* f = find_shape(24, 10, 1, 16))
+    if (order != PCX_ORDER_NORMAL && lg >= 21 && lg <= 24)
+        if (const FourStep* f = find_shape(lg, lg - 14, 1, 16384 >> (lg - 14)))
             return f;
     return find_shape(lg, 0, 0, 0);
 }
@@ -413,6 +415,10 @@ extern "C" int pcx_plan_element_size(pcx_plan_t p) { return p ? (p->f64 ? 8 : 4)
 
 extern "C" int64_t pcx_plan_size(pcx_plan_t p) { return p ? p->n : 0; }
 
+namespace {
+int plan_apply_shape(pcx_plan_t p, const FourStep* sh);
+}
+
 extern "C" int pcx_plan_set_precision(pcx_plan_t p, int precision) {
     if (!p)
         return fail(PCX_ERR_INVALID_ARGUMENT, "plan is null");
@@ -424,6 +430,14 @@ extern "C" int pcx_plan_set_precision(pcx_plan_t p, int precision) {
     }
     if (precision != PCX_PREC_BF16X3 && precision != PCX_PREC_FP16X2)
         return fail(PCX_ERR_UNSUPPORTED, "precision " + s(precision) + " is not implemented");
+    // above 2^20 the split kernels follow the first listed shape's split: a plan on another one
+    // (the permuted orders' default) moves to it
+    if (p->lg > 20 && p->lg <= 24 && !find_mfma_big(p->lg, p->shape, precision)) {
+        const FourStep* d = find_shape(p->lg, 0, 0, 0);
+        if (d && find_mfma_big(p->lg, d, precision))
+            if (int e = plan_apply_shape(p, d))
+                return e;
+    }
     const bool covered = p->lg >= 4 && (p->lg <= 20 ? (!p->two_pass || find_mfma(p->lg, p->shape))
                                                      : find_mfma_big(p->lg, p->shape, precision) != nullptr);
     if (!covered)
