@@ -21,7 +21,9 @@ constexpr FourStepT<double> kAlt64[] = {
     shape64<14, 7, 7, 32, 32>(), shape64<15, 8, 7, 16, 32>(), shape64<16, 8, 8, 16, 16>(),
     shape64<17, 8, 9, 16, 8>(),   // 1.88
     shape64<18, 10, 8, 8, 16>(),  // 2.02
-    shape64<19, 10, 9, 8, 8>()};  // 2.44
+    shape64<19, 10, 9, 8, 8>(),   // 2.44
+    // 2^13-point rows, one per workgroup, and the widest column tiles (bit_reversed candidates)
+    shape64<21, 13, 8, 1, 32>(), shape64<22, 13, 9, 1, 16>(), shape64<23, 13, 10, 1, 8>()};
 } // namespace
 ShapeList<double> shapes64_alt() { return {kAlt64, (int)(sizeof(kAlt64) / sizeof(kAlt64[0]))}; }
 } // namespace pcxd

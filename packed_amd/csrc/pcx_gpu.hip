@@ -132,6 +132,15 @@ const FourStepT<double>* find_shape64(int lg, int n2, int g, int c) {
         return f;
     return find_in(shapes64_alt(), lg, n2, g, c);
 }
+// The same for FP64: bit_reversed 2^21 .. 2^23 take rows of 2^13 points, one per workgroup, and
+// the widest column tiles (per 2 GiB, forward / inverse: 2.02 / 1.92, 2.06 / 2.14, 2.53 / 2.61 ms
+// against 2.41 / 2.47, 2.53 / 2.64, 4.23 / 4.44 with the first listed shapes).
+const FourStepT<double>* default_shape64(int lg, int order) {
+    if (order != PCX_ORDER_NORMAL && lg >= 21 && lg <= 23)
+        if (const FourStepT<double>* f = find_shape64(lg, lg - 13, 1, 8192 >> (lg - 13)))
+            return f;
+    return find_shape64(lg, 0, 0, 0);
+}
 // The split kernels for the plan's shape: same size and the same N1 x N2 split (the plan's
 // scratch layout and row / column counts follow its shape); only the default shapes have them.
 const FourStepMfma* find_mfma(int lg, const FourStep* sh) {
@@ -374,7 +383,7 @@ extern "C" int pcx_plan_create_f64(int64_t fft_size, int64_t sub_size, int order
             he = gen_twiddles(p->tw64_row, fft_size, 1, fft_size, st);
     } else {
         p->two_pass = true;
-        p->shape64  = find_shape64(lg, 0, 0, 0);
+        p->shape64  = default_shape64(lg, order);
         p->n1_lg    = p->shape64->n1;
         p->n2_lg    = p->shape64->n2;
         p->tw_ls    = (lg + 1) / 2;
