@@ -9,6 +9,8 @@ constexpr FourStep kAlt[] = {
     shape<16, 8, 8, 32, 64>(), shape<16, 8, 8, 16, 64>(), shape<16, 8, 8, 64, 64>(), shape<16, 8, 8, 32, 32>(),
     shape<16, 8, 8, 16, 32>(), shape<16, 10, 6, 16, 64>(), shape<16, 7, 9, 64, 32>(), shape<16, 6, 10, 64, 16>(),
     shape<16, 9, 7, 32, 32>(), shape<16, 10, 6, 16, 32>(), shape<16, 9, 7, 32, 64>(),
+    // 2^20 permuted orders (the default there): 4096-point rows, 64-column tiles
+    shape<20, 12, 8, 4, 64>(),
     // N = 2^15, 2^19 (the second: 2^19's former default, 16-column tiles)
     shape<15, 8, 7, 32, 64>(), shape<15, 8, 7, 32, 128>(), shape<19, 10, 9, 16, 32>(), shape<19, 9, 10, 32, 16>(),
     // the largest sizes

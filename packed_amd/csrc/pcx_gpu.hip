@@ -120,10 +120,15 @@ const FourStep* find_shape(int lg, int n2, int g, int c) {
 // workgroup, with the widest column tiles that leaves win (per 2 GiB, forward / inverse, against
 // the first listed shape, whose wide row groups serve the natural order's strided store):
 // 2^21 2.09 / 2.12 ms (2.33 / 2.25), 2^22 2.05 / 2.19 (2.58 / 2.51), 2^23 2.29 / 2.46
-// (3.14 / 3.21), 2^24 2.75 / 2.99 (2.93 / 3.29).
+// (3.14 / 3.21), 2^24 2.75 / 2.99 (2.93 / 3.29).  At 2^20, 4096-point rows (four per workgroup)
+// and 64-column tiles: 1.97 / 1.93 ms against 2.12 / 2.06 (one 16384-point row per workgroup
+// made the inverse slower there, 2.66 ms).
 const FourStep* default_shape(int lg, int order) {
     if (order != PCX_ORDER_NORMAL && lg >= 21 && lg <= 24)
         if (const FourStep* f = find_shape(lg, lg - 14, 1, 16384 >> (lg - 14)))
+            return f;
+    if (order != PCX_ORDER_NORMAL && lg == 20)
+        if (const FourStep* f = find_shape(20, 8, 4, 64))
             return f;
     return find_shape(lg, 0, 0, 0);
 }
@@ -439,11 +444,15 @@ extern "C" int pcx_plan_set_precision(pcx_plan_t p, int precision) {
     }
     if (precision != PCX_PREC_BF16X3 && precision != PCX_PREC_FP16X2)
         return fail(PCX_ERR_UNSUPPORTED, "precision " + s(precision) + " is not implemented");
-    // above 2^20 the split kernels follow the first listed shape's split: a plan on another one
-    // (the permuted orders' default) moves to it
-    if (p->lg > 20 && p->lg <= 24 && !find_mfma_big(p->lg, p->shape, precision)) {
+    // the split kernels of four-step sizes follow the first listed shape's split: a plan on
+    // another one (the permuted orders' default at 2^20 .. 2^24) moves to it
+    if (p->two_pass && p->lg <= 24) {
         const FourStep* d = find_shape(p->lg, 0, 0, 0);
-        if (d && find_mfma_big(p->lg, d, precision))
+        const bool has = p->lg <= 20 ? find_mfma(p->lg, p->shape) != nullptr
+                                     : find_mfma_big(p->lg, p->shape, precision) != nullptr;
+        const bool d_has = d && (p->lg <= 20 ? find_mfma(p->lg, d) != nullptr
+                                             : find_mfma_big(p->lg, d, precision) != nullptr);
+        if (!has && d_has)
             if (int e = plan_apply_shape(p, d))
                 return e;
     }
