@@ -261,6 +261,42 @@ struct StageMap {
     const Addr& addr;
 };
 
+// Twiddles W^(r e), r = 1 .. 15, of NB radix-16 butterflies at v, v + 16, ... (the same e) from a
+// two-level table: W^e, W^2e, W^4e, W^8e by lookup (a product each), the other eleven as
+// products of those (at most three deep), so four lookups instead of fifteen.  e < A / 16.
+template<int LA, int NB, class T>
+__device__ __forceinline__ void twiddle16_pow(cx<T>* v, const T* ltw, uint32_t e) {
+    using TW       = TwTable<LA, T>;
+    const cx<T> w1 = TW::get(ltw, e);
+    const cx<T> w2 = TW::get(ltw, e << 1);
+    const cx<T> w4 = TW::get(ltw, e << 2);
+    const cx<T> w8 = TW::get(ltw, e << 3);
+    auto        tw = [&](int r, cx<T> w) {
+#pragma unroll
+        for (int m = 0; m < NB; ++m)
+            v[m * 16 + r] = cmul(v[m * 16 + r], w);
+    };
+    const cx<T> w6  = cmul(w4, w2);
+    const cx<T> w10 = cmul(w8, w2);
+    const cx<T> w12 = cmul(w8, w4);
+    const cx<T> w14 = cmul(w12, w2);
+    tw(1, w1);
+    tw(2, w2);
+    tw(3, cmul(w2, w1));
+    tw(4, w4);
+    tw(5, cmul(w4, w1));
+    tw(6, w6);
+    tw(7, cmul(w6, w1));
+    tw(8, w8);
+    tw(9, cmul(w8, w1));
+    tw(10, w10);
+    tw(11, cmul(w10, w1));
+    tw(12, w12);
+    tw(13, cmul(w12, w1));
+    tw(14, w14);
+    tw(15, cmul(w14, w1));
+}
+
 // PREC_BF16X3 / PREC_FP16X2 run the radix-16 butterflies on the matrix cores (mf holds the
 // operands).
 // With Plan::HALF, lre and lim are the same array and each exchange runs in two halves.
@@ -328,6 +364,9 @@ __device__ __forceinline__ void stockham(cx<T>* v, T* lre, T* lim, int jt0, cons
         // table and the others from them by compile-time rotations (twc).
         constexpr bool ROT = ((PX & kTwChain) != 0 || !TwTable<LA, T>::direct) && NB > 1 && Ns > 1 &&
                              Ns * R == P::A && PPT <= 32;
+        // radix 16 on a two-level table below its high factor's granularity: twiddles by powers
+        constexpr bool POW = R == 16 && Ns > 1 && !TwTable<LA, T>::direct &&
+                             LA - ilog2c(Ns * R) < TwTable<LA, T>::LOB;
         if constexpr (ROT) {
             cx<T> bw[R];
             typename TwTable<LA, T>::template Seq<0> tw(ltw, (uint32_t)jt);
@@ -342,6 +381,12 @@ __device__ __forceinline__ void stockham(cx<T>* v, T* lre, T* lim, int jt0, cons
                 });
                 butterfly(m);
             });
+        } else if constexpr (POW && (NB == 1 || Ns <= TA)) {
+            // every butterfly of the thread has the same k = jt mod Ns
+            twiddle16_pow<LA, NB>(v, ltw, ((uint32_t)jt & (Ns - 1)) << (LA - ilog2c(Ns * R)));
+#pragma unroll
+            for (int m = 0; m < NB; ++m)
+                butterfly(m);
         } else if constexpr ((PX & kTwChain) != 0 && NB > 1 && Ns > 1 && Ns <= TA) {
             // every butterfly of the thread has the same k = jt mod Ns: one twiddle sequence,
             // each twiddle applied to all of them while it is live
@@ -364,7 +409,9 @@ __device__ __forceinline__ void stockham(cx<T>* v, T* lre, T* lim, int jt0, cons
                 if constexpr (Ns > 1) {
                     constexpr int shift = LA - ilog2c(Ns * R);
                     uint32_t      k     = (uint32_t)(jt + m * TA) & (Ns - 1);
-                    if constexpr ((PX & kTwChain) != 0) {
+                    if constexpr (POW) {
+                        twiddle16_pow<LA, 1>(v + m * R, ltw, k << shift);
+                    } else if constexpr ((PX & kTwChain) != 0) {
                         typename TwTable<LA, T>::template Seq<shift> tw(ltw, k);
 #pragma unroll
                         for (int r = 1; r < R; ++r)
