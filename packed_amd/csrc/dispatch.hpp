@@ -82,7 +82,14 @@ void launch_xc(const XcArgs& a, hipStream_t st) {
     for (s.blk0 = a.blk0; s.blk0 < end; s.blk0 += per) {
         s.blk1            = std::min(end, s.blk0 + per);
         const int64_t nwg = grid8((s.blk1 - s.blk0 + G - 1) / G);
-        if (small)
+        if (a.order) // a fresh ticket counter and read flags for this launch (a.order holds nwg + 1 words)
+            hipLaunchKernelGGL(k_xc_order_clear, dim3((unsigned)std::min<int64_t>((nwg + 256) / 256, 1024)), dim3(256),
+                               0, st, a.order, nwg + 1);
+        if (a.order && small)
+            hipLaunchKernelGGL((k_fxcorr<LA, G, true, PREC, true>), dim3((unsigned)nwg), dim3(NT), 0, st, s);
+        else if (a.order)
+            hipLaunchKernelGGL((k_fxcorr<LA, G, false, PREC, true>), dim3((unsigned)nwg), dim3(NT), 0, st, s);
+        else if (small)
             hipLaunchKernelGGL((k_fxcorr<LA, G, true, PREC>), dim3((unsigned)nwg), dim3(NT), 0, st, s);
         else
             hipLaunchKernelGGL((k_fxcorr<LA, G, false, PREC>), dim3((unsigned)nwg), dim3(NT), 0, st, s);

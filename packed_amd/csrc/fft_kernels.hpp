@@ -72,12 +72,13 @@ using ColArgs = ColArgsT<float>;
 // lines when a tile covers 64 bytes of a row, or input samples in overlap-save -- meet in one L2
 // instead of being fetched from HBM by two XCDs.  (Placement only affects speed.)  Grids that are
 // not a multiple of 8 keep the plain order.
-__device__ __forceinline__ int64_t xcd_item(int64_t round_base) {
+__device__ __forceinline__ int64_t xcd_map(uint32_t i) {
     const uint32_t g = gridDim.x;
     if (g & 7u)
-        return round_base + blockIdx.x;
-    return round_base + (int64_t)(blockIdx.x & 7u) * (g >> 3) + (blockIdx.x >> 3);
+        return i;
+    return (int64_t)(i & 7u) * (g >> 3) + (i >> 3);
 }
+__device__ __forceinline__ int64_t xcd_item(int64_t round_base) { return round_base + xcd_map(blockIdx.x); }
 
 // Base of transform b: b * dist elements, or offs[b] when a per-transform offset table is given.
 template<class T>

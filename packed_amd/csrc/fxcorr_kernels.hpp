@@ -43,7 +43,17 @@ struct XcArgs {
     int64_t      blk0;               // first block of this launch (a launch covers < 2^32 work-items)
     int64_t      blk1;               // end of this launch's blocks
     int64_t      hblk0;              // block whose saved overlap is first in hre / him
+    // In place without a saved overlap (N - step <= step): order[0] hands out workgroup tickets,
+    // order[1 + w] is set once workgroup w has read its samples; a workgroup stores only after its
+    // predecessor's flag (zeroed by k_xc_order_clear before each launch), or null.
+    uint32_t*    order;
 };
+
+// Clears a launch's ticket counter and read flags (a kernel, so that graph replays clear too).
+static __global__ __launch_bounds__(256) void k_xc_order_clear(uint32_t* o, int64_t words) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += (int64_t)gridDim.x * blockDim.x)
+        o[i] = 0u;
+}
 
 // Plan options of the correlation kernel.  The two transforms per block are bound by the
 // latency of the LDS exchanges, and more waves per CU hide more of it:
@@ -96,7 +106,9 @@ __device__ __forceinline__ int xl_lds(int e) {
 
 // B32: the signal fits 32-bit buffer offsets (one SGPR descriptor, one VGPR per address).
 // PREC_BF16X3 runs the radix-16 butterflies of both transforms on the matrix cores.
-template<int LA, int G, bool B32, int PREC = PREC_FP32>
+// ORD: in place in ticket order (XcArgs::order), a variant of its own so that the out-of-place
+// kernel's register allocation does not change.
+template<int LA, int G, bool B32, int PREC = PREC_FP32, bool ORD = false>
 __global__ __launch_bounds__(G * xc_threads<LA>(),
                              PREC == PREC_FP32 ? waves_per_simd(G * xc_threads<LA>(), 4 * xc_lds_floats<LA, G>())
                                                : (waves_per_simd(G * xc_threads<LA>(), 4 * xc_lds_floats<LA, G>()) > 4
@@ -119,12 +131,20 @@ void k_fxcorr(XcArgs p) {
     static_assert(XL || PI::first_radix == RL, "fused multiply needs matching register maps");
     static_assert(!XL || (G == 1 && R0 == PPT), "register renaming assumes one block and one first-stage butterfly");
     __shared__ float lds[xc_lds_floats<LA, G>()];
+    __shared__ uint32_t ticket;
     using LC             = LdsCarve<float, RS::DATA, TwTable<LA>::FLOATS, 0>;
     float*           ltw = lds + LC::TOFF;
     float*           lre = lds + LC::DOFF;
     float*           lim = RS::im(lre);
+    if (ORD && threadIdx.x == 0)
+        ticket = __hip_atomic_fetch_add(p.order, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     TwTable<LA>::fill(ltw, p.twA, threadIdx.x, G * TA);
     __syncthreads();
+    // In place, workgroups take their blocks in ticket order (the order in which they started), so
+    // the predecessor a workgroup waits for has started and reads its samples without waiting: the
+    // wait always ends.  Tickets are spread over the XCDs as block indices are (xcd_item), except
+    // that the first workgroup of each XCD's range then waits for the last of the previous range.
+    const int64_t wgi = ORD ? xcd_map(__builtin_amdgcn_readfirstlane(ticket)) : xcd_item(0);
 
     MfmaDft16 mf;
     if constexpr (PREC != PREC_FP32)
@@ -133,7 +153,7 @@ void k_fxcorr(XcArgs p) {
     // a wavefront lies in one block when TA >= 64: keep the block's offsets in SGPRs
     const int     g    = TA >= 64 ? __builtin_amdgcn_readfirstlane(t / TA) : t / TA;
     const int     jt   = XL ? xl_jt(t % TA) : t % TA;
-    const int64_t bg   = p.blk0 + xcd_item(0) * G + g; // block over all channels; neighbours share one L2
+    const int64_t bg   = p.blk0 + wgi * G + g; // block over all channels; neighbours share one L2
     const bool    ok   = bg < p.blk1;
     const int64_t ch   = ok ? bg / p.nblocks : 0;
     const int64_t b    = ok ? bg - ch * p.nblocks : 0;
@@ -189,6 +209,22 @@ void k_fxcorr(XcArgs p) {
     }
     stockham<LA, false, decltype(addr), PREC, float, PX, false, false, decltype(addr), decltype(addr), true,
              XL ? 4 : -1>(v, lre, lim, jt, addr, ltw, &mf);
+    // In place, the read flag goes up as early as here, except at 2^13, where the registers for
+    // it are not there (it then goes up right before the wait).
+    constexpr bool FLAG_EARLY = LA != 13;
+    if constexpr (ORD && FLAG_EARLY) {
+        // Every load of this workgroup has returned: each wave's values went through the LDS
+        // exchange of the first stage, behind a barrier (one-stage plans: the barrier here), so
+        // the successor may overwrite them.  No data passes through the flags, so they are relaxed
+        // device-coherent accesses: an acquire or release at device scope would write back or
+        // invalidate the XCD's whole L2.
+        if constexpr (Plan<LA, false, float, PX>::NSTAGES == 1)
+            __syncthreads();
+        if (threadIdx.x == 0) {
+            __builtin_amdgcn_s_waitcnt(0);
+            __hip_atomic_store(p.order + 1 + wgi, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
     {
         // Y = conj(X) * K.  The spectrum is read in two halves: the second half's offset depends
         // (opaquely) on the first half's products, so at most half of it is in registers beside v.
@@ -226,6 +262,26 @@ void k_fxcorr(XcArgs p) {
         stockham<LA, true, decltype(addr), PREC, float, PX>(v, lre, lim, jt, addr, ltw, &mf);
     }
     const int64_t keep = p.len - off < p.step ? p.len - off : p.step;
+    if constexpr (ORD) {
+        // the first N - step samples of these blocks, which the predecessor reads, are
+        // overwritten only after its flag (bounded wait: 2^22 polls, seconds, far beyond a
+        // launch's run time); the flag is set without waiting, so every wait ends
+        if constexpr (!FLAG_EARLY) {
+            __builtin_amdgcn_s_waitcnt(0); // this wave's loads have returned; the barrier: every wave's
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            if constexpr (!FLAG_EARLY)
+                __hip_atomic_store(p.order + 1 + wgi, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (wgi > 0)
+                for (uint32_t spin = 0; spin < (1u << 22); ++spin) {
+                    if (__hip_atomic_load(p.order + wgi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u)
+                        break;
+                    __builtin_amdgcn_s_sleep(2);
+                }
+        }
+        __syncthreads();
+    }
     if (aligned) {
         const int      jt = opaque_after_memory(XL ? xl_jt(t % TA) : t % TA);
         const BufW<>   dw(dst + poff((uint32_t)off, p.dst_lp), p.dst_lp, ok ? poff((uint32_t)keep, p.dst_lp) : 0u);

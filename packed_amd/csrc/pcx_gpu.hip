@@ -1214,9 +1214,32 @@ int fxcorr_run(pcx_fxcorr_t x, float* dst, int64_t dst_pack, int64_t dst_dist, c
         HIPCHK(hipGetLastError());
         return PCX_SUCCESS;
     }
-    // In place: blocks overwrite the overlap that the next block still reads, so it is saved
-    // first.  With a workspace of W bytes the blocks run in groups of W / (8 * (N - step)), one
-    // group after another (a group only overwrites samples of its own blocks).
+    // In place: a block overwrites samples that its predecessor reads.  When they lie within the
+    // next block only (N - step <= step), workgroups run in ticket order and each stores after its
+    // predecessor has read (XcArgs::order: one word per workgroup and launch); a workspace of W
+    // bytes takes W / 4 - 8 blocks per group of launches.
+    const int64_t owords = ws ? ws->bytes / 4 - 8 : total;
+    if (a.hlen <= step && owords >= 1) {
+        const int64_t group = std::min(total, owords);
+        uint32_t*     obuf  = nullptr;
+        if (ws) {
+            obuf = reinterpret_cast<uint32_t*>(ws->buf);
+        } else {
+            if (int e = xc_own(x, total + 8, st))
+                return e;
+            obuf = reinterpret_cast<uint32_t*>(x->own);
+        }
+        a.order = obuf;
+        for (int64_t g0 = 0; g0 < total; g0 += group) {
+            a.blk0 = g0, a.blk1 = std::min(total, g0 + group);
+            launch(a, st);
+            HIPCHK(hipGetLastError());
+        }
+        return ws ? PCX_SUCCESS : xc_own_done(x, st);
+    }
+    // Otherwise the overlap is saved first.  With a workspace of W bytes the blocks run in groups
+    // of W / (8 * (N - step)), one group after another (a group only overwrites samples of its own
+    // blocks).
     const int64_t group = ws ? std::min(total, ws->bytes / (8 * a.hlen)) : total;
     float*        hbuf  = nullptr;
     if (ws) {

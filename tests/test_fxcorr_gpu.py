@@ -78,7 +78,7 @@ def test_reference_random_phase_case(orc, pa):
 
 
 @pytest.mark.parametrize("L,M,N", [(5000, 100, 256), (4096, 64, 512), (70000, 4096, 16384), (12345, 1, 64),
-                                   (3000, 57, 128), (1 << 16, 1024, 4096)])
+                                   (3000, 57, 128), (1 << 16, 1024, 4096), (5000, 200, 256), (40000, 9000, 16384)])
 @pytest.mark.parametrize("inplace", [True, False])
 def test_multi_block(orc, pa, L, M, N, inplace):
     f = orc.gen_normal(0x5EED0003, L, np.sqrt(0.5)).astype(np.complex128)
@@ -188,7 +188,8 @@ def test_block_path_above_16384(orc, pa, L, M, N, inplace):
     assert orc.rel_l2(got, orc.xcorr_naive(f, g)) < 4e-6
 
 
-@pytest.mark.parametrize("L,M,N,C", [(20000, 300, 1024, 3), (3000, 57, 128, 4), (150000, 20000, 1 << 15, 2)])
+@pytest.mark.parametrize("L,M,N,C", [(20000, 300, 1024, 3), (3000, 57, 128, 4), (150000, 20000, 1 << 15, 2),
+                                     (100000, 10, 64, 2), (30000, 150, 256, 3)])
 @pytest.mark.parametrize("inplace", [True, False])
 def test_workspace_matches_own_buffers(orc, pa, L, M, N, C, inplace):
     """pcx_fxcorr_exec_batch_ws (the TmpFactory_ slot, fxcorr.hpp:10-32): with a workspace of one
