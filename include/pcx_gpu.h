@@ -190,8 +190,10 @@ int pcx_fxcorr_set_precision(pcx_fxcorr_t xc, int precision);
 /* The reference's TmpFactory_ slot (fxcorr.hpp:10-32, 47, 82-91: scratch supplied by the caller,
  * one fft_size vector).  pcx_fxcorr_exec_batch_ws is pcx_fxcorr_exec_batch with the unit's
  * scratch taken from `workspace` (device memory of workspace_bytes >= pcx_fxcorr_workspace_min,
- * which is 8 * fft_size bytes): in place, the overlap of up to workspace_bytes / (8 * (fft_size -
- * step)) blocks is saved there at a time; above fft_size 16384 it holds the blocks in flight
+ * which is 8 * fft_size bytes): in place, it holds the block order's ticket counter and read
+ * flags (4 bytes per block, up to workspace_bytes / 4 - 8 blocks per group of launches) when the
+ * overlap fft_size - step is at most step, else the saved overlap of up to workspace_bytes /
+ * (8 * (fft_size - step)) blocks at a time; above fft_size 16384 it holds the blocks in flight
  * (the four-step FFT keeps its intermediate in its plan).  The unit allocates no buffer of its
  * own.  Calls that share one workspace must not overlap. */
 int64_t pcx_fxcorr_workspace_min(pcx_fxcorr_t xc);
