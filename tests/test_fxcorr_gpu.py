@@ -262,3 +262,46 @@ def test_2p14_blocks_every_pack(orc, pa, pack, inplace):
     for c in range(C):
         got = orc.from_packed(out[c * dist:], L, pack)
         assert orc.rel_l2(got, orc.xcorr_naive(fs[c], g.astype(np.complex128))) < 4e-6, (pack, c)
+
+
+@pytest.mark.parametrize("N,M", [(16384, 4096), (8192, 1024), (256, 30)])
+def test_in_place_block_order_repeats_and_replays(orc, pa, N, M):
+    """In place with N - step <= step the blocks run in ticket order and the unit's order words
+    are left zero by every launch: calls of growing and shrinking length on one unit, then a
+    captured graph replayed three times, each equal bit for bit to the out-of-place result."""
+    import torch
+
+    pack = 8
+    g = np.exp(1j * np.pi * np.real(orc.gen_uniform(0x5EED0071, M))).astype(np.complex64)
+    xc = pa.fxcorr_unit(pa.vector.from_complex(g, pack), N)
+    step = xc.step(pack)
+    assert N - step <= step
+    for L in (40 * step + 17, 7 * step, 90 * step + 3, 3 * step + 1):
+        f = orc.gen_normal(0x5EED0072 + L, L, np.sqrt(0.5)).astype(np.complex64)
+        src = pa.vector.from_complex(f, pack)
+        oop = pa.vector(L, pack)
+        xc(src, oop)
+        xc(src)
+        torch.cuda.synchronize()
+        assert torch.equal(src.data, oop.data), L
+    L = 60 * step + 5
+    f = orc.gen_normal(0x5EED0073, L, np.sqrt(0.5)).astype(np.complex64)
+    x0 = pa.vector.from_complex(f, pack)
+    want = pa.vector(L, pack)
+    xc(x0, want)
+    v = pa.vector.from_complex(f, pack)
+    torch.cuda.synchronize()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.stream(s):
+        graph.capture_begin()
+        xc(v, stream=s)
+        graph.capture_end()
+    torch.cuda.synchronize()
+    for _ in range(3):
+        v.data.copy_(x0.data)
+        torch.cuda.synchronize()
+        graph.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(v.data, want.data)
